@@ -1,0 +1,270 @@
+// hc_api.hip -- C ABI entry points of libhipclaw (declared in include/hipclaw.h)
+// and the small batched kernels that sit around the patch step: swirl
+// velocities, physical boundary conditions and the level CFL reduction.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hc_internal.hpp"
+
+namespace hc {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return HIPCLAW_OK;
+    return set_error(HIPCLAW_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int validate_physics(const hipclaw_physics* p)
+{
+    if (!p) return set_error(HIPCLAW_ERR_ARG, "physics is NULL");
+    if (p->mbc != 2) return set_error(HIPCLAW_ERR_UNSUPPORTED, "mbc must be 2");
+    if (p->mx < 4 || p->my < 4) return set_error(HIPCLAW_ERR_ARG, "mx, my must be >= 4");
+    if (p->precision != HIPCLAW_FP64 && p->precision != HIPCLAW_FP32)
+        return set_error(HIPCLAW_ERR_ARG, "precision must be HIPCLAW_FP64 or HIPCLAW_FP32");
+    if (p->order[0] < 1 || p->order[0] > 2 || p->order[1] < 0 || p->order[1] > 2)
+        return set_error(HIPCLAW_ERR_ARG, "order must be (1|2, 0|1|2)");
+    switch (p->solver) {
+    case HIPCLAW_SOLVER_ADV:
+        if (p->meqn != 1 || p->mwaves != 1 || p->maux != 2)
+            return set_error(HIPCLAW_ERR_ARG, "advection needs meqn=mwaves=1, maux=2");
+        break;
+    case HIPCLAW_SOLVER_SWE_ROE:
+        if (p->meqn != 3 || p->mwaves != 3 || p->maux != 0)
+            return set_error(HIPCLAW_ERR_ARG, "shallow water needs meqn=mwaves=3, maux=0");
+        if (!(p->grav > 0)) return set_error(HIPCLAW_ERR_ARG, "grav must be > 0");
+        break;
+    case HIPCLAW_SOLVER_EULER5_ROE:
+        if (p->meqn != 5 || p->mwaves != 5 || p->maux != 0)
+            return set_error(HIPCLAW_ERR_ARG, "euler5 needs meqn=mwaves=5, maux=0");
+        if (!(p->gamma > 1)) return set_error(HIPCLAW_ERR_ARG, "gamma must be > 1");
+        break;
+    default:
+        return set_error(HIPCLAW_ERR_ARG, "unknown solver");
+    }
+    for (int w = 0; w < p->mwaves; w++)
+        if (p->mthlim[w] < 0 || p->mthlim[w] > 6)
+            return set_error(HIPCLAW_ERR_ARG, "mthlim must be in 0..6");
+    return HIPCLAW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Swirl edge velocities from the stream function psi = sin^2(pi x) sin^2(pi y)/pi
+// (applications/clawpack/advection/2d/swirl/user_4.6/b4step2.f:25-39).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_swirl_aux(int npatch, int mx, int my, const double* xylower,
+                            const double* dxdy, double vt, R* aux)
+{
+    const int NX = mx + 4, NY = my + 4;
+    const long long T = (long long)NX * NY;
+    long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= T * npatch) return;
+    const int p = (int)(gid / T);
+    const int cell = (int)(gid - (long long)p * T);
+    const int j = cell / NX, i = cell - j * NX;
+    const double pi = 4.0 * atan(1.0);
+    const double dx = dxdy[2 * p], dy = dxdy[2 * p + 1];
+    const double xll = xylower[2 * p] + (double)(i - 2) * dx;
+    const double yll = xylower[2 * p + 1] + (double)(j - 2) * dy;
+    auto psi = [pi](double x, double y) {
+        double sx = sin(pi * x), sy = sin(pi * y);
+        return ((sx * sx) * (sy * sy)) / pi;
+    };
+    const double pij = psi(xll, yll);
+    const double u = (vt / dy) * (psi(xll, yll + dy) - pij);
+    const double v = -(vt / dx) * (psi(xll + dx, yll) - pij);
+    aux[p * 2 * T + cell] = R(u);
+    aux[p * 2 * T + T + cell] = R(v);
+}
+
+// ---------------------------------------------------------------------------
+// clawpack46_bc2_default: left/right strips over the full height first, then
+// bottom/top over the full width (so corners come from the second pass).
+// One 64-lane group per patch.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_bc2(int npatch, int mx, int my, int meqn, R* q, const int* ids,
+                      const int* mthbc)
+{
+    const int p = blockIdx.x;
+    if (p >= npatch) return;
+    const int tile = ids ? ids[p] : p;
+    const int NX = mx + 4, NY = my + 4;
+    R* qp = q + (long long)tile * NX * NY * meqn;
+    const int* bc = mthbc + 4 * p;
+    const int mbc = 2;
+    const int i1 = mbc, imx = mx + mbc - 1, j1 = mbc, jmy = my + mbc - 1;
+#define QB(i, j, m) qp[((long long)(m) * NY + (j)) * NX + (i)]
+    for (int t = threadIdx.x; t < NY * meqn; t += blockDim.x) {
+        const int m = t / NY, j = t - m * NY;
+        for (int ibc = 1; ibc <= mbc; ibc++) {
+            if (bc[0] == 1) QB(i1 - ibc, j, m) = QB(i1, j, m);
+            else if (bc[0] == 3) QB(i1 - ibc, j, m) = (m == 1) ? -QB(i1 + ibc - 1, j, m) : QB(i1 + ibc - 1, j, m);
+            if (bc[1] == 1) QB(imx + ibc, j, m) = QB(imx, j, m);
+            else if (bc[1] == 3) QB(imx + ibc, j, m) = (m == 1) ? -QB(imx + 1 - ibc, j, m) : QB(imx + 1 - ibc, j, m);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NX * meqn; t += blockDim.x) {
+        const int m = t / NX, i = t - m * NX;
+        for (int jbc = 1; jbc <= mbc; jbc++) {
+            if (bc[2] == 1) QB(i, j1 - jbc, m) = QB(i, j1, m);
+            else if (bc[2] == 3) QB(i, j1 - jbc, m) = (m == 2) ? -QB(i, j1 + jbc - 1, m) : QB(i, j1 + jbc - 1, m);
+            if (bc[3] == 1) QB(i, jmy + jbc, m) = QB(i, jmy, m);
+            else if (bc[3] == 3) QB(i, jmy + jbc, m) = (m == 2) ? -QB(i, jmy + 1 - jbc, m) : QB(i, jmy + 1 - jbc, m);
+        }
+    }
+#undef QB
+}
+
+// Maximum of n non-negative doubles (one workgroup; n is a patch count).
+__global__ void k_max_reduce(const double* x, int n, double* out)
+{
+    __shared__ double s[1024];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v = x[i] > v ? x[i] : v;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x + w] > s[threadIdx.x] ? s[threadIdx.x + w] : s[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+}  // namespace hc
+
+using namespace hc;
+
+extern "C" {
+
+int hipclaw_abi_version(void) { return HIPCLAW_ABI_VERSION; }
+
+const char* hipclaw_last_error(void) { return g_last_error.c_str(); }
+
+int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
+{
+    if (!out) return set_error(HIPCLAW_ERR_ARG, "out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    out->solver = solver;
+    out->mx = mx;
+    out->my = my;
+    out->mbc = 2;
+    out->order[0] = 2;
+    out->order[1] = 2;
+    out->efix = 1;
+    out->precision = HIPCLAW_FP64;
+    out->grav = 1.0;
+    out->gamma = 1.4;
+    for (int f = 0; f < 4; f++) out->mthbc[f] = 1;
+    switch (solver) {
+    case HIPCLAW_SOLVER_ADV: out->meqn = out->mwaves = 1; out->maux = 2; break;
+    case HIPCLAW_SOLVER_SWE_ROE: out->meqn = out->mwaves = 3; break;
+    case HIPCLAW_SOLVER_EULER5_ROE: out->meqn = out->mwaves = 5; break;
+    default: return set_error(HIPCLAW_ERR_ARG, "unknown solver");
+    }
+    for (int w = 0; w < 5; w++) out->mthlim[w] = w < out->mwaves ? 4 : 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
+                        const void* aux, const double* dxdy, const int* patch_ids, double dt,
+                        double* cfl_patch, const hipclaw_launch* launch, void* stream)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (npatch < 0) return set_error(HIPCLAW_ERR_ARG, "npatch < 0");
+    if (npatch == 0) return HIPCLAW_OK;
+    if (!q_in || !q_out || !dxdy) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
+    if (phys->maux > 0 && !aux) return set_error(HIPCLAW_ERR_ARG, "aux is NULL but maux > 0");
+    if (!(dt >= 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be >= 0");
+    int cpl = 1, wpg = 4;
+    if (launch) {
+        cpl = launch->cols_per_lane;
+        wpg = launch->waves_per_group;
+    } else if (phys->precision == HIPCLAW_FP32 && phys->mx + 4 > 32) {
+        cpl = 2;
+    }
+    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4)
+        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
+    StepArgs a{};
+    a.npatch = npatch;
+    a.mx = phys->mx;
+    a.my = phys->my;
+    a.order2 = phys->order[0];
+    a.transverse = phys->order[1];
+    for (int w = 0; w < 5; w++) a.mthlim[w] = w < phys->mwaves ? phys->mthlim[w] : 0;
+    a.pp.grav = phys->grav;
+    a.pp.gamma = phys->gamma;
+    a.pp.efix = phys->efix;
+    a.dt = dt;
+    a.dxdy = dxdy;
+    a.patch_ids = patch_ids;
+    const long long T = (long long)(phys->mx + 4) * (phys->my + 4);
+    a.qstride = T * phys->meqn;
+    a.astride = T * phys->maux;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (phys->precision == HIPCLAW_FP64)
+        return launch_step2_fp64(a, phys->solver, cpl, wpg, q_in, q_out, aux, cfl_patch, s);
+    return launch_step2_fp32(a, phys->solver, cpl, wpg, q_in, q_out, aux, cfl_patch, s);
+}
+
+int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream)
+{
+    if (!x_dev || !out_dev || n < 0) return set_error(HIPCLAW_ERR_ARG, "bad max_reduce args");
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                       x_dev, n, out_dev);
+    return check_hip(hipGetLastError(), "k_max_reduce launch");
+}
+
+int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const double* xylower,
+                            const double* dxdy, double t, double dt, double tperiod, void* aux,
+                            void* stream)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (phys->solver != HIPCLAW_SOLVER_ADV) return set_error(HIPCLAW_ERR_ARG, "swirl aux needs the advection solver");
+    if (npatch <= 0) return HIPCLAW_OK;
+    if (tperiod == 0.0) return HIPCLAW_OK;  // velocities fixed by setaux
+    const double pi2 = 8.0 * atan(1.0);
+    const double vt = cos(pi2 * (t + dt / 2.0) / tperiod);
+    const long long n = (long long)(phys->mx + 4) * (phys->my + 4) * npatch;
+    const int bs = 256;
+    const unsigned nb = (unsigned)((n + bs - 1) / bs);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (phys->precision == HIPCLAW_FP64)
+        hipLaunchKernelGGL(k_swirl_aux<double>, dim3(nb), dim3(bs), 0, s, npatch, phys->mx, phys->my,
+                           xylower, dxdy, vt, static_cast<double*>(aux));
+    else
+        hipLaunchKernelGGL(k_swirl_aux<float>, dim3(nb), dim3(bs), 0, s, npatch, phys->mx, phys->my,
+                           xylower, dxdy, vt, static_cast<float*>(aux));
+    return check_hip(hipGetLastError(), "k_swirl_aux launch");
+}
+
+int hipclaw_bc2_batch(const hipclaw_physics* phys, int npatch, void* q, const int* patch_ids,
+                      const int* mthbc, void* stream)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (npatch <= 0) return HIPCLAW_OK;
+    if (!q || !mthbc) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (phys->precision == HIPCLAW_FP64)
+        hipLaunchKernelGGL(k_bc2<double>, dim3(npatch), dim3(64), 0, s, npatch, phys->mx, phys->my,
+                           phys->meqn, static_cast<double*>(q), patch_ids, mthbc);
+    else
+        hipLaunchKernelGGL(k_bc2<float>, dim3(npatch), dim3(64), 0, s, npatch, phys->mx, phys->my,
+                           phys->meqn, static_cast<float*>(q), patch_ids, mthbc);
+    return check_hip(hipGetLastError(), "k_bc2 launch");
+}
+
+}  // extern "C"

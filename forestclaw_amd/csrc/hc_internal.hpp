@@ -1,0 +1,27 @@
+// hc_internal.hpp -- host-side declarations shared by the hipclaw translation
+// units (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/hipclaw.h"
+#include "hc_step2.hpp"
+
+namespace hc {
+
+// Records a message for hipclaw_last_error() and returns code.
+int set_error(int code, const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+// Kernel launchers, one per precision translation unit.  Return HIPCLAW_*.
+int launch_step2_fp64(const StepArgs& a, int solver, int cpl, int wpg, const void* qin,
+                      void* qout, const void* aux, double* cfl, hipStream_t s);
+int launch_step2_fp32(const StepArgs& a, int solver, int cpl, int wpg, const void* qin,
+                      void* qout, const void* aux, double* cfl, hipStream_t s);
+
+// Validates a physics description; returns HIPCLAW_OK or an error code with
+// the message set.
+int validate_physics(const hipclaw_physics* p);
+
+}  // namespace hc

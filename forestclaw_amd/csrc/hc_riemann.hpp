@@ -1,0 +1,363 @@
+// hc_riemann.hpp -- normal (rpn2) and transverse (rpt2) Riemann solvers for
+// one face, evaluated in registers by one lane.
+//
+// Each routine restates the reference Fortran solver for a single interface
+// with the same operation order, so the FP64 instantiation (built without FMA
+// contraction) reproduces the reference bit for bit:
+//   Adv     applications/clawpack/advection/2d/all/clawpack46_rpn2adv.f:30-58,
+//           clawpack46_rpt2adv.f:18-25
+//   SWE     applications/clawpack/shallow/2d/rp/clawpack46_rpn2.f:83-245,
+//           clawpack46_rpt2.f:34-72 (Roe averages shared with rpn2 as /comroe/ does)
+//   Euler5  applications/clawpack/euler/2d/rp/clawpack46_rpn2_euler5.f:62-278,
+//           clawpack46_rpt2_euler5.f:48-118 (rpt2 recomputes the same Roe
+//           averages from the same two cells; they are reused here)
+//
+// Naming: qa is the cell on the low side of the face ("qr(i-1)" in the
+// reference), qb the cell on the high side ("ql(i)").  DIR = 0 for an x-face,
+// 1 for a y-face (reference ixy = DIR+1).
+#pragma once
+#include "hc_common.hpp"
+
+namespace hc {
+
+enum { SOLVER_ADV = 0, SOLVER_SWE = 1, SOLVER_EULER5 = 2 };
+
+template <class R, int ME_, int MW_> struct FaceT {
+    static constexpr int ME = ME_, MW = MW_;
+    R w[MW][ME];
+    R s[MW];
+    R am[ME];
+    R ap[ME];
+};
+
+struct PhysParams {
+    double grav;
+    double gamma;
+    int efix;
+};
+
+// ---------------------------------------------------------------------------
+// Scalar advection with edge velocities (aux field 0 = u at the left edge,
+// field 1 = v at the bottom edge).
+// ---------------------------------------------------------------------------
+template <class R> struct Adv {
+    static constexpr int ME = 1, MW = 1, MAUX = 2;
+    using Face = FaceT<R, 1, 1>;
+    // transverse speeds for the amdq split (cell on the low side) and the
+    // apdq split (cell on the high side): {down, up}
+    struct Trans { R m_dn, m_up, p_dn, p_up; };
+
+    template <int DIR>
+    __device__ static inline void rpn2(const R* qa, const R* qb, R sface, const PhysParams&,
+                                       Face& f, Trans&)
+    {
+        R delta = qb[0] - qa[0];
+        f.w[0][0] = delta;
+        f.s[0] = sface;
+        f.am[0] = R(0) + dmin(f.s[0], R(0)) * f.w[0][0];
+        f.ap[0] = R(0) + dmax(f.s[0], R(0)) * f.w[0][0];
+    }
+
+    template <int DIR>
+    __device__ static inline void rpt2(int imp, const R* asdq, const Trans& t,
+                                       const PhysParams&, R* bm, R* bp)
+    {
+        R dn = imp == 1 ? t.m_dn : t.p_dn;
+        R up = imp == 1 ? t.m_up : t.p_up;
+        bm[0] = dmin(dn, R(0)) * asdq[0];
+        bp[0] = dmax(up, R(0)) * asdq[0];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Shallow water, Roe solver with Harten-Hyman entropy fix.
+// ---------------------------------------------------------------------------
+template <class R> struct Swe {
+    static constexpr int ME = 3, MW = 3, MAUX = 0;
+    using Face = FaceT<R, 3, 3>;
+    struct Trans { R u, v, a; };
+
+    template <int DIR>
+    __device__ static inline void rpn2(const R* qa, const R* qb, R, const PhysParams& pp,
+                                       Face& f, Trans& roe)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R g = R(pp.grav);
+        R h = (qa[0] + qb[0]) / R(2);
+        R hsl = MM::sqrt_(qa[0]);
+        R hsr = MM::sqrt_(qb[0]);
+        R hsq2 = hsl + hsr;
+        R u = MM::div(MM::div(qa[mu], hsl) + MM::div(qb[mu], hsr), hsq2);
+        R v = MM::div(MM::div(qa[mv], hsl) + MM::div(qb[mv], hsr), hsq2);
+        R a = MM::sqrt_(g * h);
+        roe.u = u; roe.v = v; roe.a = a;
+
+        R d0 = qb[0] - qa[0];
+        R d1 = qb[mu] - qa[mu];
+        R d2 = qb[mv] - qa[mv];
+        R a1 = MM::div((u + a) * d0 - d1, R(2) * a);
+        R a2 = -v * d0 + d2;
+        R a3 = MM::div(-(u - a) * d0 + d1, R(2) * a);
+
+        f.w[0][0] = a1; f.w[0][mu] = a1 * (u - a); f.w[0][mv] = a1 * v;
+        f.s[0] = u - a;
+        f.w[1][0] = R(0); f.w[1][mu] = R(0); f.w[1][mv] = a2;
+        f.s[1] = u;
+        f.w[2][0] = a3; f.w[2][mu] = a3 * (u + a); f.w[2][mv] = a3 * v;
+        f.s[2] = u + a;
+
+        if (!pp.efix) {
+            for (int m = 0; m < 3; m++) {
+                R am = R(0), ap = R(0);
+                for (int w = 0; w < 3; w++) {
+                    am = am + dmin(f.s[w], R(0)) * f.w[w][m];
+                    ap = ap + dmax(f.s[w], R(0)) * f.w[w][m];
+                }
+                f.am[m] = am; f.ap[m] = ap;
+            }
+            return;
+        }
+
+        // entropy fix: amdq from the left-going parts
+        do {
+            R s0 = MM::div(qa[mu], qa[0]) - MM::sqrt_(g * qa[0]);
+            if (s0 > R(0) && f.s[0] > R(0)) {
+                f.am[0] = R(0); f.am[1] = R(0); f.am[2] = R(0);
+                break;
+            }
+            R h1 = qa[0] + f.w[0][0];
+            R hu1 = qa[mu] + f.w[0][mu];
+            R s1 = MM::div(hu1, h1) - MM::sqrt_(g * h1);
+            R sfract;
+            if (s0 < R(0) && s1 > R(0))
+                sfract = s0 * MM::div(s1 - f.s[0], s1 - s0);
+            else if (f.s[0] < R(0))
+                sfract = f.s[0];
+            else
+                sfract = R(0);
+            for (int m = 0; m < 3; m++) f.am[m] = sfract * f.w[0][m];
+            if (f.s[1] > R(0)) break;
+            for (int m = 0; m < 3; m++) f.am[m] = f.am[m] + f.s[1] * f.w[1][m];
+            R s03 = MM::div(qb[mu], qb[0]) + MM::sqrt_(g * qb[0]);
+            R h3 = qb[0] - f.w[2][0];
+            R hu3 = qb[mu] - f.w[2][mu];
+            R s3 = MM::div(hu3, h3) + MM::sqrt_(g * h3);
+            if (s3 < R(0) && s03 > R(0))
+                sfract = s3 * MM::div(s03 - f.s[2], s03 - s3);
+            else if (f.s[2] < R(0))
+                sfract = f.s[2];
+            else
+                break;
+            for (int m = 0; m < 3; m++) f.am[m] = f.am[m] + sfract * f.w[2][m];
+        } while (0);
+        for (int m = 0; m < 3; m++) {
+            R df = R(0);
+            for (int w = 0; w < 3; w++) df = df + f.s[w] * f.w[w][m];
+            f.ap[m] = df - f.am[m];
+        }
+    }
+
+    template <int DIR>
+    __device__ static inline void rpt2(int, const R* asdq, const Trans& roe, const PhysParams&,
+                                       R* bm, R* bp)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R u = roe.u, v = roe.v, a = roe.a;
+        R a1 = MM::div((v + a) * asdq[0] - asdq[mv], R(2) * a);
+        R a2 = asdq[mu] - u * asdq[0];
+        R a3 = MM::div(-(v - a) * asdq[0] + asdq[mv], R(2) * a);
+        R wb[3][3], sb[3];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
+        wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
+        wb[2][0] = a3; wb[2][mu] = a3 * u; wb[2][mv] = a3 * (v + a); sb[2] = v + a;
+        for (int m = 0; m < 3; m++) {
+            R b0 = R(0), b1 = R(0);
+            for (int w = 0; w < 3; w++) {
+                b0 = b0 + dmin(sb[w], R(0)) * wb[w][m];
+                b1 = b1 + dmax(sb[w], R(0)) * wb[w][m];
+            }
+            bm[m] = b0; bp[m] = b1;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Euler equations + advected tracer, Roe solver with entropy fix.
+// ---------------------------------------------------------------------------
+template <class R> struct Euler5 {
+    static constexpr int ME = 5, MW = 5, MAUX = 0;
+    using Face = FaceT<R, 5, 5>;
+    struct Trans { R u, v, enth, a, g1a2, euv, u2v2; };
+
+    template <int DIR>
+    __device__ static inline void rpn2(const R* qa, const R* qb, R, const PhysParams& pp,
+                                       Face& f, Trans& t)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R gam = R(pp.gamma);
+        const R g1 = R(pp.gamma - 1.0);
+        R rl = MM::sqrt_(qa[0]);
+        R rr = MM::sqrt_(qb[0]);
+        R pl = g1 * (qa[3] - MM::div(R(0.5) * (qa[1] * qa[1] + qa[2] * qa[2]), qa[0]));
+        R pr = g1 * (qb[3] - MM::div(R(0.5) * (qb[1] * qb[1] + qb[2] * qb[2]), qb[0]));
+        R rsq2 = rl + rr;
+        R u = MM::div(MM::div(qa[mu], rl) + MM::div(qb[mu], rr), rsq2);
+        R v = MM::div(MM::div(qa[mv], rl) + MM::div(qb[mv], rr), rsq2);
+        R enth = MM::div(MM::div(qa[3] + pl, rl) + MM::div(qb[3] + pr, rr), rsq2);
+        R u2v2 = u * u + v * v;
+        R a2c = g1 * (enth - R(.5) * u2v2);
+        R a = MM::sqrt_(a2c);
+        R g1a2 = MM::div(g1, a2c);
+        R euv = enth - u2v2;
+        t.u = u; t.v = v; t.enth = enth; t.a = a; t.g1a2 = g1a2; t.euv = euv; t.u2v2 = u2v2;
+
+        R d0 = qb[0] - qa[0], d1 = qb[mu] - qa[mu], d2 = qb[mv] - qa[mv], d3 = qb[3] - qa[3];
+        R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
+        R a2 = d2 - v * d0;
+        R a4 = MM::div(d1 + (a - u) * d0 - a * a3, R(2) * a);
+        R a1 = d0 - a3 - a4;
+
+        f.w[0][0] = a1; f.w[0][mu] = a1 * (u - a); f.w[0][mv] = a1 * v;
+        f.w[0][3] = a1 * (enth - u * a); f.w[0][4] = R(0); f.s[0] = u - a;
+        f.w[1][0] = R(0); f.w[1][mu] = R(0); f.w[1][mv] = a2;
+        f.w[1][3] = a2 * v; f.w[1][4] = R(0); f.s[1] = u;
+        f.w[2][0] = a3; f.w[2][mu] = a3 * u; f.w[2][mv] = a3 * v;
+        f.w[2][3] = a3 * R(0.5) * u2v2; f.w[2][4] = R(0); f.s[2] = u;
+        f.w[3][0] = a4; f.w[3][mu] = a4 * (u + a); f.w[3][mv] = a4 * v;
+        f.w[3][3] = a4 * (enth + u * a); f.w[3][4] = R(0); f.s[3] = u + a;
+        f.w[4][0] = R(0); f.w[4][mu] = R(0); f.w[4][mv] = R(0);
+        f.w[4][3] = R(0); f.w[4][4] = qb[4] - qa[4]; f.s[4] = u;
+
+        if (!pp.efix) {
+            for (int m = 0; m < 5; m++) {
+                R am = R(0), ap = R(0);
+                for (int w = 0; w < 5; w++) {
+                    if (f.s[w] < R(0))
+                        am = am + f.s[w] * f.w[w][m];
+                    else
+                        ap = ap + f.s[w] * f.w[w][m];
+                }
+                f.am[m] = am; f.ap[m] = ap;
+            }
+            return;
+        }
+
+        do {
+            R rhoim1 = qa[0];
+            R pim1 = g1 * (qa[3] - MM::div(R(0.5) * (qa[mu] * qa[mu] + qa[mv] * qa[mv]), rhoim1));
+            R cim1 = MM::sqrt_(MM::div(gam * pim1, rhoim1));
+            R s0 = MM::div(qa[mu], rhoim1) - cim1;
+            if (s0 >= R(0) && f.s[0] > R(0)) {
+                for (int m = 0; m < 5; m++) f.am[m] = R(0);
+                break;
+            }
+            R rho1 = qa[0] + f.w[0][0];
+            R rhou1 = qa[mu] + f.w[0][mu];
+            R rhov1 = qa[mv] + f.w[0][mv];
+            R en1 = qa[3] + f.w[0][3];
+            R p1 = g1 * (en1 - MM::div(R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1), rho1));
+            R c1 = MM::sqrt_(MM::div(gam * p1, rho1));
+            R s1 = MM::div(rhou1, rho1) - c1;
+            R sfract;
+            if (s0 < R(0) && s1 > R(0))
+                sfract = MM::div(s0 * (s1 - f.s[0]), s1 - s0);
+            else if (f.s[0] < R(0))
+                sfract = f.s[0];
+            else
+                sfract = R(0);
+            for (int m = 0; m < 5; m++) f.am[m] = sfract * f.w[0][m];
+            if (f.s[1] >= R(0)) break;
+            for (int m = 0; m < 5; m++) {
+                f.am[m] = f.am[m] + f.s[1] * f.w[1][m];
+                f.am[m] = f.am[m] + f.s[2] * f.w[2][m];
+                f.am[m] = f.am[m] + f.s[4] * f.w[4][m];
+            }
+            R rhoi = qb[0];
+            R pi_ = g1 * (qb[3] - MM::div(R(0.5) * (qb[mu] * qb[mu] + qb[mv] * qb[mv]), rhoi));
+            R ci = MM::sqrt_(MM::div(gam * pi_, rhoi));
+            R s3 = MM::div(qb[mu], rhoi) + ci;
+            R rho2 = qb[0] - f.w[3][0];
+            R rhou2 = qb[mu] - f.w[3][mu];
+            R rhov2 = qb[mv] - f.w[3][mv];
+            R en2 = qb[3] - f.w[3][3];
+            R p2 = g1 * (en2 - MM::div(R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2), rho2));
+            R c2 = MM::sqrt_(MM::div(gam * p2, rho2));
+            R s2 = MM::div(rhou2, rho2) + c2;
+            if (s2 < R(0) && s3 > R(0))
+                sfract = MM::div(s2 * (s3 - f.s[3]), s3 - s2);
+            else if (f.s[3] < R(0))
+                sfract = f.s[3];
+            else
+                break;
+            for (int m = 0; m < 5; m++) f.am[m] = f.am[m] + sfract * f.w[3][m];
+        } while (0);
+        for (int m = 0; m < 5; m++) {
+            R df = R(0);
+            for (int w = 0; w < 5; w++) df = df + f.s[w] * f.w[w][m];
+            f.ap[m] = df - f.am[m];
+        }
+    }
+
+    template <int DIR>
+    __device__ static inline void rpt2(int, const R* asdq, const Trans& t, const PhysParams&,
+                                       R* bm, R* bp)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R u = t.u, v = t.v, enth = t.enth, a = t.a, g1a2 = t.g1a2, euv = t.euv,
+                u2v2 = t.u2v2;
+        R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
+        R a2 = asdq[mu] - u * asdq[0];
+        R a4 = MM::div(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
+        R a1 = asdq[0] - a3 - a4;
+        R wb[4][5], sb[4];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
+        wb[0][3] = a1 * (enth - v * a); wb[0][4] = R(0); sb[0] = v - a;
+        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v;
+        wb[1][3] = a3 * R(0.5) * u2v2 + a2 * u; wb[1][4] = R(0); sb[1] = v;
+        wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a);
+        wb[2][3] = a4 * (enth + v * a); wb[2][4] = R(0); sb[2] = v + a;
+        wb[3][0] = R(0); wb[3][mu] = R(0); wb[3][mv] = R(0); wb[3][3] = R(0);
+        wb[3][4] = asdq[4]; sb[3] = v;
+        for (int m = 0; m < 5; m++) {
+            R b0 = R(0), b1 = R(0);
+            for (int w = 0; w < 4; w++) {
+                if (sb[w] < R(0))
+                    b0 = b0 + sb[w] * wb[w][m];
+                else
+                    b1 = b1 + sb[w] * wb[w][m];
+            }
+            bm[m] = b0; bp[m] = b1;
+        }
+    }
+};
+
+// Limiter phi(r) of the reference's inline limiter
+// (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_inlinelimiter.f:66-108).
+template <class R> __device__ inline R limiter_phi(int mthlim, R r)
+{
+    switch (mthlim) {
+    case 1: return dmax(R(0), dmin(R(1), r));
+    case 2: return dmax(dmax(R(0), dmin(R(1), R(2) * r)), dmin(R(2), r));
+    case 3: return M<R>::div(r + M<R>::abs_(r), R(1) + M<R>::abs_(r));
+    case 4: {
+        R c = (R(1) + r) / R(2);
+        return dmax(R(0), dmin(dmin(c, R(2)), R(2) * r));
+    }
+    case 5: return r;
+    case 6: {
+        const R th = R((double)1.3f);  // single-precision literal in the reference
+        return dmax(R(0), dmin(dmin(th * r, (R(1) + r) / R(2)), th));
+    }
+    default: return R(1);
+    }
+}
+
+}  // namespace hc

@@ -1,0 +1,504 @@
+// hc_step2.hpp -- fused wave-propagation patch step for gfx950.
+//
+// Computes, for every patch of a batch, what clawpack46_step2_wrap computes
+// for the patch interior (src/solvers/fc2d_clawpack4.6/fortran_source/
+// clawpack46_step2_wrap.f:104-125, clawpack46_step2.f:113-236,
+// clawpack46_flux2.f:116-224, clawpack46_inlinelimiter.f:39-115), plus the
+// patch Courant number (clawpack46_flux2.f:129-137, clawpack46_step2.f:153,222).
+//
+// Mapping.  A patch tile of NX = mx+4 columns is owned by LPP = ceil(NX/CPL)
+// consecutive lanes of one wave; lane lp owns columns lp*CPL .. lp*CPL+CPL-1
+// and walks the tile rows J = 0 .. NY-1 once.  At row J the lane
+//   * solves the x-face to the left of each of its columns on row J
+//     (neighbour columns arrive by cross-lane shuffles),
+//   * solves the y-face below (c, J) of its own columns,
+//   * finalises y-face J-1 (its limiter needed face J), and
+//   * writes the updated interior cell (c, J-2).
+// The whole x/y sweep therefore reads each tile row from HBM once, keeps the
+// sliding windows in registers and uses no LDS.  Every flux sum is assembled
+// in one lane in the order the reference accumulates fm/fp/gm/gp, so the FP64
+// instantiation (no FMA contraction) is bitwise equal to the reference on the
+// interior.  Only interior cells are written; the ghost band of q_out is left
+// untouched (the reference's post-step ghost values are overwritten by the
+// next ghost fill in any case).
+//
+// mbc must be 2 (ForestClaw's default); the host rejects other values.
+#pragma once
+#include "hc_riemann.hpp"
+
+namespace hc {
+
+struct StepArgs {
+    int npatch;
+    int mx, my;
+    int order2, transverse;
+    int mthlim[5];
+    int lpp, ppw;          // lanes per patch, patches per wave
+    PhysParams pp;
+    double dt;
+    const double* dxdy;    // per patch (dx, dy)
+    const int* patch_ids;  // optional indirection (nullptr = identity)
+    long long qstride;     // elements per patch in q (NX*NY*meqn)
+    long long astride;     // elements per patch in aux (NX*NY*maux)
+};
+
+template <int CPL, class T> __device__ inline T from_left(const T (&v)[CPL], int k)
+{
+    T s = shl_up(v[CPL - 1]);
+    return k > 0 ? v[k > 0 ? k - 1 : 0] : s;
+}
+template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], int k)
+{
+    T s = shl_dn(v[0]);
+    return k < CPL - 1 ? v[k < CPL - 1 ? k + 1 : 0] : s;
+}
+
+template <class R, template <class> class SolverT, int CPL>
+__global__ void __launch_bounds__(256)
+k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
+        const R* __restrict__ aux, double* __restrict__ cflp)
+{
+    using S = SolverT<R>;
+    constexpr int ME = S::ME, MW = S::MW;
+    using Face = typename S::Face;
+    using Trans = typename S::Trans;
+
+    const int lane = threadIdx.x & (HC_WAVE - 1);
+    const int gwave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+    const int slot = lane / A.lpp;
+    const int lp = lane - slot * A.lpp;
+    const int pidx = gwave * A.ppw + slot;
+    const bool pact = (slot < A.ppw) && (pidx < A.npatch);
+    const int patch = pact ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
+    const int NX = A.mx + 4, NY = A.my + 4;
+
+    const R* __restrict__ qp = qin + (long long)patch * A.qstride;
+    R* __restrict__ qo = qout + (long long)patch * A.qstride;
+    const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)patch * A.astride : nullptr;
+
+    const double dx = pact ? A.dxdy[2 * patch] : 1.0;
+    const double dy = pact ? A.dxdy[2 * patch + 1] : 1.0;
+    const R dtdx = R(A.dt / dx);
+    const R dtdy = R(A.dt / dy);
+    const R hdtdx = R(0.5) * dtdx;
+    const R hdtdy = R(0.5) * dtdy;
+    const PhysParams pp = A.pp;
+    const bool second = A.order2 == 2;
+    const bool trans = A.transverse > 0;
+    const bool cqsplit = (A.order2 > 1) && (A.transverse == 2);
+
+    int col[CPL];
+    bool cv[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+        col[k] = lp * CPL + k;
+        cv[k] = pact && col[k] < NX;
+    }
+
+    // ---- carried state -------------------------------------------------
+    R qm1[CPL][ME], qm2[CPL][ME];
+    Face fyp[CPL] = {};
+    Trans typ[CPL] = {};
+    R dotly[CPL][MW];
+    R hmp_prev[CPL][ME], hpp_prev[CPL][ME], fdp_prev[CPL][ME];
+    R fmx1[CPL][ME], fpx1[CPL][ME], fmx2[CPL][ME], fpx2[CPL][ME];
+    R G1[CPL][ME], G2[CPL][ME], d2p[CPL][ME];
+    R um1[CPL], um2[CPL];  // advection: u(c, J-1), u(c, J-2)
+    R cfl[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) {
+        cfl[k] = R(0);
+        um1[k] = um2[k] = R(0);
+#pragma unroll
+        for (int m = 0; m < ME; m++) {
+            qm1[k][m] = qm2[k][m] = R(0);
+            hmp_prev[k][m] = hpp_prev[k][m] = fdp_prev[k][m] = R(0);
+            fmx1[k][m] = fpx1[k][m] = fmx2[k][m] = fpx2[k][m] = R(0);
+            G1[k][m] = G2[k][m] = d2p[k][m] = R(0);
+        }
+#pragma unroll
+        for (int w = 0; w < MW; w++) dotly[k][w] = R(0);
+    }
+
+    for (int J = 0; J < NY; J++) {
+        // ---- load row J ------------------------------------------------
+        R qc[CPL][ME];
+        R ucur[CPL], vcur[CPL], vnext[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+#pragma unroll
+            for (int m = 0; m < ME; m++)
+                qc[k][m] = cv[k] ? qp[((long long)m * NY + J) * NX + col[k]] : R(0);
+            if constexpr (S::MAUX > 0) {
+                ucur[k] = cv[k] ? ap[(long long)J * NX + col[k]] : R(0);
+                vcur[k] = cv[k] ? ap[((long long)NY + J) * NX + col[k]] : R(0);
+                vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + J + 1) * NX + col[k]] : R(0);
+            } else {
+                ucur[k] = vcur[k] = vnext[k] = R(0);
+            }
+        }
+
+        // ---- x-sweep on row J (reference rows 2-mbc .. my+mbc-1) -----------
+        R fmxn[CPL][ME], fpxn[CPL][ME], Gn[CPL][ME], d2n[CPL][ME];
+        if (J >= 1 && J <= NY - 2) {
+            Face fx[CPL];
+            Trans tx[CPL];
+            R qa[CPL][ME];
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+                R t[CPL];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) t[k] = qc[k][m];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) qa[k][m] = from_left<CPL>(t, k);
+            }
+            R vl_cur[CPL], vl_next[CPL];
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                vl_cur[k] = from_left<CPL>(vcur, k);
+                vl_next[k] = from_left<CPL>(vnext, k);
+            }
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                S::template rpn2<0>(qa[k], qc[k], ucur[k], pp, fx[k], tx[k]);
+                if constexpr (S::MAUX > 0) {
+                    tx[k] = Trans{vl_cur[k], vl_next[k], vcur[k], vnext[k]};
+                }
+                // Courant number on faces i = 1 .. mx+1 of every swept row
+                if (cv[k] && col[k] >= 2 && col[k] <= NX - 2) {
+#pragma unroll
+                    for (int w = 0; w < MW; w++) {
+                        R a = dtdx * fx[k].s[w];
+                        R b = -dtdx * fx[k].s[w];
+                        cfl[k] = dmax(dmax(cfl[k], a), b);
+                    }
+                }
+            }
+            R cq[CPL][ME];
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+#pragma unroll
+                for (int m = 0; m < ME; m++) cq[k][m] = R(0);
+            if (second) {
+                // limiter: needs the unlimited waves of the faces on each side
+                R dotr[CPL][MW];
+#pragma unroll
+                for (int w = 0; w < MW; w++) {
+                    R wr[CPL][ME];
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        R t[CPL];
+#pragma unroll
+                        for (int k = 0; k < CPL; k++) t[k] = fx[k].w[w][m];
+#pragma unroll
+                        for (int k = 0; k < CPL; k++) wr[k][m] = from_right<CPL>(t, k);
+                    }
+#pragma unroll
+                    for (int k = 0; k < CPL; k++) {
+                        R d = R(0);
+#pragma unroll
+                        for (int m = 0; m < ME; m++) d = d + fx[k].w[w][m] * wr[k][m];
+                        dotr[k][w] = d;
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < MW; w++) {
+                    R t[CPL];
+#pragma unroll
+                    for (int k = 0; k < CPL; k++) t[k] = dotr[k][w];
+#pragma unroll
+                    for (int k = 0; k < CPL; k++) {
+                        R dotl = from_left<CPL>(t, k);
+                        if (A.mthlim[w] != 0 && col[k] >= 2 && col[k] <= NX - 2) {
+                            R wn = R(0);
+#pragma unroll
+                            for (int m = 0; m < ME; m++) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
+                            if (wn != R(0)) {
+                                R r = fx[k].s[w] > R(0) ? M<R>::div(dotl, wn) : M<R>::div(dotr[k][w], wn);
+                                R phi = limiter_phi<R>(A.mthlim[w], r);
+#pragma unroll
+                                for (int m = 0; m < ME; m++) fx[k].w[w][m] = phi * fx[k].w[w][m];
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++)
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        R c = R(0);
+#pragma unroll
+                        for (int w = 0; w < MW; w++) {
+                            R as = M<R>::abs_(fx[k].s[w]);
+                            c = c + as * (R(1) - as * dtdx) * fx[k].w[w][m];
+                        }
+                        cq[k][m] = c;
+                    }
+            }
+            R fam[CPL][ME], fap[CPL][ME];
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    R a = R(0) + fx[k].am[m];
+                    R b = R(0) - fx[k].ap[m];
+                    if (second) {
+                        a = a + R(0.5) * cq[k][m];
+                        b = b + R(0.5) * cq[k][m];
+                    }
+                    fam[k][m] = a;
+                    fap[k][m] = b;
+                }
+            R hmm[CPL][ME], hpm[CPL][ME], hmp[CPL][ME], hpp[CPL][ME];
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                if (trans) {
+                    R am2[ME], ap2[ME], bm[ME], bp[ME];
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        am2[m] = cqsplit ? fx[k].am[m] + cq[k][m] : fx[k].am[m];
+                        ap2[m] = cqsplit ? fx[k].ap[m] - cq[k][m] : fx[k].ap[m];
+                    }
+                    S::template rpt2<0>(1, am2, tx[k], pp, bm, bp);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        hmm[k][m] = hdtdx * bm[m];
+                        hpm[k][m] = hdtdx * bp[m];
+                    }
+                    S::template rpt2<0>(2, ap2, tx[k], pp, bm, bp);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        hmp[k][m] = hdtdx * bm[m];
+                        hpp[k][m] = hdtdx * bp[m];
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < ME; m++) hmm[k][m] = hpm[k][m] = hmp[k][m] = hpp[k][m] = R(0);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+                R t1[CPL], t2[CPL], t3[CPL];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    t1[k] = hmm[k][m];
+                    t2[k] = hpm[k][m];
+                    t3[k] = fam[k][m];
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    R d1 = trans ? (R(0) - from_right<CPL>(t1, k)) - hmp[k][m] : R(0);
+                    R d2 = trans ? (R(0) - from_right<CPL>(t2, k)) - hpp[k][m] : R(0);
+                    fmxn[k][m] = R(0) + from_right<CPL>(t3, k);
+                    fpxn[k][m] = R(0) + fap[k][m];
+                    Gn[k][m] = (R(0) + d2p[k][m]) + d1;
+                    d2n[k][m] = d2;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+#pragma unroll
+                for (int m = 0; m < ME; m++) fmxn[k][m] = fpxn[k][m] = Gn[k][m] = d2n[k][m] = R(0);
+        }
+
+        // ---- y-sweep: solve face J of each column, finalise face J-1 --------
+        R e1[CPL][ME], e2[CPL][ME], fdm[CPL][ME], fdp[CPL][ME];
+        R hmp_new[CPL][ME], hpp_new[CPL][ME];
+        Face fyn[CPL];
+        Trans tyn[CPL];
+        R dotry[CPL][MW];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+                e1[k][m] = e2[k][m] = fdm[k][m] = fdp[k][m] = R(0);
+                hmp_new[k][m] = hpp_new[k][m] = R(0);
+            }
+#pragma unroll
+            for (int w = 0; w < MW; w++) dotry[k][w] = R(0);
+        }
+        if (J == 0) {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                fyn[k] = Face{};
+                tyn[k] = Trans{};
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                S::template rpn2<1>(qm1[k], qc[k], vcur[k], pp, fyn[k], tyn[k]);
+                if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2) {
+#pragma unroll
+                    for (int w = 0; w < MW; w++) {
+                        R a = dtdy * fyn[k].s[w];
+                        R b = -dtdy * fyn[k].s[w];
+                        cfl[k] = dmax(dmax(cfl[k], a), b);
+                    }
+                }
+            }
+        }
+        if (J >= 2) {
+            // dot product of face J-1 with face J (both unlimited)
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+#pragma unroll
+                for (int w = 0; w < MW; w++) {
+                    R d = R(0);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) d = d + fyp[k].w[w][m] * fyn[k].w[w][m];
+                    dotry[k][w] = d;
+                }
+        }
+        if (J >= 3) {
+            // finalise y-face F = J-1 (2 <= F <= NY-2)
+            R urt_m1[CPL], urt_m2[CPL];
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                urt_m1[k] = from_right<CPL>(um1, k);
+                urt_m2[k] = from_right<CPL>(um2, k);
+            }
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                Face& f = fyp[k];
+                R cq[ME];
+#pragma unroll
+                for (int m = 0; m < ME; m++) cq[m] = R(0);
+                if (second) {
+#pragma unroll
+                    for (int w = 0; w < MW; w++) {
+                        if (A.mthlim[w] != 0) {
+                            R wn = R(0);
+#pragma unroll
+                            for (int m = 0; m < ME; m++) wn = wn + f.w[w][m] * f.w[w][m];
+                            if (wn != R(0)) {
+                                R r = f.s[w] > R(0) ? M<R>::div(dotly[k][w], wn)
+                                                    : M<R>::div(dotry[k][w], wn);
+                                R phi = limiter_phi<R>(A.mthlim[w], r);
+#pragma unroll
+                                for (int m = 0; m < ME; m++) f.w[w][m] = phi * f.w[w][m];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        R c = R(0);
+#pragma unroll
+                        for (int w = 0; w < MW; w++) {
+                            R as = M<R>::abs_(f.s[w]);
+                            c = c + as * (R(1) - as * dtdy) * f.w[w][m];
+                        }
+                        cq[m] = c;
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    R a = R(0) + f.am[m];
+                    R b = R(0) - f.ap[m];
+                    if (second) {
+                        a = a + R(0.5) * cq[m];
+                        b = b + R(0.5) * cq[m];
+                    }
+                    fdm[k][m] = a;
+                    fdp[k][m] = b;
+                }
+                if (trans) {
+                    Trans t = typ[k];
+                    if constexpr (S::MAUX > 0) {
+                        // amdq split uses cell F-1 = J-2, apdq split cell F = J-1;
+                        // "down" is this column, "up" the column to the right
+                        t = Trans{um2[k], urt_m2[k], um1[k], urt_m1[k]};
+                    }
+                    R am2[ME], ap2[ME], bm[ME], bp[ME];
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        am2[m] = cqsplit ? f.am[m] + cq[m] : f.am[m];
+                        ap2[m] = cqsplit ? f.ap[m] - cq[m] : f.ap[m];
+                    }
+                    S::template rpt2<1>(1, am2, t, pp, bm, bp);
+                    R hmm_y[ME], hpm_y[ME];
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        hmm_y[m] = hdtdy * bm[m];
+                        hpm_y[m] = hdtdy * bp[m];
+                    }
+                    S::template rpt2<1>(2, ap2, t, pp, bm, bp);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        hmp_new[k][m] = hdtdy * bm[m];
+                        hpp_new[k][m] = hdtdy * bp[m];
+                        e1[k][m] = (R(0) - hmm_y[m]) - hmp_prev[k][m];
+                        e2[k][m] = (R(0) - hpm_y[m]) - hpp_prev[k][m];
+                    }
+                }
+            }
+        }
+
+        // ---- interior cell row R = J-2 ------------------------------------
+        const int Rw = J - 2;
+        if (Rw >= 2 && Rw <= NY - 3) {
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+                R t1[CPL], t2[CPL];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    t1[k] = e1[k][m];
+                    t2[k] = e2[k][m];
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    R e1r = from_right<CPL>(t1, k);
+                    R e2l = from_left<CPL>(t2, k);
+                    if (cv[k] && col[k] >= 2 && col[k] <= NX - 3) {
+                        R fmr = (fmx2[k][m] + e2[k][m]) + e1r;
+                        R fpc = (fpx2[k][m] + e2l) + e1[k][m];
+                        R gmu = G1[k][m] + fdm[k][m];
+                        R gpc = G2[k][m] + fdp_prev[k][m];
+                        R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
+                        qo[((long long)m * NY + Rw) * NX + col[k]] = qn;
+                    }
+                }
+            }
+        }
+
+        // ---- rotate the sliding windows -----------------------------------
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+                qm2[k][m] = qm1[k][m];
+                qm1[k][m] = qc[k][m];
+                fmx2[k][m] = fmx1[k][m];
+                fpx2[k][m] = fpx1[k][m];
+                fmx1[k][m] = fmxn[k][m];
+                fpx1[k][m] = fpxn[k][m];
+                G2[k][m] = G1[k][m];
+                G1[k][m] = Gn[k][m];
+                d2p[k][m] = d2n[k][m];
+                hmp_prev[k][m] = hmp_new[k][m];
+                hpp_prev[k][m] = hpp_new[k][m];
+                fdp_prev[k][m] = fdp[k][m];
+            }
+#pragma unroll
+            for (int w = 0; w < MW; w++) dotly[k][w] = dotry[k][w];
+            fyp[k] = fyn[k];
+            typ[k] = tyn[k];
+            um2[k] = um1[k];
+            um1[k] = ucur[k];
+        }
+    }
+
+    // ---- per-patch Courant number: segmented max over the patch's lanes ----
+    R c = R(0);
+#pragma unroll
+    for (int k = 0; k < CPL; k++) c = dmax(c, cfl[k]);
+    double cd = (double)c;
+#pragma unroll
+    for (int d = 1; d < HC_WAVE; d <<= 1) {
+        double t = __shfl_up(cd, d, HC_WAVE);
+        if (lp - d >= 0) cd = dmax(cd, t);
+    }
+    if (pact && lp == A.lpp - 1 && cflp) cflp[patch] = cd;
+}
+
+}  // namespace hc

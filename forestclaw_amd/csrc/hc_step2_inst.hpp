@@ -1,0 +1,47 @@
+// hc_step2_inst.hpp -- instantiates the patch-step kernel for one precision
+// (included by hc_step2_fp64.hip / hc_step2_fp32.hip, which differ only in
+// the element type and in their floating-point contraction flags).
+#pragma once
+#include "hc_internal.hpp"
+
+namespace hc {
+
+template <class R, template <class> class SolverT, int CPL>
+static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
+                      const void* aux, double* cfl, hipStream_t s)
+{
+    StepArgs a = a0;
+    const int NX = a.mx + 4;
+    a.lpp = (NX + CPL - 1) / CPL;
+    if (a.lpp > HC_WAVE) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patch too wide for one wave");
+    a.ppw = HC_WAVE / a.lpp;
+    const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
+    const long long blocks = (waves + wpg - 1) / wpg;
+    if (blocks == 0) return HIPCLAW_OK;
+    hipLaunchKernelGGL((k_step2<R, SolverT, CPL>), dim3((unsigned)blocks), dim3(HC_WAVE * wpg),
+                       0, s, a, static_cast<const R*>(qin), static_cast<R*>(qout),
+                       static_cast<const R*>(aux), cfl);
+    return check_hip(hipGetLastError(), "k_step2 launch");
+}
+
+template <class R>
+static int launch_step2_t(const StepArgs& a, int solver, int cpl, int wpg, const void* qin,
+                          void* qout, const void* aux, double* cfl, hipStream_t s)
+{
+    if (cpl == 1) {
+        switch (solver) {
+        case SOLVER_ADV: return launch_one<R, Adv, 1>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_SWE: return launch_one<R, Swe, 1>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_EULER5: return launch_one<R, Euler5, 1>(a, wpg, qin, qout, aux, cfl, s);
+        }
+    } else if (cpl == 2) {
+        switch (solver) {
+        case SOLVER_ADV: return launch_one<R, Adv, 2>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_SWE: return launch_one<R, Swe, 2>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_EULER5: return launch_one<R, Euler5, 2>(a, wpg, qin, qout, aux, cfl, s);
+        }
+    }
+    return set_error(HIPCLAW_ERR_ARG, "unsupported solver / cols_per_lane");
+}
+
+}  // namespace hc

@@ -1,0 +1,144 @@
+"""ctypes binding of libhipclaw.so (include/hipclaw.h).
+
+The product path: every call goes to the HIP library.  If the library is not
+built, or no GPU is present, calls raise; there is no CPU fallback.
+Device buffers are passed as raw device addresses (torch tensors on cuda are
+accepted and converted with .data_ptr()).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libhipclaw.so")
+
+SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
+FP64, FP32 = 0, 1
+ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "unsupported",
+          4: "out of device memory", 5: "call out of sequence"}
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [
+        ("solver", ctypes.c_int),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("mbc", ctypes.c_int),
+        ("meqn", ctypes.c_int), ("maux", ctypes.c_int), ("mwaves", ctypes.c_int),
+        ("order", ctypes.c_int * 2),
+        ("mthlim", ctypes.c_int * 5),
+        ("mthbc", ctypes.c_int * 4),
+        ("efix", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("grav", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+    ]
+
+
+class Launch(ctypes.Structure):
+    _fields_ = [("cols_per_lane", ctypes.c_int), ("waves_per_group", ctypes.c_int)]
+
+
+class HipclawError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_VP = ctypes.c_void_p
+_SIGS = {
+    "hipclaw_abi_version": ([], ctypes.c_int),
+    "hipclaw_last_error": ([], ctypes.c_char_p),
+    "hipclaw_physics_default": ([ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.POINTER(Physics)], ctypes.c_int),
+    "hipclaw_step2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP, _VP,
+                             ctypes.c_double, _VP, ctypes.POINTER(Launch), _VP], ctypes.c_int),
+    "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_swirl_aux_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP, _VP],
+                                ctypes.c_int),
+    "hipclaw_bc2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP],
+                          ctypes.c_int),
+}
+
+
+def lib() -> ctypes.CDLL:
+    """Load libhipclaw.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HipclawError(
+                f"{LIB_PATH} is missing: build it with `python -m forestclaw_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().hipclaw_last_error().decode(errors="replace")
+        raise HipclawError(f"hipclaw error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return stream
+
+
+def physics(solver: int, mx: int, my: int, *, precision: int = FP64, order=(2, 2),
+            mthlim=None, mthbc=(1, 1, 1, 1), grav: float = 1.0, gamma: float = 1.4,
+            efix: int = 1) -> Physics:
+    p = Physics()
+    _check(lib().hipclaw_physics_default(solver, mx, my, ctypes.byref(p)))
+    p.precision = precision
+    p.order[0], p.order[1] = order
+    if mthlim is not None:
+        for k in range(5):
+            p.mthlim[k] = mthlim[k] if k < len(mthlim) else 0
+    for k in range(4):
+        p.mthbc[k] = mthbc[k]
+    p.grav, p.gamma, p.efix = grav, gamma, efix
+    return p
+
+
+def step2_batch(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, cfl=None,
+                patch_ids=None, launch: tuple[int, int] | None = None, stream=None) -> None:
+    ln = None
+    if launch is not None:
+        ln = Launch(*launch)
+    _check(lib().hipclaw_step2_batch(
+        ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy),
+        _ptr(patch_ids), float(dt), _ptr(cfl), ctypes.byref(ln) if ln is not None else None,
+        _stream(stream)))
+
+
+def max_reduce(x, n: int, out, stream=None) -> None:
+    _check(lib().hipclaw_max_reduce(_ptr(x), n, _ptr(out), _stream(stream)))
+
+
+def swirl_aux_batch(phys: Physics, npatch: int, xylower, dxdy, t: float, dt: float,
+                    tperiod: float, aux, stream=None) -> None:
+    _check(lib().hipclaw_swirl_aux_batch(ctypes.byref(phys), npatch, _ptr(xylower), _ptr(dxdy),
+                                         t, dt, tperiod, _ptr(aux), _stream(stream)))
+
+
+def bc2_batch(phys: Physics, npatch: int, q, mthbc, patch_ids=None, stream=None) -> None:
+    _check(lib().hipclaw_bc2_batch(ctypes.byref(phys), npatch, _ptr(q), _ptr(patch_ids),
+                                   _ptr(mthbc), _stream(stream)))

@@ -1,0 +1,121 @@
+/*
+ * hipclaw.h -- C ABI of the MI355X-native (gfx950) Clawpack-4.6 wave-propagation
+ * patch solver for ForestClaw.
+ *
+ * Plain C types only (no HIP or torch types): device buffers are passed as
+ * void* device addresses, HIP streams as void* (NULL = the default stream).
+ * Every entry point returns an int status (HIPCLAW_OK = 0) and never exits;
+ * hipclaw_last_error() returns a message for the calling thread's last
+ * failure.
+ *
+ * Storage layout of one patch is the clawpatch FArrayBox layout
+ * q(1-mbc:mx+mbc, 1-mbc:my+mbc, meqn), i fastest, field-major
+ * (reference src/patches/clawpatch/fclaw2d_clawpatch.cpp:305-335); a batch of
+ * patches is packed back to back, T*meqn elements per patch with
+ * T = (mx+2mbc)(my+2mbc).  Element type is double (HIPCLAW_FP64) or float
+ * (HIPCLAW_FP32) as selected in hipclaw_physics.precision.
+ */
+#ifndef HIPCLAW_H
+#define HIPCLAW_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPCLAW_ABI_VERSION 1
+
+#define HIPCLAW_OK 0
+#define HIPCLAW_ERR_ARG 1         /* invalid argument or configuration      */
+#define HIPCLAW_ERR_HIP 2         /* HIP runtime failure                    */
+#define HIPCLAW_ERR_UNSUPPORTED 3 /* valid in the reference, not built here */
+#define HIPCLAW_ERR_NOMEM 4       /* device allocation failed               */
+#define HIPCLAW_ERR_STATE 5       /* call out of sequence                   */
+
+/* Riemann solver families (replace the fc2d_clawpack46_vtable fort_rpn2 /
+ * fort_rpt2 function pointers, which cannot run on the device). */
+#define HIPCLAW_SOLVER_ADV 0        /* clawpack46_rpn2adv / rpt2adv (meqn 1, maux 2) */
+#define HIPCLAW_SOLVER_SWE_ROE 1    /* shallow-water Roe + entropy fix (meqn 3)      */
+#define HIPCLAW_SOLVER_EULER5_ROE 2 /* Euler + tracer Roe + entropy fix (meqn 5)     */
+
+#define HIPCLAW_FP64 0
+#define HIPCLAW_FP32 1
+
+/* Solver configuration.  Replaces the Fortran common blocks /cparam/
+ * (grav | gamma, gamma1), the [clawpatch] and [clawpack46] option groups
+ * (reference src/solvers/fc2d_clawpack4.6/fc2d_clawpack46_options.c:93-99:
+ * method(2) = order[0], method(3) = order[1]) and the entropy-fix flag. */
+typedef struct hipclaw_physics {
+    int solver;
+    int mx, my, mbc;  /* mbc must be 2 */
+    int meqn, maux, mwaves;
+    int order[2];
+    int mthlim[5];    /* limiter per wave family: 0 none, 1 minmod, 2 superbee,
+                         3 van Leer, 4 MC, 5 Beam-Warming, 6 generalized minmod */
+    int mthbc[4];     /* left, right, bottom, top: 1 extrapolation, 3 wall     */
+    int efix;         /* 1 = entropy fix (the reference always uses it)        */
+    int precision;    /* HIPCLAW_FP64 or HIPCLAW_FP32                          */
+    double grav;      /* shallow water                                         */
+    double gamma;     /* Euler                                                 */
+} hipclaw_physics;
+
+/* Launch shape of the patch-step kernel (chosen by the autotuner when NULL):
+ * cols_per_lane 1 or 2; waves_per_group 1..4. */
+typedef struct hipclaw_launch {
+    int cols_per_lane;
+    int waves_per_group;
+} hipclaw_launch;
+
+int hipclaw_abi_version(void);
+const char* hipclaw_last_error(void);
+
+/* Fill in the defaults of the reference options for a solver family. */
+int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out);
+
+/* ------------------------------------------------------------------------
+ * Batched single-step update.
+ *
+ * Replaces patch_vt->single_step_update = clawpack46_update
+ * (reference src/solvers/fc2d_clawpack4.6/fc2d_clawpack46.cpp:402-444 ->
+ * clawpack46_step2 :258-400 -> CLAWPACK46_STEP2_WRAP, fortran_source/
+ * clawpack46_step2_wrap.f:1-127) for npatch patches at once, as the deferred
+ * batching of fclaw2d_single_step_buffer_data allows
+ * (src/fclaw2d_update_single_step.h:47-55).
+ *
+ * q_in   device, npatch tiles (ghost cells filled)
+ * q_out  device, npatch tiles; interior cells receive q after the step, the
+ *        ghost band is not written.  q_out == q_in updates in place; a
+ *        distinct q_out leaves q_in as the reference's save_current_step copy
+ *        (fc2d_clawpack46.cpp:295).
+ * aux    device, npatch aux tiles (maux fields) or NULL when maux == 0
+ * dxdy   device, 2 doubles per patch (dx, dy)
+ * patch_ids device, optional list of npatch tile indices (NULL = 0..npatch-1)
+ * cfl_patch device, npatch doubles: the per-patch Courant number returned by
+ *        clawpack46_update (may be NULL)
+ * ------------------------------------------------------------------------ */
+int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_in,
+                        void* q_out, const void* aux, const double* dxdy,
+                        const int* patch_ids, double dt, double* cfl_patch,
+                        const hipclaw_launch* launch, void* stream);
+
+/* Maximum of n device doubles into *out_dev (device), the level CFL of
+ * fclaw2d_update_single_step (src/fclaw2d_update_single_step.c:68). */
+int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream);
+
+/* Swirl velocity field (clawpack46_b4step2 of the swirl application,
+ * applications/clawpack/advection/2d/swirl/user_4.6/b4step2.f:1-41 + psi.f).
+ * xylower: 2 doubles per patch; writes maux = 2 fields per patch. */
+int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const double* xylower,
+                            const double* dxdy, double t, double dt, double tperiod,
+                            void* aux, void* stream);
+
+/* Physical boundary conditions, clawpack46_bc2_default
+ * (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_bc2_default.f:1-213),
+ * per patch mthbc[4] (-1 where the face is not on the physical boundary,
+ * as clawpack46_bc2 builds it, fc2d_clawpack46.cpp:120-128). */
+int hipclaw_bc2_batch(const hipclaw_physics* phys, int npatch, void* q, const int* patch_ids,
+                      const int* mthbc, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HIPCLAW_H */

@@ -1,0 +1,66 @@
+/*
+ * claw46_oracle.h -- CPU FP64 restatement of the Clawpack-4.6 wave-propagation
+ * patch step used by ForestClaw.  TEST INFRASTRUCTURE ONLY: the GPU product
+ * path never links or calls this code; tests, smoke() and the bench's
+ * cpu_baseline leg use it as the checker and as the CPU baseline.
+ *
+ * Storage follows the clawpatch FArrayBox layout
+ * q(1-mbc:mx+mbc, 1-mbc:my+mbc, meqn), i fastest, field-major
+ * (src/patches/clawpatch/fclaw2d_clawpatch.cpp:305-335).  In C the tile index
+ * is (i, j) with i in [0, mx+2mbc), j in [0, my+2mbc); Fortran index
+ * i_f = i - mbc + 1.
+ */
+#ifndef CLAW46_ORACLE_H
+#define CLAW46_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_SOLVER_ADV = 0,        /* clawpack46_rpn2adv / rpt2adv            */
+    ORC_SOLVER_SWE_ROE = 1,    /* shallow/2d/rp clawpack46_rpn2 / rpt2    */
+    ORC_SOLVER_EULER5_ROE = 2  /* euler/2d/rp clawpack46_rpn2/rpt2_euler5 */
+};
+
+typedef struct {
+    int solver;
+    int mx, my, mbc;
+    int meqn, maux, mwaves;
+    int order2;        /* method(2): 1 = Godunov only, 2 = + limited corrections */
+    int transverse;    /* method(3): 0, 1, 2                                    */
+    int mthlim[8];     /* limiter per wave family, 0..6                         */
+    int efix;          /* entropy fix (reference: always on, data efix /.true./)*/
+    double grav;       /* /cparam/ grav  (SWE)                                  */
+    double gamma;      /* /cparam/ gamma (Euler), gamma1 = gamma - 1            */
+} orc_cfg;
+
+/* One patch step in place (clawpack46_step2_wrap semantics, mcapa = 0).
+ * Returns 0 on success, nonzero on bad configuration.  *cfl receives the
+ * patch Courant number (cflgrid). */
+int orc_step2(const orc_cfg* cfg, double* q, const double* aux,
+              double dx, double dy, double dt, double* cfl);
+
+/* npatch independent patches, tiles packed back to back; OpenMP over
+ * patches when compiled with -fopenmp.  cfl[p] per patch. */
+int orc_step2_batch(const orc_cfg* cfg, int npatch, double* q,
+                    const double* aux, const double* dx, const double* dy,
+                    double dt, double* cfl);
+
+/* Swirl b4step2: edge velocities from the stream function (2 aux fields). */
+void orc_swirl_aux(int mx, int my, int mbc, double xlower, double ylower,
+                   double dx, double dy, double t, double dt, double tperiod,
+                   double* aux);
+
+/* clawpack46_bc2_default: mthbc[4] (left,right,bottom,top), -1 = not a
+ * physical face, 1 = extrapolation, 3 = solid wall.  Returns nonzero for
+ * 0 (user) or 2 (periodic), which the reference rejects. */
+int orc_bc2(int mx, int my, int mbc, int meqn, const int mthbc[4], double* q);
+
+/* Limiter function phi(r) of clawpack46_inlinelimiter for one family. */
+double orc_limiter_phi(int mthlim, double r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

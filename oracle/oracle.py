@@ -1,0 +1,131 @@
+"""ctypes front end of the CPU FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; the product path
+(forestclaw_amd) never imports it.
+
+Arrays use the clawpatch tile layout: shape (meqn, my+2mbc, mx+2mbc), C order,
+i.e. q(i, j, m) with i fastest (src/patches/clawpatch/fclaw2d_clawpatch.cpp:305-335).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
+
+
+class OrcCfg(ctypes.Structure):
+    _fields_ = [
+        ("solver", ctypes.c_int),
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("mbc", ctypes.c_int),
+        ("meqn", ctypes.c_int), ("maux", ctypes.c_int), ("mwaves", ctypes.c_int),
+        ("order2", ctypes.c_int), ("transverse", ctypes.c_int),
+        ("mthlim", ctypes.c_int * 8),
+        ("efix", ctypes.c_int),
+        ("grav", ctypes.c_double), ("gamma", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_step2.argtypes = [ctypes.POINTER(OrcCfg), dp, dp, ctypes.c_double,
+                                ctypes.c_double, ctypes.c_double, dp]
+        L.orc_step2_batch.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_int, dp, dp, dp,
+                                      dp, ctypes.c_double, dp]
+        L.orc_swirl_aux.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 7 + [dp]
+        L.orc_swirl_aux.restype = None
+        L.orc_bc2.argtypes = [ctypes.c_int] * 4 + [ip, dp]
+        L.orc_limiter_phi.argtypes = [ctypes.c_int, ctypes.c_double]
+        L.orc_limiter_phi.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = None,
+             order=(2, 2), mthlim=None, grav: float = 1.0, gamma: float = 1.4,
+             efix: int = 1) -> OrcCfg:
+    meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5}[solver]
+    mwaves = meqn
+    if maux is None:
+        maux = 2 if solver == SOLVER_ADV else 0
+    c = OrcCfg()
+    c.solver, c.mx, c.my, c.mbc = solver, mx, my, mbc
+    c.meqn, c.maux, c.mwaves = meqn, maux, mwaves
+    c.order2, c.transverse = order
+    lim = list(mthlim) if mthlim is not None else [4] * mwaves
+    for k in range(8):
+        c.mthlim[k] = lim[k] if k < len(lim) else 0
+    c.efix = efix
+    c.grav, c.gamma = grav, gamma
+    return c
+
+
+def step2(cfg: OrcCfg, q: np.ndarray, aux: np.ndarray | None, dx: float, dy: float,
+          dt: float):
+    """One patch step on a copy of q; returns (q_new, cfl)."""
+    qn = np.ascontiguousarray(q, dtype=np.float64).copy()
+    a = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    cfl = ctypes.c_double(0.0)
+    rc = lib().orc_step2(ctypes.byref(cfg), _dp(qn), None if a is None else _dp(a),
+                         dx, dy, dt, ctypes.byref(cfl))
+    if rc != 0:
+        raise ValueError("oracle rejected the configuration")
+    return qn, cfl.value
+
+
+def step2_batch(cfg: OrcCfg, q: np.ndarray, aux: np.ndarray | None, dx, dy, dt: float):
+    """Step npatch tiles in place (q shape (npatch, meqn, NY, NX)); returns per-patch CFL."""
+    assert q.dtype == np.float64 and q.flags.c_contiguous
+    n = q.shape[0]
+    dxa = np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)).copy()
+    dya = np.broadcast_to(np.asarray(dy, dtype=np.float64), (n,)).copy()
+    cfl = np.zeros(n)
+    a = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+    rc = lib().orc_step2_batch(ctypes.byref(cfg), n, _dp(q), None if a is None else _dp(a),
+                               _dp(dxa), _dp(dya), dt, _dp(cfl))
+    if rc != 0:
+        raise ValueError("oracle rejected the configuration")
+    return cfl
+
+
+def swirl_aux(mx, my, mbc, xlower, ylower, dx, dy, t, dt, tperiod):
+    aux = np.zeros((2, my + 2 * mbc, mx + 2 * mbc))
+    lib().orc_swirl_aux(mx, my, mbc, xlower, ylower, dx, dy, t, dt, tperiod, _dp(aux))
+    return aux
+
+
+def bc2(q: np.ndarray, mx, my, mbc, mthbc):
+    qn = np.ascontiguousarray(q, dtype=np.float64).copy()
+    mb = (ctypes.c_int * 4)(*mthbc)
+    rc = lib().orc_bc2(mx, my, mbc, qn.shape[0], mb, _dp(qn))
+    if rc != 0:
+        raise ValueError("bc2: mthbc 0 (user) and 2 (periodic) are rejected")
+    return qn
+
+
+def limiter_phi(mthlim: int, r: float) -> float:
+    return lib().orc_limiter_phi(mthlim, r)

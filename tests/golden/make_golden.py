@@ -1,0 +1,101 @@
+"""Write golden input/output vectors from the reference's own Fortran.
+
+Run in the build container (needs /root/reference and `make -C oracle ref`):
+    python tests/golden/make_golden.py
+Writes tests/golden/*.npz (inputs, reference outputs and parameters).  The
+fixtures are data only; the reference itself never leaves this container.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import cases as C  # noqa: E402
+import refdrive as R  # noqa: E402
+
+SEED = 20261203
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def main():
+    if not R.available():
+        raise SystemExit("oracle/_ref not built: run `make -C oracle ref` first")
+    rng = np.random.default_rng(SEED)
+
+    # ---- shallow water, Roe + entropy fix -------------------------------
+    swe = []
+    for (mx, my), order, lim in [((16, 12), (2, 2), (4, 4, 4)),
+                                 ((32, 32), (2, 2), (4, 4, 4)),
+                                 ((16, 16), (2, 1), (1, 2, 3)),
+                                 ((16, 16), (1, 0), (4, 4, 4)),
+                                 ((12, 16), (2, 2), (5, 6, 4)),
+                                 ((16, 16), (1, 2), (4, 4, 4))]:
+        mbc = 2
+        q, dx, dy = C.swe_state(rng, mx, my, mbc)
+        dt = C.swe_dt(q, dx, dy)
+        R.set_params("swe", grav=1.0)
+        qo, cfl = R.step2("swe", q, None, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt,
+                          order=order, mthlim=lim)
+        swe.append(dict(q=q, qout=qo, cfl=cfl, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt,
+                        order=np.array(order), mthlim=np.array(lim), grav=1.0))
+    for k, c in enumerate(swe):
+        save(f"swe_step_{k}", **c)
+
+    # ---- Euler + tracer, Roe + entropy fix ------------------------------
+    for k, ((mx, my), order) in enumerate([((16, 16), (2, 2)), ((16, 12), (2, 1))]):
+        mbc = 2
+        q, dx, dy = C.euler_state(rng, mx, my, mbc)
+        dt = C.euler_dt(q, dx, dy)
+        R.set_params("euler", gamma=1.4)
+        qo, cfl = R.step2("euler", q, None, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt,
+                          order=order, mthlim=[4] * 5)
+        save(f"euler_step_{k}", q=q, qout=qo, cfl=cfl, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy,
+             dt=dt, order=np.array(order), mthlim=np.array([4] * 5), gamma=1.4)
+
+    # ---- advection with the swirl velocity field ------------------------
+    for k, ((mx, my), order, t) in enumerate([((16, 16), (2, 2), 0.3),
+                                              ((32, 32), (2, 2), 1.7),
+                                              ((16, 16), (2, 1), 2.9)]):
+        mbc = 2
+        q, dx, dy = C.adv_state(rng, mx, my, mbc)
+        xlower, ylower = 2 * mx * dx, 3 * my * dy
+        aux = R.swirl_aux(mx=mx, my=my, mbc=mbc, xlower=xlower, ylower=ylower, dx=dx, dy=dy,
+                          t=t, dt=0.01, tperiod=8.0)
+        dt = C.adv_dt(aux, dx, dy)
+        R.set_params("adv", tperiod=8.0)
+        qo, cfl = R.step2("adv", q, aux, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt,
+                          order=order, mthlim=[4])
+        save(f"adv_step_{k}", q=q, aux=aux, qout=qo, cfl=cfl, mx=mx, my=my, mbc=mbc, dx=dx,
+             dy=dy, dt=dt, order=np.array(order), mthlim=np.array([4]), xlower=xlower,
+             ylower=ylower, t=t, auxdt=0.01, tperiod=8.0)
+
+    # ---- physical boundary conditions -----------------------------------
+    bcs = []
+    for mthbc in ([1, 1, 1, 1], [3, 3, 3, 3], [1, 3, -1, 3], [-1, -1, 1, 3], [3, -1, 3, -1]):
+        mx, my, mbc = 10, 8, 2
+        q = rng.standard_normal((3, my + 2 * mbc, mx + 2 * mbc))
+        bcs.append((np.array(mthbc), q, R.bc2(q, mx=mx, my=my, mbc=mbc, mthbc=mthbc)))
+    save("bc2", mthbc=np.stack([b[0] for b in bcs]), q=np.stack([b[1] for b in bcs]),
+         qout=np.stack([b[2] for b in bcs]), mx=10, my=8, mbc=2)
+
+    # ---- time interpolation band ----------------------------------------
+    mx, my, mbc = 12, 10, 2
+    qc = rng.standard_normal((3, my + 2 * mbc, mx + 2 * mbc))
+    ql = rng.standard_normal((3, my + 2 * mbc, mx + 2 * mbc))
+    qi, ierr = R.timeinterp(qc, ql, 0.375, mx=mx, my=my, mbc=mbc)
+    assert ierr == 0
+    save("timeinterp", qcurr=qc, qlast=ql, qinterp=qi, alpha=0.375, mx=mx, my=my, mbc=mbc)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,151 @@
+"""Drive the reference's own Fortran patch step (compiled unchanged from
+/root/reference into oracle/_ref/ by oracle/Makefile) through ctypes.
+
+Used only in the build container, by make_golden.py to write fixtures and by
+the CPU tests to cross-check the oracle when oracle/_ref exists.  Nothing here
+runs on a GPU box unless the libraries were built.
+
+Entry points driven (reference file:line):
+  CLAWPACK46_STEP2_WRAP  src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_step2_wrap.f:1
+  (work sizes as fc2d_clawpack46.cpp:345-352, method[] as fc2d_clawpack46_options.c:93-99)
+  clawpack46_b4step2     applications/clawpack/advection/2d/swirl/user_4.6/b4step2.f:1
+  clawpack46_bc2_default src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_bc2_default.f:1
+  fclaw2d_clawpatch46_fort_timeinterp src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:11
+Common blocks (/cparam/, /compi/, /comvt/) are written directly as data.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+
+_ci = ctypes.c_int
+_cd = ctypes.c_double
+_libs: dict[str, ctypes.CDLL] = {}
+
+RP = {  # physics -> (library, rpn2 symbol, rpt2 symbol)
+    "adv": ("libref_adv.so", "clawpack46_rpn2adv_", "clawpack46_rpt2adv_"),
+    "swe": ("libref_swe.so", "clawpack46_rpn2_", "clawpack46_rpt2_"),
+    "euler": ("libref_euler.so", "clawpack46_rpn2_euler5_", "clawpack46_rpt2_euler5_"),
+}
+
+
+def available() -> bool:
+    return all(os.path.exists(os.path.join(REF_DIR, RP[k][0])) for k in RP)
+
+
+def _lib(name: str) -> ctypes.CDLL:
+    if name not in _libs:
+        _libs[name] = ctypes.CDLL(os.path.join(REF_DIR, name), mode=ctypes.RTLD_LOCAL)
+    return _libs[name]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_params(physics: str, *, grav=1.0, gamma=1.4, tperiod=4.0):
+    L = _lib(RP[physics][0])
+    if physics == "swe":
+        _cd.in_dll(L, "cparam_").value = grav
+    elif physics == "euler":
+        arr = (_cd * 2).in_dll(L, "cparam_")
+        arr[0] = gamma
+        arr[1] = gamma - 1.0
+    else:
+        pi = 4.0 * math.atan(1.0)
+        arr = (_cd * 2).in_dll(L, "compi_")
+        arr[0] = pi
+        arr[1] = 2.0 * pi
+        _cd.in_dll(L, "comvt_").value = tperiod
+
+
+def step2(physics: str, q: np.ndarray, aux: np.ndarray | None, *, mx, my, mbc, dx, dy,
+          dt, order=(2, 2), mthlim=None, xlower=0.0, ylower=0.0, t=0.0):
+    """Run CLAWPACK46_STEP2_WRAP on a copy of q (tile layout (meqn, NY, NX)).
+    Returns (q_after, cfl)."""
+    libname, rpn2_sym, rpt2_sym = RP[physics]
+    L = _lib(libname)
+    meqn = q.shape[0]
+    mwaves = meqn
+    maux = 0 if aux is None else aux.shape[0]
+    qf = np.ascontiguousarray(q, dtype=np.float64).copy()
+    auxf = (np.zeros(1) if aux is None else np.ascontiguousarray(aux, dtype=np.float64).copy())
+    maxm = max(mx, my)
+    mwork = (maxm + 2 * mbc) * (12 * meqn + (meqn + 1) * mwaves + 3 * maux + 2)
+    work = np.zeros(mwork)
+    size = meqn * (mx + 2 * mbc) * (my + 2 * mbc)
+    fp, fm, gp, gm = (np.zeros(size) for _ in range(4))
+    method = (_ci * 7)(0, order[0], order[1], 0, 0, 0, 0)
+    lim = list(mthlim) if mthlim is not None else [4] * mwaves
+    mthlim_c = (_ci * mwaves)(*lim)
+    bcc = (_ci * 4)(0, 0, 0, 0)
+    cfl = _cd(0.0)
+    ierr = _ci(0)
+    rpn2 = ctypes.cast(getattr(L, rpn2_sym), ctypes.c_void_p)
+    rpt2 = ctypes.cast(getattr(L, rpt2_sym), ctypes.c_void_p)
+    flux2 = ctypes.cast(L.clawpack46_flux2_, ctypes.c_void_p)
+    L.clawpack46_set_block_(ctypes.byref(_ci(0)))
+    L.clawpack46_step2_wrap_(
+        ctypes.byref(_ci(maxm)), ctypes.byref(_ci(meqn)), ctypes.byref(_ci(maux)),
+        ctypes.byref(_ci(mbc)), method, mthlim_c, ctypes.byref(_ci(0)),
+        ctypes.byref(_ci(mwaves)), ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)),
+        _p(qf), _p(auxf), ctypes.byref(_cd(dx)), ctypes.byref(_cd(dy)),
+        ctypes.byref(_cd(dt)), ctypes.byref(cfl), _p(work), ctypes.byref(_ci(mwork)),
+        ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_ci(0)),
+        ctypes.byref(_cd(t)), _p(fp), _p(fm), _p(gp), _p(gm), rpn2, rpt2, rpn2, rpt2,
+        flux2, bcc, ctypes.byref(ierr), ctypes.byref(_ci(0)))
+    L.clawpack46_unset_block_()
+    if ierr.value != 0:
+        raise RuntimeError(f"step2_wrap ierror={ierr.value}")
+    return qf, cfl.value
+
+
+def swirl_aux(*, mx, my, mbc, xlower, ylower, dx, dy, t, dt, tperiod):
+    L = _lib("libref_adv.so")
+    set_params("adv", tperiod=tperiod)
+    aux = np.zeros((2, my + 2 * mbc, mx + 2 * mbc))
+    q = np.zeros((1, my + 2 * mbc, mx + 2 * mbc))
+    L.clawpack46_b4step2_(
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(mbc)),
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(1)), _p(q),
+        ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_cd(dx)),
+        ctypes.byref(_cd(dy)), ctypes.byref(_cd(t)), ctypes.byref(_cd(dt)),
+        ctypes.byref(_ci(2)), _p(aux))
+    return aux
+
+
+def bc2(q: np.ndarray, *, mx, my, mbc, mthbc):
+    L = _lib("libref_swe.so")
+    meqn = q.shape[0]
+    qf = np.ascontiguousarray(q, dtype=np.float64).copy()
+    aux = np.zeros(1)
+    mb = (_ci * 4)(*mthbc)
+    L.clawpack46_bc2_default_(
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(meqn)),
+        ctypes.byref(_ci(mbc)), ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)),
+        ctypes.byref(_cd(0.0)), ctypes.byref(_cd(0.0)), ctypes.byref(_cd(1.0)),
+        ctypes.byref(_cd(1.0)), _p(qf), ctypes.byref(_ci(0)), _p(aux),
+        ctypes.byref(_cd(0.0)), ctypes.byref(_cd(0.0)), mb)
+    return qf
+
+
+def timeinterp(qcurr, qlast, alpha, *, mx, my, mbc):
+    L = _lib("libref_timeinterp.so")
+    meqn = qcurr.shape[0]
+    qc = np.ascontiguousarray(qcurr, dtype=np.float64)
+    ql = np.ascontiguousarray(qlast, dtype=np.float64)
+    qi = np.zeros_like(qc)
+    mint = mbc
+    psize = (mx * my - (mx - 2 * mint) * (my - 2 * mint)) * meqn
+    ierr = _ci(0)
+    L.fclaw2d_clawpatch46_fort_timeinterp_(
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(mbc)),
+        ctypes.byref(_ci(meqn)), ctypes.byref(_ci(psize)), _p(qc), _p(ql), _p(qi),
+        ctypes.byref(_cd(alpha)), ctypes.byref(ierr))
+    return qi, ierr.value

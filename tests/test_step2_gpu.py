@@ -1,0 +1,226 @@
+"""GPU parity of the batched patch step (libhipclaw.so, hipclaw_step2_batch)
+against the reference golden vectors and the FP64 oracle.
+
+Bar: FP64 interior and per-patch CFL bitwise equal to the reference / oracle;
+FP32 relative L1 <= 1e-5 (and L-inf reported) against the FP64 oracle;
+physical BCs bitwise.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import cases as C
+from forestclaw_amd import hipclaw as H
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FP32_REL_L1 = 1e-5
+
+SOLVERS = {"swe": (H.SOLVER_SWE_ROE, O.SOLVER_SWE_ROE),
+           "euler": (H.SOLVER_EULER5_ROE, O.SOLVER_EULER5_ROE),
+           "adv": (H.SOLVER_ADV, O.SOLVER_ADV)}
+
+
+def _t(a, dev, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(dev)
+
+
+def run_gpu(dev, kind, q, aux, dx, dy, dt, *, order=(2, 2), mthlim=None, precision=H.FP64,
+            launch=None, inplace=False, patch_ids=None, grav=1.0, gamma=1.4):
+    """q: (npatch, meqn, NY, NX) float64 numpy; returns (q_out numpy f64, cfl numpy)."""
+    import torch
+    n, meqn, NY, NX = q.shape
+    hs, _ = SOLVERS[kind]
+    p = H.physics(hs, NX - 4, NY - 4, precision=precision, order=order, mthlim=mthlim,
+                  grav=grav, gamma=gamma)
+    dt_ = torch.float64 if precision == H.FP64 else torch.float32
+    qin = _t(q, dev, dt_)
+    qout = qin if inplace else qin.clone()
+    a = None if aux is None else _t(aux, dev, dt_)
+    dxdy = _t(np.stack([np.broadcast_to(dx, (n,)), np.broadcast_to(dy, (n,))], 1).astype(
+        np.float64), dev)
+    cfl = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
+    ids = None if patch_ids is None else _t(np.asarray(patch_ids, dtype=np.int32), dev)
+    H.step2_batch(p, n, qin, qout, a, dxdy, dt, cfl, patch_ids=ids, launch=launch)
+    torch.cuda.synchronize()
+    return qout.double().cpu().numpy(), cfl.cpu().numpy()
+
+
+def _golden(prefix):
+    return sorted(glob.glob(os.path.join(GOLD, prefix + "*.npz")))
+
+
+@pytest.mark.parametrize("path", _golden("swe_step") + _golden("euler_step") + _golden("adv_step"))
+@pytest.mark.parametrize("launch", [None, (1, 1), (2, 4)])
+def test_fp64_matches_reference_golden_bitwise(dev, path, launch):
+    g = dict(np.load(path))
+    kind = os.path.basename(path).split("_")[0]
+    if kind == "euler" and launch == (2, 4):
+        launch = (1, 4)  # two columns per lane do not fit the Euler registers in FP64
+    q = g["q"][None]
+    aux = g["aux"][None] if "aux" in g else None
+    qo, cfl = run_gpu(dev, kind, q, aux, float(g["dx"]), float(g["dy"]), float(g["dt"]),
+                      order=tuple(int(x) for x in g["order"]),
+                      mthlim=[int(x) for x in g["mthlim"]], launch=launch)
+    inner = (slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(qo[0][inner], g["qout"][inner]), np.abs(qo[0][inner] - g["qout"][inner]).max()
+    # the ghost band of q_out is left as it was
+    assert np.array_equal(qo[0][:, :2, :], g["q"][:, :2, :])
+    assert cfl[0] == float(g["cfl"])
+
+
+def _batch(kind, rng, n, mx, my):
+    qs, auxs, dxs, dys, dts = [], [], [], [], []
+    for _ in range(n):
+        if kind == "swe":
+            q, dx, dy = C.swe_state(rng, mx, my, 2)
+            dts.append(C.swe_dt(q, dx, dy))
+            aux = None
+        elif kind == "euler":
+            q, dx, dy = C.euler_state(rng, mx, my, 2)
+            dts.append(C.euler_dt(q, dx, dy))
+            aux = None
+        else:
+            q, dx, dy = C.adv_state(rng, mx, my, 2)
+            aux = O.swirl_aux(mx, my, 2, rng.uniform(0, 0.5), rng.uniform(0, 0.5), dx, dy,
+                              rng.uniform(0, 3), 0.01, 8.0)
+            dts.append(C.adv_dt(aux, dx, dy))
+        # vary the mesh size per patch (levels of a batch differ)
+        scale = 2.0 ** rng.integers(0, 3)
+        qs.append(q)
+        auxs.append(aux)
+        dxs.append(dx * scale)
+        dys.append(dy * scale)
+    dt = min(d * s for d, s in zip(dts, [1] * n))
+    aux = None if auxs[0] is None else np.stack(auxs)
+    return np.stack(qs), aux, np.array(dxs), np.array(dys), dt
+
+
+def _oracle_batch(kind, q, aux, dx, dy, dt, order=(2, 2), mthlim=None):
+    _, os_ = SOLVERS[kind]
+    n, meqn, NY, NX = q.shape
+    cfg = O.make_cfg(os_, NX - 4, NY - 4, 2, order=order, mthlim=mthlim)
+    qq = q.copy()
+    cfl = O.step2_batch(cfg, qq, aux, dx, dy, dt)
+    return qq, cfl
+
+
+@pytest.mark.parametrize("kind,mx,my,n", [("swe", 32, 32, 37), ("swe", 16, 24, 11),
+                                          ("euler", 16, 16, 13), ("adv", 32, 32, 9),
+                                          ("adv", 20, 12, 7)])
+@pytest.mark.parametrize("launch", [(1, 4), (2, 2)])
+def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
+    if kind == "euler" and launch[0] == 2:
+        pytest.skip("Euler FP64 runs one column per lane")
+    rng = np.random.default_rng(100 + n)
+    q, aux, dx, dy, dt = _batch(kind, rng, n, mx, my)
+    ref, rcfl = _oracle_batch(kind, q, aux, dx, dy, dt)
+    got, cfl = run_gpu(dev, kind, q, aux, dx, dy, dt, launch=launch)
+    inner = (slice(None), slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(got[inner], ref[inner]), np.abs(got[inner] - ref[inner]).max()
+    assert np.array_equal(cfl, rcfl)
+
+
+def test_patch_ids_and_inplace(dev):
+    rng = np.random.default_rng(7)
+    q, aux, dx, dy, dt = _batch("swe", rng, 12, 32, 32)
+    ref, rcfl = _oracle_batch("swe", q, aux, dx, dy, dt)
+    # step only every other tile, in place, in a permuted order
+    ids = [10, 2, 6, 0, 8, 4]
+    got, cfl = run_gpu(dev, "swe", q, aux, dx, dy, dt, inplace=True, patch_ids=ids)
+    inner = (slice(None), slice(2, -2), slice(2, -2))
+    for t in range(12):
+        if t in ids:
+            assert np.array_equal(got[t][inner], ref[t][inner])
+            assert cfl[t] == rcfl[t]
+        else:
+            assert np.array_equal(got[t], q[t])
+            assert cfl[t] == -1.0
+
+
+@pytest.mark.parametrize("order,mthlim", [((2, 1), (1, 2, 3)), ((1, 0), (4, 4, 4)),
+                                          ((2, 2), (5, 6, 0)), ((1, 2), (4, 4, 4))])
+def test_fp64_methods_and_limiters(dev, order, mthlim):
+    rng = np.random.default_rng(sum(mthlim) + 10 * order[1])
+    q, aux, dx, dy, dt = _batch("swe", rng, 5, 32, 32)
+    ref, rcfl = _oracle_batch("swe", q, aux, dx, dy, dt, order=order, mthlim=mthlim)
+    got, cfl = run_gpu(dev, "swe", q, aux, dx, dy, dt, order=order, mthlim=mthlim)
+    inner = (slice(None), slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(got[inner], ref[inner])
+    assert np.array_equal(cfl, rcfl)
+
+
+@pytest.mark.parametrize("kind,mx,n", [("swe", 32, 64), ("euler", 16, 32), ("adv", 32, 16)])
+@pytest.mark.parametrize("launch", [None, (1, 4), (2, 4)])
+def test_fp32_error_vs_fp64_oracle(dev, kind, mx, n, launch):
+    rng = np.random.default_rng(1000 + n)
+    q, aux, dx, dy, dt = _batch(kind, rng, n, mx, mx)
+    ref, rcfl = _oracle_batch(kind, q, aux, dx, dy, dt)
+    # the FP64 reference of the FP32 path starts from the same FP32-rounded data
+    q32 = q.astype(np.float32).astype(np.float64)
+    a32 = None if aux is None else aux.astype(np.float32).astype(np.float64)
+    ref32, _ = _oracle_batch(kind, q32, a32, dx, dy, dt)
+    got, cfl = run_gpu(dev, kind, q, aux, dx, dy, dt, precision=H.FP32, launch=launch)
+    inner = (slice(None), slice(None), slice(2, -2), slice(2, -2))
+    d = got[inner] - ref32[inner]
+    rel_l1 = np.abs(d).sum() / np.abs(ref32[inner]).sum()
+    rel_linf = np.abs(d).max() / np.abs(ref32[inner]).max()
+    print(f"\nFP32 {kind} rel L1 {rel_l1:.3e}  rel Linf {rel_linf:.3e}")
+    assert rel_l1 <= FP32_REL_L1
+    assert rel_linf <= 1e-3
+    np.testing.assert_allclose(cfl, rcfl, rtol=1e-5)
+
+
+def test_bc2_bitwise(dev):
+    import torch
+    rng = np.random.default_rng(21)
+    n, mx, my = 6, 12, 10
+    q = rng.standard_normal((n, 3, my + 4, mx + 4))
+    bcs = np.array([[1, 1, 1, 1], [3, 3, 3, 3], [1, 3, -1, 3], [-1, -1, 1, 3],
+                    [3, -1, 3, -1], [-1, -1, -1, -1]], dtype=np.int32)
+    ref = np.stack([O.bc2(q[k], mx, my, 2, list(bcs[k])) for k in range(n)])
+    for prec, dt_ in [(H.FP64, torch.float64), (H.FP32, torch.float32)]:
+        p = H.physics(H.SOLVER_SWE_ROE, mx, my, precision=prec)
+        qt = _t(q, dev, dt_)
+        H.bc2_batch(p, n, qt, _t(bcs, dev))
+        torch.cuda.synchronize()
+        got = qt.double().cpu().numpy()
+        exp = ref if prec == H.FP64 else np.stack(
+            [O.bc2(q[k].astype(np.float32).astype(np.float64), mx, my, 2, list(bcs[k]))
+             for k in range(n)])
+        assert np.array_equal(got, exp)
+
+
+def test_swirl_aux_device(dev):
+    import torch
+    mx = my = 32
+    n = 4
+    xy = np.array([[0.0, 0.0], [0.25, 0.5], [0.5, 0.125], [0.875, 0.875]])
+    dxdy = np.full((n, 2), 1.0 / 256)
+    p = H.physics(H.SOLVER_ADV, mx, my)
+    aux = torch.zeros((n, 2, my + 4, mx + 4), dtype=torch.float64, device=dev)
+    H.swirl_aux_batch(p, n, _t(xy, dev), _t(dxdy, dev), 1.3, 0.002, 8.0, aux)
+    torch.cuda.synchronize()
+    got = aux.cpu().numpy()
+    for k in range(n):
+        ref = O.swirl_aux(mx, my, 2, xy[k, 0], xy[k, 1], dxdy[k, 0], dxdy[k, 1], 1.3, 0.002, 8.0)
+        scale = np.abs(ref).max()
+        assert np.abs(got[k] - ref).max() <= 1e-14 * scale
+
+
+def test_max_reduce_exact(dev):
+    import torch
+    rng = np.random.default_rng(3)
+    for n in [1, 5, 1023, 1024, 1025, 70001]:
+        x = rng.uniform(0, 2, n)
+        out = torch.zeros(1, dtype=torch.float64, device=dev)
+        H.max_reduce(_t(x, dev), n, out)
+        torch.cuda.synchronize()
+        assert out.item() == x.max()
