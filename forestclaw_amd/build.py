@@ -32,6 +32,9 @@ SOURCES = {
     "hc_api.hip": STRICT,
     "hc_step2_fp64.hip": STRICT,
     "hc_step2_fp32.hip": FAST,
+    "hc_amr_kernels.hip": STRICT,
+    "hc_ctx.hip": STRICT,
+    "hc_forest.cpp": STRICT,
 }
 
 

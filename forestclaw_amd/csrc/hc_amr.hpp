@@ -1,0 +1,58 @@
+// hc_amr.hpp -- device data structures of the AMR driver: ghost-fill
+// operations, physical-BC and time-interpolation lists, regrid transfers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hc {
+
+constexpr int MAXL = 22;  // levels 0..21
+
+// Per-level current solution buffers plus the shared time-interpolated pool.
+// A slot index addresses the same tile position in every pool.
+template <class R> struct Pools {
+    R* q[MAXL];   // current q of level L (one of two ping-pong pools)
+    R* qlast[MAXL];
+    R* ti;        // time-interpolated data (coarse level of a subcycle)
+};
+
+enum GhostKind {
+    G_COPY_FACE = 0,     // fclaw2d_clawpatch46_fort_copy_face
+    G_COPY_CORNER = 1,   // fclaw2d_clawpatch46_fort_copy_corner
+    G_AVG_FACE = 2,      // fclaw2d_clawpatch46_fort_average_face
+    G_AVG_CORNER = 3,    // fclaw2d_clawpatch46_fort_average_corner
+    G_INTERP_FACE = 4,   // fclaw2d_clawpatch46_fort_interpolate_face
+    G_INTERP_CORNER = 5  // fclaw2d_clawpatch46_fort_interpolate_corner
+};
+
+// One patch-pair exchange.  For COPY/AVG the destination is the patch whose
+// ghost cells are written (this patch / coarse patch) and sub its face or
+// corner; for INTERP the destination is the fine patch and sub is the face
+// or corner of the coarse (source) patch.  (ox, oy) is the index offset
+// between the two tiles: same size, src = dst + (ox, oy); across levels,
+// fine = 2*coarse - mbc + (ox, oy) in 0-based tile indices.
+struct GhostOp {
+    int kind, sub;
+    int dst, src;       // slots
+    int dlev, slev;     // levels (select pools)
+    int ox, oy;
+    int flags;          // bit0: dst uses the time-interpolated pool, bit1: src does
+};
+
+struct BcOp {           // physical boundary conditions on one patch
+    int slot, level, flags;  // flags bit0: time-interpolated pool
+    int mthbc[4];
+};
+
+struct TiOp {           // time interpolation of one coarse patch
+    int slot, level;
+};
+
+struct XferOp {         // regrid transfer into the new pools
+    int kind;           // 0 keep (copy), 1 interpolate2fine, 2 average2coarse
+    int dst;            // new slot
+    int src[4];         // old slots (keep/refine: [0])
+    int srclev;         // level of the old data (selects its current pool)
+    int igrid;
+};
+
+}  // namespace hc

@@ -1,0 +1,529 @@
+// hc_amr_kernels.hip -- device kernels of the AMR driver (gfx950):
+// ghost fill, physical BCs on lists, time interpolation, regrid transfers,
+// tagging, initial conditions and conservation sums.  Built with
+// -ffp-contract=off so that the FP64 instantiations round like the
+// reference's FP64 Fortran (copies and averages are then bitwise equal).
+//
+// Tile layout: q[(m*NY + j)*NX + i], NX = mx+4, NY = my+4 (mbc = 2); the
+// 0-based tile index i corresponds to the reference's i_f = i - 1.
+#include "hc_amr.hpp"
+#include "hc_amr_launch.hpp"
+#include "hc_common.hpp"
+
+namespace hc {
+
+constexpr int MBC = 2;
+
+__device__ inline int floordiv2(int a) { return a >= 0 ? a / 2 : -((1 - a) / 2); }
+
+// Fortran sign(1, x): +1 or -1 with the sign bit of x
+__device__ inline double sign1(double x) { return __builtin_copysign(1.0, x); }
+__device__ inline float sign1(float x) { return __builtin_copysignf(1.0f, x); }
+
+template <class R> __device__ inline R* pool_ptr(const Pools<R>& P, int level, bool ti)
+{
+    return ti ? P.ti : P.q[level];
+}
+
+// AMRClaw slope limiter used for ghost interpolation (mth = 5,
+// src/patches/clawpatch/fclaw2d_clawpatch_utils.f:52-55):
+// min(2|sl|, 2|sr|, |sc|) * max(0, sign(1, sl*sr)) * sign(1, sc)
+template <class R> __device__ inline R amr_slope(R sl, R sr)
+{
+    R sc = (sl + sr) / R(2);
+    R a = R(2) * M<R>::abs_(sl), b = R(2) * M<R>::abs_(sr), c = M<R>::abs_(sc);
+    R mn = dmin(dmin(a, b), c);
+    R f = dmax(R(0), sign1(sl * sr));
+    return (mn * f) * sign1(sc);
+}
+
+// ---------------------------------------------------------------------------
+// Ghost fill.  One thread per (operation, item); an item is one ghost cell
+// (copy / average) or one coarse cell whose four children are written
+// (interpolation).  All meqn fields are handled by the thread.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_ghost(const GhostOp* __restrict__ ops, int nops, Pools<R> P, int mx, int my,
+                        int meqn)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const long long T = (long long)NX * NY;
+    const int items = MBC * (mx > my ? mx : my);
+    const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= (long long)nops * items) return;
+    const int o = (int)(gid / items);
+    const int k = (int)(gid - (long long)o * items);
+    const GhostOp op = ops[o];
+    R* __restrict__ D = pool_ptr(P, op.dlev, op.flags & 1) + (long long)op.dst * T * meqn;
+    const R* __restrict__ S = pool_ptr(P, op.slev, op.flags & 2) + (long long)op.src * T * meqn;
+#define AT(B, i, j, m) B[((long long)(m) * NY + (j)) * NX + (i)]
+
+    if (op.kind == G_COPY_FACE || op.kind == G_AVG_FACE) {
+        const int f = op.sub;
+        const int len = f < 2 ? my : mx;
+        if (k >= MBC * len) return;
+        int i, j;
+        const int layer = k % MBC, along = MBC + k / MBC;
+        switch (f) {
+        case 0: i = layer; j = along; break;
+        case 1: i = MBC + mx + layer; j = along; break;
+        case 2: j = layer; i = along; break;
+        default: j = MBC + my + layer; i = along; break;
+        }
+        if (op.kind == G_COPY_FACE) {
+            for (int m = 0; m < meqn; m++) AT(D, i, j, m) = AT(S, i + op.ox, j + op.oy, m);
+        } else {
+            const int fi = 2 * i - MBC + op.ox, fj = 2 * j - MBC + op.oy;
+            if (fi < MBC || fi + 1 > MBC + mx - 1 || fj < MBC || fj + 1 > MBC + my - 1) return;
+            for (int m = 0; m < meqn; m++) {
+                R s = R(0);
+                s = s + AT(S, fi, fj, m);
+                s = s + AT(S, fi + 1, fj, m);
+                s = s + AT(S, fi, fj + 1, m);
+                s = s + AT(S, fi + 1, fj + 1, m);
+                AT(D, i, j, m) = s / R(4);
+            }
+        }
+        return;
+    }
+    if (op.kind == G_COPY_CORNER || op.kind == G_AVG_CORNER) {
+        if (k >= MBC * MBC) return;
+        const int ibc = k % MBC + 1, jbc = k / MBC + 1;
+        const int c = op.sub;
+        const int i = (c & 1) ? MBC + mx - 1 + ibc : MBC - ibc;
+        const int j = (c & 2) ? MBC + my - 1 + jbc : MBC - jbc;
+        if (op.kind == G_COPY_CORNER) {
+            for (int m = 0; m < meqn; m++) AT(D, i, j, m) = AT(S, i + op.ox, j + op.oy, m);
+        } else {
+            const int fi = 2 * i - MBC + op.ox, fj = 2 * j - MBC + op.oy;
+            for (int m = 0; m < meqn; m++) {
+                R s = R(0);
+                s = s + AT(S, fi, fj, m);
+                s = s + AT(S, fi + 1, fj, m);
+                s = s + AT(S, fi, fj + 1, m);
+                s = s + AT(S, fi + 1, fj + 1, m);
+                AT(D, i, j, m) = s / R(4);
+            }
+        }
+        return;
+    }
+    // interpolation: coarse source cell (ci, cj), fine destination children
+    int ci, cj;
+    bool check;
+    if (op.kind == G_INTERP_FACE) {
+        const int f = op.sub;
+        const int len = f < 2 ? my : mx;
+        if (k >= len) return;
+        switch (f) {
+        case 0: ci = MBC; cj = MBC + k; break;
+        case 1: ci = MBC + mx - 1; cj = MBC + k; break;
+        case 2: cj = MBC; ci = MBC + k; break;
+        default: cj = MBC + my - 1; ci = MBC + k; break;
+        }
+        check = true;
+    } else {
+        if (k >= 1) return;
+        const int c = op.sub;
+        ci = (c & 1) ? MBC + mx - 1 : MBC;
+        cj = (c & 2) ? MBC + my - 1 : MBC;
+        check = false;
+    }
+    const int fi = 2 * ci - MBC + op.ox, fj = 2 * cj - MBC + op.oy;
+    if (check && (fi < 0 || fi + 1 > NX - 1 || fj < 0 || fj + 1 > NY - 1)) return;
+    for (int m = 0; m < meqn; m++) {
+        const R qc = AT(S, ci, cj, m);
+        R gx = amr_slope(qc - AT(S, ci - 1, cj, m), AT(S, ci + 1, cj, m) - qc);
+        R gy = amr_slope(qc - AT(S, ci, cj - 1, m), AT(S, ci, cj + 1, m) - qc);
+        const R lo = R(-0.25), hi = R(0.25);
+        AT(D, fi, fj, m) = (qc + gx * lo) + gy * lo;
+        AT(D, fi + 1, fj, m) = (qc + gx * hi) + gy * lo;
+        AT(D, fi, fj + 1, m) = (qc + gx * lo) + gy * hi;
+        AT(D, fi + 1, fj + 1, m) = (qc + gx * hi) + gy * hi;
+    }
+#undef AT
+}
+
+// ---------------------------------------------------------------------------
+// clawpack46_bc2_default on a list of patches (left/right over the full
+// height, then bottom/top over the full width).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_bc_list(const BcOp* __restrict__ ops, int nops, Pools<R> P, int mx, int my,
+                          int meqn)
+{
+    const int o = blockIdx.x;
+    if (o >= nops) return;
+    const BcOp op = ops[o];
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    R* qp = pool_ptr(P, op.level, op.flags & 1) + (long long)op.slot * NX * NY * meqn;
+    const int i1 = MBC, imx = mx + MBC - 1, j1 = MBC, jmy = my + MBC - 1;
+#define QB(i, j, m) qp[((long long)(m) * NY + (j)) * NX + (i)]
+    for (int t = threadIdx.x; t < NY * meqn; t += blockDim.x) {
+        const int m = t / NY, j = t - m * NY;
+        for (int ibc = 1; ibc <= MBC; ibc++) {
+            if (op.mthbc[0] == 1) QB(i1 - ibc, j, m) = QB(i1, j, m);
+            else if (op.mthbc[0] == 3) QB(i1 - ibc, j, m) = (m == 1) ? -QB(i1 + ibc - 1, j, m) : QB(i1 + ibc - 1, j, m);
+            if (op.mthbc[1] == 1) QB(imx + ibc, j, m) = QB(imx, j, m);
+            else if (op.mthbc[1] == 3) QB(imx + ibc, j, m) = (m == 1) ? -QB(imx + 1 - ibc, j, m) : QB(imx + 1 - ibc, j, m);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NX * meqn; t += blockDim.x) {
+        const int m = t / NX, i = t - m * NX;
+        for (int jbc = 1; jbc <= MBC; jbc++) {
+            if (op.mthbc[2] == 1) QB(i, j1 - jbc, m) = QB(i, j1, m);
+            else if (op.mthbc[2] == 3) QB(i, j1 - jbc, m) = (m == 2) ? -QB(i, j1 + jbc - 1, m) : QB(i, j1 + jbc - 1, m);
+            if (op.mthbc[3] == 1) QB(i, jmy + jbc, m) = QB(i, jmy, m);
+            else if (op.mthbc[3] == 3) QB(i, jmy + jbc, m) = (m == 2) ? -QB(i, jmy + 1 - jbc, m) : QB(i, jmy + 1 - jbc, m);
+        }
+    }
+#undef QB
+}
+
+// ---------------------------------------------------------------------------
+// Time interpolation on the interior band of width mbc
+// (src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:33-69).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_timeinterp(const TiOp* __restrict__ ops, int nops, Pools<R> P, int mx, int my,
+                             int meqn, R alpha)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const long long cells = (long long)mx * my;
+    const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= nops * cells) return;
+    const int o = (int)(gid / cells);
+    const int c = (int)(gid - (long long)o * cells);
+    const int ii = c % mx, jj = c / mx;  // 0-based interior indices
+    if (!(ii < MBC || ii >= mx - MBC || jj < MBC || jj >= my - MBC)) return;
+    const TiOp op = ops[o];
+    const long long base = (long long)op.slot * NX * NY * meqn;
+    const R* qc = P.q[op.level] + base;
+    const R* ql = P.qlast[op.level] + base;
+    R* qi = P.ti + base;
+    for (int m = 0; m < meqn; m++) {
+        const long long idx = ((long long)m * NY + jj + MBC) * NX + ii + MBC;
+        qi[idx] = ql[idx] + alpha * (qc[idx] - ql[idx]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Regrid transfers into new pools: keep (copy whole tile), interpolate2fine
+// (fclaw2d_clawpatch46_interpolate.f:324-400) and average2coarse
+// (fclaw2d_clawpatch46_average.f:282-364).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_xfer(const XferOp* __restrict__ ops, int nops, Pools<R> Old, R* __restrict__ qnew,
+                       int mx, int my, int meqn)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const long long T = (long long)NX * NY;
+    const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= nops * T) return;
+    const int o = (int)(gid / T);
+    const int cell = (int)(gid - (long long)o * T);
+    const int i = cell % NX, j = cell / NX;
+    const XferOp op = ops[o];
+    R* D = qnew + (long long)op.dst * T * meqn;
+#define AT(B, i, j, m) B[((long long)(m) * NY + (j)) * NX + (i)]
+    if (op.kind == 0) {
+        const R* S = Old.q[op.srclev] + (long long)op.src[0] * T * meqn;
+        for (int m = 0; m < meqn; m++) AT(D, i, j, m) = AT(S, i, j, m);
+        return;
+    }
+    const int ig = op.igrid & 1, jg = op.igrid >> 1;
+    if (op.kind == 1) {
+        // reference loops: coarse-local i in [1-ig, mx/2+1-ig], child ii in
+        // {1,2}, fine index (i-1)*2 + ii (1-based); invert for this fine cell
+        const int fi_f = i - MBC + 1, fj_f = j - MBC + 1;
+        const int icl = floordiv2(fi_f - 1) + 1, jcl = floordiv2(fj_f - 1) + 1;
+        const int ii = fi_f - (icl - 1) * 2, jj = fj_f - (jcl - 1) * 2;
+        if (icl < 1 - ig || icl > mx / 2 + 1 - ig || jcl < 1 - jg || jcl > my / 2 + 1 - jg) return;
+        const int ci = icl + ig * (mx / 2) + MBC - 1, cj = jcl + jg * (my / 2) + MBC - 1;
+        const R* S = Old.q[op.srclev] + (long long)op.src[0] * T * meqn;
+        const R sx = (R(ii) - R(1.0) - R(0.5)) / R(2), sy = (R(jj) - R(1.0) - R(0.5)) / R(2);
+        for (int m = 0; m < meqn; m++) {
+            const R qc = AT(S, ci, cj, m);
+            R gx = amr_slope(qc - AT(S, ci - 1, cj, m), AT(S, ci + 1, cj, m) - qc);
+            R gy = amr_slope(qc - AT(S, ci, cj - 1, m), AT(S, ci, cj + 1, m) - qc);
+            AT(D, i, j, m) = (qc + sx * gx) + sy * gy;
+        }
+        return;
+    }
+    // average2coarse: coarse interior (i_f + ic_add, j_f + jc_add), i_f in [1, mx/2]
+    const int ci_f = i - MBC + 1, cj_f = j - MBC + 1;
+    if (ci_f < 1 || ci_f > mx || cj_f < 1 || cj_f > my) return;
+    const int g = ((ci_f > mx / 2) ? 1 : 0) + 2 * ((cj_f > my / 2) ? 1 : 0);
+    const int il = ci_f - (g & 1) * (mx / 2), jl = cj_f - (g >> 1) * (mx / 2);
+    const R* S = Old.q[op.srclev] + (long long)op.src[g] * T * meqn;
+    for (int m = 0; m < meqn; m++) {
+        R s = R(0);
+        const int f0 = (il - 1) * 2 + 1 + MBC - 1, g0 = (jl - 1) * 2 + 1 + MBC - 1;
+        s = s + AT(S, f0, g0, m);
+        s = s + AT(S, f0 + 1, g0, m);
+        s = s + AT(S, f0, g0 + 1, m);
+        s = s + AT(S, f0 + 1, g0 + 1, m);
+        AT(D, i, j, m) = s / R(4);
+    }
+#undef AT
+}
+
+// ---------------------------------------------------------------------------
+// Gradient tagging (fclaw2d_clawpatch_gradient_exceeds_th.f90:47-109 on the
+// interior cells, fclaw2d_clawpatch46_tag4refinement.f:33-74).  One block
+// per patch; tag = 1 if any interior cell exceeds the threshold.
+// mode 0: refine test -> tags[p] = exceeds;  mode 1: coarsen test for the
+// patch -> tags[p] = !exceeds (a family coarsens only if all four agree).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_tag_gradient(const int* __restrict__ slots, const int* __restrict__ levels,
+                               int n, Pools<R> P, const double* __restrict__ dxdy, int mx, int my,
+                               int meqn, int ivar, double threshold, int mode, int* __restrict__ tags)
+{
+    const int p = blockIdx.x;
+    if (p >= n) return;
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const int slot = slots[p];
+    const R* q = P.q[levels[p]] + ((long long)slot * meqn + ivar) * NX * NY;
+    const double dx2 = 2 * dxdy[2 * slot], dy2 = 2 * dxdy[2 * slot + 1];
+    int hit = 0;
+    for (int c = threadIdx.x; c < mx * my; c += blockDim.x) {
+        const int i = c % mx + MBC, j = c / mx + MBC;
+        const double dqx = ((double)q[j * NX + i + 1] - (double)q[j * NX + i - 1]) / dx2;
+        const double dqy = ((double)q[(j + 1) * NX + i] - (double)q[(j - 1) * NX + i]) / dy2;
+        const double ds = sqrt(dqx * dqx + dqy * dqy + 0.0 * 0.0);
+        if (ds > threshold) hit = 1;
+    }
+    hit = __syncthreads_or(hit);
+    if (threadIdx.x == 0) tags[p] = mode == 0 ? hit : !hit;
+}
+
+// ---------------------------------------------------------------------------
+// Initial conditions (the applications' qinit hooks) on cell centres.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_qinit(int kind, QinitParams prm, const int* __restrict__ slots,
+                        const int* __restrict__ levels, int n, Pools<R> P,
+                        const double* __restrict__ xylower, const double* __restrict__ dxdy,
+                        int mx, int my, int meqn)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const long long T = (long long)NX * NY;
+    const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= n * T) return;
+    const int p = (int)(gid / T);
+    const int cell = (int)(gid - (long long)p * T);
+    const int i = cell % NX, j = cell / NX;
+    const int slot = slots[p];
+    R* q = P.q[levels[p]] + (long long)slot * T * meqn;
+    const double dx = dxdy[2 * slot], dy = dxdy[2 * slot + 1];
+    const double xlo = xylower[2 * slot] + (i - MBC) * dx, ylo = xylower[2 * slot + 1] + (j - MBC) * dy;
+    const double xc = xlo + 0.5 * dx, yc = ylo + 0.5 * dy;
+    double v[5] = {0, 0, 0, 0, 0};
+    const int S = prm.nsub > 0 ? prm.nsub : 1;
+    switch (kind) {
+    case QINIT_SWIRL:  // q = 1 for x < 0.5 (swirl/user_4.6/qinit.f:13-25)
+        v[0] = xc < prm.p[0] ? 1.0 : 0.0;
+        break;
+    case QINIT_RADIAL_DAM: {  // h = hin inside r0, hout outside; u = v = 0
+        int inside = 0;
+        for (int b = 0; b < S; b++)
+            for (int a = 0; a < S; a++) {
+                const double x = xlo + (a + 0.5) * dx / S - prm.p[0];
+                const double y = ylo + (b + 0.5) * dy / S - prm.p[1];
+                inside += (x * x + y * y < prm.p[2] * prm.p[2]);
+            }
+        const double fr = (double)inside / (S * S);
+        v[0] = prm.p[3] * fr + prm.p[4] * (1.0 - fr);
+        break;
+    }
+    case QINIT_HUMPS: {  // h = h0 + sum A exp(-|x-c|^2/w^2), at rest
+        double h = prm.p[0];
+        const int nh = (int)prm.p[1];
+        for (int k = 0; k < nh && k < 12; k++) {
+            const double cx = prm.p[2 + 4 * k], cy = prm.p[3 + 4 * k];
+            const double A = prm.p[4 + 4 * k], w = prm.p[5 + 4 * k];
+            const double r2 = (xc - cx) * (xc - cx) + (yc - cy) * (yc - cy);
+            h += A * exp(-r2 / (w * w));
+        }
+        v[0] = h;
+        break;
+    }
+    case QINIT_SHOCK_BUBBLE: {
+        // p: gamma, xshock, rinf, vinf, pinf, x0, y0, r0, rhob
+        const double g1 = prm.p[0] - 1.0;
+        double acc[5] = {0, 0, 0, 0, 0};
+        for (int b = 0; b < S; b++)
+            for (int a = 0; a < S; a++) {
+                const double x = xlo + (a + 0.5) * dx / S, y = ylo + (b + 0.5) * dy / S;
+                double rho = 1.0, u = 0.0, pr = 1.0, tr = 0.0;
+                if (x < prm.p[1]) {
+                    rho = prm.p[2];
+                    u = prm.p[3];
+                    pr = prm.p[4];
+                } else {
+                    const double rx = x - prm.p[5], ry = y - prm.p[6];
+                    if (rx * rx + ry * ry < prm.p[7] * prm.p[7]) {
+                        rho = prm.p[8];
+                        tr = 1.0;
+                    }
+                }
+                acc[0] += rho;
+                acc[1] += rho * u;
+                acc[3] += pr / g1 + 0.5 * rho * u * u;
+                acc[4] += rho * tr;
+            }
+        for (int m = 0; m < 5; m++) v[m] = acc[m] / (S * S);
+        break;
+    }
+    default:
+        v[0] = prm.p[0];
+        break;
+    }
+    for (int m = 0; m < meqn; m++) q[((long long)m * NY + j) * NX + i] = R(v[m]);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 conservation sums per patch: sum over interior of q*dx*dy with
+// compensated accumulation per thread
+// (fclaw2d_clawpatch46_diagnostics.f:13-57), then a fixed-order tree.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_mass(const int* __restrict__ slots, const int* __restrict__ levels, int n,
+                       Pools<R> P, const double* __restrict__ dxdy, int mx, int my, int meqn,
+                       double* __restrict__ out)
+{
+    __shared__ double red[256];
+    const int p = blockIdx.x;
+    if (p >= n) return;
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const int slot = slots[p];
+    const double area = dxdy[2 * slot] * dxdy[2 * slot + 1];
+    for (int m = 0; m < meqn; m++) {
+        const R* q = P.q[levels[p]] + ((long long)slot * meqn + m) * NX * NY;
+        double s = 0.0, c = 0.0;
+        for (int k = threadIdx.x; k < mx * my; k += blockDim.x) {
+            const int i = k % mx + MBC, j = k / mx + MBC;
+            const double y = (double)q[j * NX + i] * area - c;
+            const double t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[(long long)p * meqn + m] = red[0];
+        __syncthreads();
+    }
+}
+
+template <class R>
+__global__ void k_copy_tiles(const int* __restrict__ slots, int n, const R* __restrict__ src,
+                             R* __restrict__ dst, long long tile)
+{
+    const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (gid >= n * tile) return;
+    const int p = (int)(gid / tile);
+    const long long off = (long long)slots[p] * tile + (gid - (long long)p * tile);
+    dst[off] = src[off];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+template <class R>
+void launch_ghost(const GhostOp* ops, int nops, const Pools<R>& P, int mx, int my, int meqn,
+                  hipStream_t s)
+{
+    if (nops <= 0) return;
+    const long long n = (long long)nops * MBC * (mx > my ? mx : my);
+    hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, ops, nops, P, mx, my, meqn);
+}
+
+template <class R>
+void launch_bc(const BcOp* ops, int nops, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
+{
+    if (nops <= 0) return;
+    hipLaunchKernelGGL(k_bc_list<R>, dim3(nops), dim3(64), 0, s, ops, nops, P, mx, my, meqn);
+}
+
+template <class R>
+void launch_timeinterp(const TiOp* ops, int nops, const Pools<R>& P, int mx, int my, int meqn,
+                       double alpha, hipStream_t s)
+{
+    if (nops <= 0) return;
+    const long long n = (long long)nops * mx * my;
+    hipLaunchKernelGGL(k_timeinterp<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, ops, nops, P, mx, my,
+                       meqn, R(alpha));
+}
+
+template <class R>
+void launch_xfer(const XferOp* ops, int nops, const Pools<R>& Old, R* qnew, int mx, int my, int meqn,
+                 hipStream_t s)
+{
+    if (nops <= 0) return;
+    const long long n = (long long)nops * (mx + 4) * (my + 4);
+    hipLaunchKernelGGL(k_xfer<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, ops, nops, Old, qnew, mx, my,
+                       meqn);
+}
+
+template <class R>
+void launch_tag(const int* slots, const int* levels, int n, const Pools<R>& P, const double* dxdy,
+                int mx, int my, int meqn, int ivar, double thr, int mode, int* tags, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tag_gradient<R>, dim3(n), dim3(256), 0, s, slots, levels, n, P, dxdy, mx, my,
+                       meqn, ivar, thr, mode, tags);
+}
+
+template <class R>
+void launch_qinit(int kind, const QinitParams& prm, const int* slots, const int* levels, int n,
+                  const Pools<R>& P, const double* xylower, const double* dxdy, int mx, int my,
+                  int meqn, hipStream_t s)
+{
+    if (n <= 0) return;
+    const long long tot = (long long)n * (mx + 4) * (my + 4);
+    hipLaunchKernelGGL(k_qinit<R>, dim3(nblocks(tot, 256)), dim3(256), 0, s, kind, prm, slots, levels,
+                       n, P, xylower, dxdy, mx, my, meqn);
+}
+
+template <class R>
+void launch_mass(const int* slots, const int* levels, int n, const Pools<R>& P, const double* dxdy,
+                 int mx, int my, int meqn, double* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_mass<R>, dim3(n), dim3(256), 0, s, slots, levels, n, P, dxdy, mx, my, meqn, out);
+}
+
+template <class R>
+void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long tile, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_tiles<R>, dim3(nblocks((long long)n * tile, 256)), dim3(256), 0, s, slots,
+                       n, src, dst, tile);
+}
+
+#define HC_INST(R)                                                                                \
+    template void launch_ghost<R>(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t); \
+    template void launch_bc<R>(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);       \
+    template void launch_timeinterp<R>(const TiOp*, int, const Pools<R>&, int, int, int, double,     \
+                                       hipStream_t);                                              \
+    template void launch_xfer<R>(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t); \
+    template void launch_tag<R>(const int*, const int*, int, const Pools<R>&, const double*, int, int, \
+                                int, int, double, int, int*, hipStream_t);                        \
+    template void launch_qinit<R>(int, const QinitParams&, const int*, const int*, int,             \
+                                  const Pools<R>&, const double*, const double*, int, int, int,   \
+                                  hipStream_t);                                                   \
+    template void launch_mass<R>(const int*, const int*, int, const Pools<R>&, const double*, int,  \
+                                 int, int, double*, hipStream_t);                                 \
+    template void launch_copy_tiles<R>(const int*, int, const R*, R*, long long, hipStream_t);
+
+HC_INST(double)
+HC_INST(float)
+
+}  // namespace hc

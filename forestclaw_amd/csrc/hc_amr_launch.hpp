@@ -1,0 +1,38 @@
+// hc_amr_launch.hpp -- host launchers of the AMR kernels (hc_amr_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "hc_amr.hpp"
+
+namespace hc {
+
+enum QinitKind {
+    QINIT_CONST = 0,        // q = p[0] (field 0), 0 elsewhere
+    QINIT_SWIRL = 1,        // swirl/user_4.6/qinit.f: q = 1 for x < p[0]
+    QINIT_RADIAL_DAM = 2,   // radialdam: h = p[3] inside circle (p[0],p[1]) radius p[2], else p[4]
+    QINIT_HUMPS = 3,        // h = p[0] + sum of p[1] Gaussians (cx, cy, A, w) from p[2]
+    QINIT_SHOCK_BUBBLE = 4  // gamma, xshock, rho_inf, u_inf, p_inf, x0, y0, r0, rho_bubble
+};
+
+struct QinitParams {
+    double p[56];
+    int nsub;  // sub-samples per direction for cell averages of discontinuous data
+};
+
+template <class R> void launch_ghost(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t);
+template <class R> void launch_bc(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);
+template <class R>
+void launch_timeinterp(const TiOp*, int, const Pools<R>&, int, int, int, double, hipStream_t);
+template <class R> void launch_xfer(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t);
+template <class R>
+void launch_tag(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, int,
+                double, int, int*, hipStream_t);
+template <class R>
+void launch_qinit(int, const QinitParams&, const int*, const int*, int, const Pools<R>&,
+                  const double*, const double*, int, int, int, hipStream_t);
+template <class R>
+void launch_mass(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double*,
+                 hipStream_t);
+template <class R> void launch_copy_tiles(const int*, int, const R*, R*, long long, hipStream_t);
+
+}  // namespace hc

@@ -1,0 +1,779 @@
+// hc_ctx.hip -- the device-resident AMR driver behind the hipclaw_ctx_* C ABI.
+//
+// Host-side restatement of the ForestClaw drivers that sit around the patch
+// step, organised for the GPU: every phase of a ghost update, every level
+// update and every regrid transfer is one batched launch over a precomputed
+// device list instead of one callback per patch or patch pair.
+//   ghost update   src/fclaw2d_ghost_fill.c:611-856 (phase order), with the
+//                  per-pair dispatch of src/fclaw2d_face_neighbors.c:195-450 and
+//                  src/fclaw2d_corner_neighbors.c:334-535 turned into plans
+//   level update   src/fclaw2d_update_single_step.c:44-103
+//   subcycling     src/fclaw2d_advance.c:54-357
+//   time interp    src/fclaw2d_timeinterp.c:30-57
+//   regrid         src/fclaw2d_regrid.c:45-335 (tagging, adapt, repopulate)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "hc_ctx.hpp"
+#include "hc_internal.hpp"
+
+namespace hc {
+
+template <class T> int DevArray<T>::upload(const std::vector<T>& v)
+{
+    free_();
+    n = (int)v.size();
+    if (n == 0) return HIPCLAW_OK;
+    if (hipMalloc(&p, sizeof(T) * n) != hipSuccess) {
+        p = nullptr;
+        n = 0;
+        return set_error(HIPCLAW_ERR_NOMEM, "hipMalloc failed for a plan");
+    }
+    return check_hip(hipMemcpy(p, v.data(), sizeof(T) * n, hipMemcpyHostToDevice), "plan upload");
+}
+
+Ctx::Ctx(const hipclaw_physics& p, const hipclaw_domain& d)
+    : phys(p), dom(d), forest(d.mi, d.mj, d.minlevel, d.maxlevel)
+{
+    esz = p.precision == HIPCLAW_FP64 ? 8 : 4;
+    const long long T = (long long)(p.mx + 4) * (p.my + 4);
+    tile = T * p.meqn;
+    atile = T * p.maux;
+}
+
+Ctx::~Ctx()
+{
+    release_pools();
+}
+
+void Ctx::release_pools()
+{
+    for (auto& b : pool) {
+        if (b) (void)hipFree(b);
+        b = nullptr;
+    }
+    void** bufs[] = {&ti, &save, &aux};
+    for (auto b : bufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass};
+    for (auto b : dbufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    int** ibufs[] = {&d_levels, &d_iota, &d_tags};
+    for (auto b : ibufs) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
+    for (auto& lp : plan) {
+        lp.avg.free_(); lp.avg_ti.free_(); lp.copy.free_(); lp.copy_ti.free_();
+        lp.interp.free_(); lp.interp_ti.free_(); lp.bc.free_(); lp.bc_ti.free_(); lp.ti.free_();
+    }
+    n = cap = 0;
+}
+
+template <class R> Pools<R> Ctx::pools() const
+{
+    Pools<R> P{};
+    for (int L = 0; L < MAXL; L++) {
+        P.q[L] = static_cast<R*>(pool[parity[L]]);
+        P.qlast[L] = static_cast<R*>(pool[parity[L] ^ 1]);
+    }
+    P.ti = static_cast<R*>(ti);
+    return P;
+}
+
+int Ctx::level_range(int level, int* s0, int* s1) const
+{
+    if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    *s0 = plan[level].s0;
+    *s1 = plan[level].s1;
+    return HIPCLAW_OK;
+}
+
+#define HC_TRY(x)                   \
+    do {                            \
+        int rc_ = (x);              \
+        if (rc_) return rc_;        \
+    } while (0)
+
+static int dmalloc(void** p, size_t bytes)
+{
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return set_error(HIPCLAW_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return check_hip(hipMemsetAsync(*p, 0, bytes, nullptr), "hipMemset");
+}
+
+// Build pools, geometry and ghost plans for the current forest.  With a
+// transfer list the previous pools are repopulated into the new ones.
+int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
+{
+    const int nn = forest.nleaves();
+    const int mx = phys.mx, my = phys.my;
+    void* npool[2] = {nullptr, nullptr};
+    void* nti = nullptr;
+    void* naux = nullptr;
+    HC_TRY(dmalloc(&npool[0], (size_t)nn * tile * esz));
+    HC_TRY(dmalloc(&npool[1], (size_t)nn * tile * esz));
+    HC_TRY(dmalloc(&nti, (size_t)nn * tile * esz));
+    if (phys.maux > 0) HC_TRY(dmalloc(&naux, (size_t)nn * atile * esz));
+
+    if (keep_data && xfer && pool[0]) {
+        std::vector<XferOp> ops;
+        ops.reserve(xfer->size());
+        std::vector<XferOp> by_level[MAXL];
+        for (const Transfer& t : *xfer) {
+            XferOp o{};
+            o.dst = t.new_leaf;
+            o.kind = t.kind == Transfer::KEEP ? 0 : (t.kind == Transfer::REFINE ? 1 : 2);
+            for (int k = 0; k < 4; k++) o.src[k] = t.old_leaf[k];
+            o.srclev = level_of[t.old_leaf[0]];
+            o.igrid = t.igrid;
+            ops.push_back(o);
+        }
+        DevArray<XferOp> dops;
+        HC_TRY(dops.upload(ops));
+        if (esz == 8)
+            launch_xfer<double>(dops.p, dops.n, pools<double>(), static_cast<double*>(npool[0]), mx, my,
+                                phys.meqn, stream);
+        else
+            launch_xfer<float>(dops.p, dops.n, pools<float>(), static_cast<float*>(npool[0]), mx, my,
+                               phys.meqn, stream);
+        HC_TRY(check_hip(hipStreamSynchronize(stream), "regrid transfer"));
+        dops.free_();
+    }
+    // drop the previous pools and plans
+    {
+        const long long cu = cell_updates;
+        const double tp = swirl_tperiod;
+        const double rt = refine_threshold, ct = coarsen_threshold;
+        const int iv = tag_ivar;
+        release_pools();
+        cell_updates = cu;
+        swirl_tperiod = tp;
+        refine_threshold = rt;
+        coarsen_threshold = ct;
+        tag_ivar = iv;
+    }
+    pool[0] = npool[0];
+    pool[1] = npool[1];
+    ti = nti;
+    aux = naux;
+    n = cap = nn;
+    for (int L = 0; L < MAXL; L++) parity[L] = 0;
+
+    // geometry
+    const double wx = (dom.bx - dom.ax) / dom.mi, wy = (dom.by - dom.ay) / dom.mj;
+    std::vector<double> xy(2 * (size_t)nn), dd(2 * (size_t)nn);
+    std::vector<int> lev(nn), iota(nn);
+    level_of.assign(nn, 0);
+    for (int k = 0; k < nn; k++) {
+        const Leaf& l = forest.leaves[k];
+        const double pw = wx / (double)(1 << l.level), ph = wy / (double)(1 << l.level);
+        xy[2 * k] = dom.ax + l.gx * pw;
+        xy[2 * k + 1] = dom.ay + l.gy * ph;
+        dd[2 * k] = pw / mx;
+        dd[2 * k + 1] = ph / my;
+        lev[k] = l.level;
+        level_of[k] = l.level;
+        iota[k] = k;
+    }
+    HC_TRY(dmalloc((void**)&d_xylower, sizeof(double) * 2 * nn));
+    HC_TRY(dmalloc((void**)&d_dxdy, sizeof(double) * 2 * nn));
+    HC_TRY(dmalloc((void**)&d_cfl, sizeof(double) * nn));
+    HC_TRY(dmalloc((void**)&d_cflacc, sizeof(double) * 4096));
+    HC_TRY(dmalloc((void**)&d_levels, sizeof(int) * nn));
+    HC_TRY(dmalloc((void**)&d_iota, sizeof(int) * nn));
+    HC_TRY(dmalloc((void**)&d_tags, sizeof(int) * nn));
+    HC_TRY(dmalloc((void**)&d_mass, sizeof(double) * nn * phys.meqn));
+    HC_TRY(check_hip(hipMemcpy(d_xylower, xy.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice), "geom"));
+    HC_TRY(check_hip(hipMemcpy(d_dxdy, dd.data(), sizeof(double) * 2 * nn, hipMemcpyHostToDevice), "geom"));
+    HC_TRY(check_hip(hipMemcpy(d_levels, lev.data(), sizeof(int) * nn, hipMemcpyHostToDevice), "levels"));
+    HC_TRY(check_hip(hipMemcpy(d_iota, iota.data(), sizeof(int) * nn, hipMemcpyHostToDevice), "iota"));
+
+    // level ranges (leaves are sorted by level)
+    for (int L = 0; L < MAXL; L++) plan[L].s0 = plan[L].s1 = 0;
+    for (int k = 0; k < nn;) {
+        int L = lev[k], e = k;
+        while (e < nn && lev[e] == L) e++;
+        plan[L].s0 = k;
+        plan[L].s1 = e;
+        k = e;
+    }
+
+    // neighbour relations -> ghost plans
+    std::vector<GhostOp> avg[MAXL], copy[MAXL], interp[MAXL];
+    std::vector<BcOp> bc[MAXL];
+    std::vector<TiOp> tio[MAXL];
+    has_fine.assign(nn, 0);
+    for (int k = 0; k < nn; k++) {
+        const Leaf& P = forest.leaves[k];
+        const int L = P.level;
+        BcOp b{};
+        b.slot = k;
+        b.level = L;
+        bool phys_any = false;
+        for (int f = 0; f < 4; f++) {
+            int nb[2];
+            const int kind = forest.face_neighbors(k, f, nb);
+            b.mthbc[f] = -1;
+            if (kind == NB_BOUNDARY) {
+                b.mthbc[f] = phys.mthbc[f];
+                phys_any = true;
+            } else if (kind == NB_SAME) {
+                const Leaf& N = forest.leaves[nb[0]];
+                copy[L].push_back({G_COPY_FACE, f, k, nb[0], L, L, (P.gx - N.gx) * mx, (P.gy - N.gy) * my, 0});
+            } else if (kind == NB_FINE) {
+                has_fine[k] = 1;
+                for (int g = 0; g < 2; g++) {
+                    const Leaf& F = forest.leaves[nb[g]];
+                    const int ox = (2 * P.gx - F.gx) * mx, oy = (2 * P.gy - F.gy) * my;
+                    avg[L].push_back({G_AVG_FACE, f, k, nb[g], L, L + 1, ox, oy, 0});
+                    interp[L].push_back({G_INTERP_FACE, f, nb[g], k, L + 1, L, ox, oy, 0});
+                }
+            }
+        }
+        for (int c = 0; c < 4; c++) {
+            int nb;
+            const int kind = forest.corner_neighbor(k, c, &nb);
+            if (kind == NB_SAME) {
+                const Leaf& N = forest.leaves[nb];
+                copy[L].push_back({G_COPY_CORNER, c, k, nb, L, L, (P.gx - N.gx) * mx, (P.gy - N.gy) * my, 0});
+            } else if (kind == NB_FINE) {
+                has_fine[k] = 1;
+                const Leaf& F = forest.leaves[nb];
+                const int ox = (2 * P.gx - F.gx) * mx, oy = (2 * P.gy - F.gy) * my;
+                avg[L].push_back({G_AVG_CORNER, c, k, nb, L, L + 1, ox, oy, 0});
+                interp[L].push_back({G_INTERP_CORNER, c, nb, k, L + 1, L, ox, oy, 0});
+            }
+        }
+        if (phys_any) bc[L].push_back(b);
+        if (has_fine[k]) tio[L].push_back({k, L});
+    }
+    for (int L = 0; L < MAXL; L++) {
+        LevelPlan& lp = plan[L];
+        auto with = [](std::vector<GhostOp> v, int flags) {
+            for (auto& o : v) o.flags = flags;
+            return v;
+        };
+        HC_TRY(lp.avg.upload(avg[L]));
+        HC_TRY(lp.avg_ti.upload(with(avg[L], 1)));
+        HC_TRY(lp.copy.upload(copy[L]));
+        HC_TRY(lp.copy_ti.upload(with(copy[L], 3)));
+        HC_TRY(lp.interp.upload(interp[L]));
+        HC_TRY(lp.interp_ti.upload(with(interp[L], 2)));
+        HC_TRY(lp.bc.upload(bc[L]));
+        std::vector<BcOp> bti = bc[L];
+        for (auto& o : bti) o.flags = 1;
+        HC_TRY(lp.bc_ti.upload(bti));
+        HC_TRY(lp.ti.upload(tio[L]));
+    }
+    return check_hip(hipDeviceSynchronize(), "rebuild");
+}
+
+// ---------------------------------------------------------------------------
+// Ghost update, phase order of fclaw2d_ghost_update_async for one rank:
+// average (maxcoarse..mincoarse [+ time-interpolated level]) -> copy
+// (maxlevel..minlevel [+ti]) -> physical BC on coarse levels [+ti] ->
+// interpolate (maxcoarse..mincoarse [+ti]) -> physical BC on all levels [+ti].
+// The physical BC pass on the fine levels that precedes the final pass in
+// the reference is subsumed by it (bc2 is idempotent on unchanged data).
+// ---------------------------------------------------------------------------
+int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
+{
+    const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
+    const int mincoarse = minlevel, maxcoarse = maxlevel - 1;
+    const int tl = minlevel - 1;
+    const bool use_ti = time_interp && tl >= 0;
+    auto run = [&](auto tag) -> int {
+        using R = decltype(tag);
+        const Pools<R> P = pools<R>();
+        for (int L = maxcoarse; L >= mincoarse; L--)
+            launch_ghost<R>(plan[L].avg.p, plan[L].avg.n, P, mx, my, meqn, stream);
+        if (use_ti) launch_ghost<R>(plan[tl].avg_ti.p, plan[tl].avg_ti.n, P, mx, my, meqn, stream);
+        for (int L = maxlevel; L >= minlevel; L--)
+            launch_ghost<R>(plan[L].copy.p, plan[L].copy.n, P, mx, my, meqn, stream);
+        if (use_ti) launch_ghost<R>(plan[tl].copy_ti.p, plan[tl].copy_ti.n, P, mx, my, meqn, stream);
+        for (int L = maxcoarse; L >= mincoarse; L--)
+            launch_bc<R>(plan[L].bc.p, plan[L].bc.n, P, mx, my, meqn, stream);
+        if (use_ti) launch_bc<R>(plan[tl].bc_ti.p, plan[tl].bc_ti.n, P, mx, my, meqn, stream);
+        for (int L = maxcoarse; L >= mincoarse; L--)
+            launch_ghost<R>(plan[L].interp.p, plan[L].interp.n, P, mx, my, meqn, stream);
+        if (use_ti) launch_ghost<R>(plan[tl].interp_ti.p, plan[tl].interp_ti.n, P, mx, my, meqn, stream);
+        for (int L = maxlevel; L >= minlevel; L--)
+            launch_bc<R>(plan[L].bc.p, plan[L].bc.n, P, mx, my, meqn, stream);
+        if (use_ti) launch_bc<R>(plan[tl].bc_ti.p, plan[tl].bc_ti.n, P, mx, my, meqn, stream);
+        return check_hip(hipGetLastError(), "ghost update");
+    };
+    return esz == 8 ? run(double(0)) : run(float(0));
+}
+
+int Ctx::update_level(int level, double t, double dt)
+{
+    const int s0 = plan[level].s0, s1 = plan[level].s1;
+    const int np = s1 - s0;
+    if (np <= 0) return HIPCLAW_OK;
+    char* qin = static_cast<char*>(pool[parity[level]]) + (size_t)s0 * tile * esz;
+    char* qout = static_cast<char*>(pool[parity[level] ^ 1]) + (size_t)s0 * tile * esz;
+    char* a = aux ? static_cast<char*>(aux) + (size_t)s0 * atile * esz : nullptr;
+    if (swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV)
+        HC_TRY(hipclaw_swirl_aux_batch(&phys, np, d_xylower + 2 * s0, d_dxdy + 2 * s0, t, dt,
+                                       swirl_tperiod, a, stream));
+    HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, dt, d_cfl + s0,
+                               nullptr, stream));
+    if (cflacc_count < 4096) {
+        HC_TRY(hipclaw_max_reduce(d_cfl + s0, np, d_cflacc + cflacc_count, stream));
+        cflacc_count++;
+    }
+    parity[level] ^= 1;
+    cell_updates += (long long)np * phys.mx * phys.my;
+    return HIPCLAW_OK;
+}
+
+int Ctx::timeinterp(int level, double alpha)
+{
+    const LevelPlan& lp = plan[level];
+    if (esz == 8)
+        launch_timeinterp<double>(lp.ti.p, lp.ti.n, pools<double>(), phys.mx, phys.my, phys.meqn, alpha, stream);
+    else
+        launch_timeinterp<float>(lp.ti.p, lp.ti.n, pools<float>(), phys.mx, phys.my, phys.meqn, alpha, stream);
+    return check_hip(hipGetLastError(), "timeinterp");
+}
+
+// Subcycled coarse step (fclaw2d_advance_all_levels).
+int Ctx::advance(double t, double dt, double* maxcfl)
+{
+    const int lo = forest.global_minlevel(), hi = forest.global_maxlevel();
+    struct Cnt { int last_step, step_inc, total_steps; double t, dt_step; } c[MAXL];
+    const bool sub = dom.subcycle != 0;
+    for (int L = lo; L <= hi; L++) {
+        c[L].last_step = 0;
+        c[L].t = t;
+        if (sub) {
+            c[L].dt_step = dt / (double)(1 << (L - lo));
+            c[L].step_inc = 1 << (hi - L);
+            c[L].total_steps = 1 << (L - lo);
+        } else {
+            c[L].step_inc = 1;
+            c[L].dt_step = dt / (double)(1 << (hi - lo));
+            c[L].total_steps = 1 << (hi - lo);
+        }
+    }
+    cflacc_count = 0;
+    // reference dt_step halving: dt_level /= 2 per level from minlevel
+    if (sub) {
+        double d = dt;
+        for (int L = lo; L <= hi; L++) {
+            c[L].dt_step = d;
+            d /= 2;
+        }
+    }
+    // recursive level advance (advance_level)
+    auto advance_level = [&](auto& self, int level, int curr_fine_step) -> int {
+        HC_TRY(update_level(level, c[level].t, c[level].dt_step));
+        c[level].last_step += c[level].step_inc;
+        c[level].t += c[level].dt_step;
+        if (level > lo) {
+            const int coarser = level - 1;
+            if (c[coarser].last_step == curr_fine_step) {
+                HC_TRY(self(self, coarser, c[coarser].last_step));
+                if (sub) {
+                    const int coarse_inc = c[level - 1].step_inc;
+                    const double alpha = (double)(c[level].last_step % coarse_inc) / coarse_inc;
+                    HC_TRY(timeinterp(coarser, alpha));
+                }
+            }
+        }
+        return HIPCLAW_OK;
+    };
+    const int nfine = c[hi].total_steps;
+    for (int nf = 0; nf < nfine; nf++) {
+        HC_TRY(advance_level(advance_level, hi, nf));
+        if (c[hi].last_step < c[hi].total_steps) {
+            if (sub) {
+                int tl = hi - 1;
+                while (c[hi].last_step % c[tl].step_inc == 0) tl--;
+                HC_TRY(ghost_update(tl + 1, hi, 1));
+            } else {
+                HC_TRY(ghost_update(lo, hi, 0));
+            }
+        }
+    }
+    HC_TRY(ghost_update(lo, hi, 0));
+    // maximum over the level CFLs of this coarse step
+    if (maxcfl) {
+        HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
+        HC_TRY(check_hip(hipMemcpyAsync(maxcfl, d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, stream),
+                         "cfl D2H"));
+        HC_TRY(check_hip(hipStreamSynchronize(stream), "advance"));
+    }
+    return HIPCLAW_OK;
+}
+
+int Ctx::qinit(int kind, const QinitParams& prm)
+{
+    if (esz == 8)
+        launch_qinit<double>(kind, prm, d_iota, d_levels, n, pools<double>(), d_xylower, d_dxdy, phys.mx,
+                             phys.my, phys.meqn, stream);
+    else
+        launch_qinit<float>(kind, prm, d_iota, d_levels, n, pools<float>(), d_xylower, d_dxdy, phys.mx,
+                            phys.my, phys.meqn, stream);
+    return check_hip(hipGetLastError(), "qinit");
+}
+
+// Tag every leaf (refine) and every complete family (coarsen), adapt the
+// forest, repopulate the new leaves and refill ghost cells.
+int Ctx::tag_and_adapt(int init_flag, int* changed)
+{
+    const int nn = n;
+    if (esz == 8)
+        launch_tag<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                           tag_ivar, refine_threshold, 0, d_tags, stream);
+    else
+        launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                          tag_ivar, refine_threshold, 0, d_tags, stream);
+    std::vector<int> rtag(nn), ctag(nn);
+    HC_TRY(check_hip(hipMemcpy(rtag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
+    if (!init_flag) {
+        if (esz == 8)
+            launch_tag<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                               tag_ivar, coarsen_threshold, 1, d_tags, stream);
+        else
+            launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                              tag_ivar, coarsen_threshold, 1, d_tags, stream);
+        HC_TRY(check_hip(hipMemcpy(ctag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
+    }
+    std::vector<int> tags(nn, 0);
+    for (int k = 0; k < nn; k++) {
+        const Leaf& l = forest.leaves[k];
+        if (rtag[k] && l.level < dom.maxlevel) tags[k] = 1;
+        else if (!init_flag && ctag[k] && l.level > dom.minlevel) tags[k] = -1;
+    }
+    std::vector<Leaf> before = forest.leaves;
+    std::vector<Transfer> xfer;
+    try {
+        forest.adapt(tags, xfer);
+    } catch (const std::exception& e) {
+        return set_error(HIPCLAW_ERR_STATE, std::string("adapt: ") + e.what());
+    }
+    bool same = before.size() == forest.leaves.size();
+    for (size_t k = 0; same && k < before.size(); k++)
+        same = before[k].level == forest.leaves[k].level && before[k].gx == forest.leaves[k].gx &&
+               before[k].gy == forest.leaves[k].gy;
+    if (changed) *changed = !same;
+    if (same) return HIPCLAW_OK;
+    HC_TRY(rebuild(true, &xfer));
+    return HIPCLAW_OK;
+}
+
+int Ctx::mass(double* sums)
+{
+    if (esz == 8)
+        launch_mass<double>(d_iota, d_levels, n, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn, d_mass, stream);
+    else
+        launch_mass<float>(d_iota, d_levels, n, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn, d_mass, stream);
+    std::vector<double> h((size_t)n * phys.meqn);
+    HC_TRY(check_hip(hipMemcpy(h.data(), d_mass, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "mass"));
+    for (int m = 0; m < phys.meqn; m++) {
+        double s = 0.0, c = 0.0;  // compensated sum over patches in leaf order
+        for (int k = 0; k < n; k++) {
+            double y = h[(size_t)k * phys.meqn + m] - c;
+            double t = s + y;
+            c = (t - s) - y;
+            s = t;
+        }
+        sums[m] = s;
+    }
+    return HIPCLAW_OK;
+}
+
+}  // namespace hc
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace hc;
+
+struct hipclaw_ctx {
+    Ctx* c;
+};
+
+#define CTX_CHECK(ctx)                                                             \
+    do {                                                                           \
+        if (!(ctx) || !(ctx)->c) return set_error(HIPCLAW_ERR_ARG, "ctx is NULL"); \
+    } while (0)
+#define CTX_READY(ctx)                                                                    \
+    do {                                                                                  \
+        CTX_CHECK(ctx);                                                                   \
+        if ((ctx)->c->n == 0) return set_error(HIPCLAW_ERR_STATE, "no forest: call hipclaw_init_uniform"); \
+    } while (0)
+
+extern "C" {
+
+int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, void* stream,
+                       hipclaw_ctx** out)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (!dom || !out) return set_error(HIPCLAW_ERR_ARG, "NULL argument");
+    if (dom->mi < 1 || dom->mj < 1 || dom->minlevel < 0 || dom->maxlevel < dom->minlevel ||
+        dom->maxlevel >= MAXL - 1 || !(dom->bx > dom->ax) || !(dom->by > dom->ay))
+        return set_error(HIPCLAW_ERR_ARG, "bad domain");
+    if (phys->mx % 2 || phys->my % 2) return set_error(HIPCLAW_ERR_ARG, "mx, my must be even for AMR");
+    for (int f = 0; f < 4; f++)
+        if (phys->mthbc[f] != 1 && phys->mthbc[f] != 3)
+            return set_error(HIPCLAW_ERR_UNSUPPORTED, "mthbc must be 1 (extrapolation) or 3 (wall)");
+    if (phys->meqn < 3)
+        for (int f = 0; f < 4; f++)
+            if (phys->mthbc[f] == 3)
+                return set_error(HIPCLAW_ERR_ARG, "wall boundaries need a momentum field (meqn >= 3)");
+    auto* h = new (std::nothrow) hipclaw_ctx;
+    if (!h) return set_error(HIPCLAW_ERR_NOMEM, "host allocation failed");
+    try {
+        h->c = new Ctx(*phys, *dom);
+    } catch (const std::exception& e) {
+        delete h;
+        return set_error(HIPCLAW_ERR_ARG, e.what());
+    }
+    h->c->stream = static_cast<hipStream_t>(stream);
+    *out = h;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_ctx_destroy(hipclaw_ctx* ctx)
+{
+    if (!ctx) return HIPCLAW_OK;
+    if (ctx->c) (void)hipStreamSynchronize(ctx->c->stream);
+    delete ctx->c;
+    delete ctx;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_init_uniform(hipclaw_ctx* ctx, int level)
+{
+    CTX_CHECK(ctx);
+    Ctx& c = *ctx->c;
+    if (level < c.dom.minlevel || level > c.dom.maxlevel) return set_error(HIPCLAW_ERR_ARG, "level outside [minlevel, maxlevel]");
+    c.forest.init_uniform(level);
+    return c.rebuild(false, nullptr);
+}
+
+int hipclaw_num_patches(hipclaw_ctx* ctx, int* n)
+{
+    CTX_CHECK(ctx);
+    if (!n) return set_error(HIPCLAW_ERR_ARG, "n is NULL");
+    *n = ctx->c->n;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* xlower, double* ylower,
+                       double* dx, double* dy)
+{
+    CTX_READY(ctx);
+    Ctx& c = *ctx->c;
+    std::vector<double> xy(2 * (size_t)c.n), dd(2 * (size_t)c.n);
+    int rc = check_hip(hipMemcpy(xy.data(), c.d_xylower, sizeof(double) * xy.size(), hipMemcpyDeviceToHost), "info");
+    if (rc) return rc;
+    rc = check_hip(hipMemcpy(dd.data(), c.d_dxdy, sizeof(double) * dd.size(), hipMemcpyDeviceToHost), "info");
+    if (rc) return rc;
+    for (int k = 0; k < c.n; k++) {
+        const Leaf& l = c.forest.leaves[k];
+        if (level) level[k] = l.level;
+        if (gx) gx[k] = l.gx;
+        if (gy) gy[k] = l.gy;
+        if (xlower) xlower[k] = xy[2 * k];
+        if (ylower) ylower[k] = xy[2 * k + 1];
+        if (dx) dx[k] = dd[2 * k];
+        if (dy) dy[k] = dd[2 * k + 1];
+    }
+    return HIPCLAW_OK;
+}
+
+static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
+{
+    const long long per = auxdata ? c.atile : c.tile;
+    const size_t tot = (size_t)c.n * per;
+    if (tot == 0) return HIPCLAW_OK;
+    std::vector<char> tmp;
+    for (int L = 0; L < MAXL; L++) {
+        const int s0 = c.plan[L].s0, s1 = c.plan[L].s1;
+        if (s1 <= s0) continue;
+        char* dev = static_cast<char*>(auxdata ? c.aux : c.pool[c.parity[L]]) + (size_t)s0 * per * c.esz;
+        const size_t cnt = (size_t)(s1 - s0) * per;
+        double* h = host + (size_t)s0 * per;
+        if (c.esz == 8) {
+            int rc = check_hip(hipMemcpy(to_device ? (void*)dev : (void*)h, to_device ? (void*)h : (void*)dev,
+                                         cnt * 8, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost),
+                               "q copy");
+            if (rc) return rc;
+        } else {
+            std::vector<float> f(cnt);
+            if (to_device) {
+                for (size_t i = 0; i < cnt; i++) f[i] = (float)h[i];
+                int rc = check_hip(hipMemcpy(dev, f.data(), cnt * 4, hipMemcpyHostToDevice), "q copy");
+                if (rc) return rc;
+            } else {
+                int rc = check_hip(hipMemcpy(f.data(), dev, cnt * 4, hipMemcpyDeviceToHost), "q copy");
+                if (rc) return rc;
+                for (size_t i = 0; i < cnt; i++) h[i] = (double)f[i];
+            }
+        }
+    }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q)
+{
+    CTX_READY(ctx);
+    if (!host_q) return set_error(HIPCLAW_ERR_ARG, "host_q is NULL");
+    return copy_q(*ctx->c, const_cast<double*>(host_q), true, false);
+}
+
+int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q)
+{
+    CTX_READY(ctx);
+    if (!host_q) return set_error(HIPCLAW_ERR_ARG, "host_q is NULL");
+    int rc = check_hip(hipStreamSynchronize(ctx->c->stream), "sync");
+    if (rc) return rc;
+    return copy_q(*ctx->c, host_q, false, false);
+}
+
+int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux)
+{
+    CTX_READY(ctx);
+    if (!host_aux || ctx->c->phys.maux == 0) return set_error(HIPCLAW_ERR_ARG, "no aux for this solver");
+    ctx->c->swirl_tperiod = 0;
+    return copy_q(*ctx->c, const_cast<double*>(host_aux), true, true);
+}
+
+int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod)
+{
+    CTX_CHECK(ctx);
+    if (ctx->c->phys.solver != HIPCLAW_SOLVER_ADV) return set_error(HIPCLAW_ERR_ARG, "swirl needs the advection solver");
+    if (!(tperiod >= 0)) return set_error(HIPCLAW_ERR_ARG, "tperiod must be >= 0");
+    ctx->c->swirl_tperiod = tperiod;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_qinit(hipclaw_ctx* ctx, int kind, const double* params, int nparams, int nsub)
+{
+    CTX_READY(ctx);
+    if (kind < 0 || kind > QINIT_SHOCK_BUBBLE) return set_error(HIPCLAW_ERR_ARG, "unknown qinit kind");
+    if (nparams < 0 || nparams > 56 || (nparams > 0 && !params)) return set_error(HIPCLAW_ERR_ARG, "bad qinit params");
+    QinitParams prm{};
+    for (int k = 0; k < nparams; k++) prm.p[k] = params[k];
+    prm.nsub = nsub;
+    return ctx->c->qinit(kind, prm);
+}
+
+int hipclaw_ghost_update(hipclaw_ctx* ctx, int minlevel, int maxlevel, double sync_time, int time_interp)
+{
+    CTX_READY(ctx);
+    (void)sync_time;
+    if (minlevel < 0 || maxlevel >= MAXL - 1 || maxlevel < minlevel) return set_error(HIPCLAW_ERR_ARG, "bad level range");
+    return ctx->c->ghost_update(minlevel, maxlevel, time_interp);
+}
+
+int hipclaw_update_level(hipclaw_ctx* ctx, int level, double t, double dt, double* cfl)
+{
+    CTX_READY(ctx);
+    Ctx& c = *ctx->c;
+    if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    c.cflacc_count = 0;
+    int rc = c.update_level(level, t, dt);
+    if (rc) return rc;
+    if (cfl) {
+        if (c.cflacc_count == 0) {
+            *cfl = 0.0;
+            return HIPCLAW_OK;
+        }
+        rc = check_hip(hipMemcpyAsync(cfl, c.d_cflacc, sizeof(double), hipMemcpyDeviceToHost, c.stream), "cfl");
+        if (rc) return rc;
+        return check_hip(hipStreamSynchronize(c.stream), "update_level");
+    }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_timeinterp(hipclaw_ctx* ctx, int level, double alpha)
+{
+    CTX_READY(ctx);
+    if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    return ctx->c->timeinterp(level, alpha);
+}
+
+int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl)
+{
+    CTX_READY(ctx);
+    if (!(dt > 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be > 0");
+    return ctx->c->advance(t, dt, maxcfl);
+}
+
+int hipclaw_set_tagging(hipclaw_ctx* ctx, int ivar, double refine_threshold, double coarsen_threshold)
+{
+    CTX_CHECK(ctx);
+    if (ivar < 0 || ivar >= ctx->c->phys.meqn) return set_error(HIPCLAW_ERR_ARG, "ivar out of range");
+    ctx->c->tag_ivar = ivar;
+    ctx->c->refine_threshold = refine_threshold;
+    ctx->c->coarsen_threshold = coarsen_threshold;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_regrid(hipclaw_ctx* ctx, int init_flag, int* changed)
+{
+    CTX_READY(ctx);
+    Ctx& c = *ctx->c;
+    int ch = 0;
+    int rc = c.tag_and_adapt(init_flag, &ch);
+    if (rc) return rc;
+    if (changed) *changed = ch;
+    if (ch) rc = c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
+    return rc;
+}
+
+int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, int nparams, int nsub)
+{
+    CTX_CHECK(ctx);
+    Ctx& c = *ctx->c;
+    if (c.n == 0) {
+        int rc = hipclaw_init_uniform(ctx, c.dom.minlevel);
+        if (rc) return rc;
+    }
+    int rc = hipclaw_qinit(ctx, qinit_kind, params, nparams, nsub);
+    if (rc) return rc;
+    for (int pass = 0; pass < c.dom.maxlevel - c.dom.minlevel; pass++) {
+        rc = c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
+        if (rc) return rc;
+        int ch = 0;
+        rc = c.tag_and_adapt(1, &ch);
+        if (rc) return rc;
+        if (!ch) break;
+        rc = hipclaw_qinit(ctx, qinit_kind, params, nparams, nsub);
+        if (rc) return rc;
+    }
+    return c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
+}
+
+int hipclaw_mass(hipclaw_ctx* ctx, double* sums)
+{
+    CTX_READY(ctx);
+    if (!sums) return set_error(HIPCLAW_ERR_ARG, "sums is NULL");
+    return ctx->c->mass(sums);
+}
+
+int hipclaw_cell_updates(hipclaw_ctx* ctx, long long* n)
+{
+    CTX_CHECK(ctx);
+    if (!n) return set_error(HIPCLAW_ERR_ARG, "n is NULL");
+    *n = ctx->c->cell_updates;
+    return HIPCLAW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+// hc_ctx.hpp -- the device-resident solver context behind the hipclaw_ctx_*
+// entry points: forest, slot pools, ghost-fill plans, subcycling state.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <vector>
+
+#include "../../include/hipclaw.h"
+#include "hc_amr.hpp"
+#include "hc_amr_launch.hpp"
+#include "hc_forest.hpp"
+
+namespace hc {
+
+template <class T> struct DevArray {
+    T* p = nullptr;
+    int n = 0;
+    void free_()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    int upload(const std::vector<T>& v);
+};
+
+struct LevelPlan {
+    DevArray<GhostOp> avg, avg_ti;       // coarse = this level
+    DevArray<GhostOp> copy, copy_ti;     // this level
+    DevArray<GhostOp> interp, interp_ti; // coarse = this level
+    DevArray<BcOp> bc, bc_ti;
+    DevArray<TiOp> ti;                   // patches with finer neighbours
+    int s0 = 0, s1 = 0;                  // slot range of the level
+};
+
+struct Ctx {
+    hipclaw_physics phys{};
+    hipclaw_domain dom{};
+    Forest forest;
+    hipStream_t stream = nullptr;
+    int esz = 8;
+    long long tile = 0, atile = 0;  // elements per slot (q, aux)
+    int n = 0, cap = 0;
+    void* pool[2] = {nullptr, nullptr};
+    void* ti = nullptr;
+    void* save = nullptr;
+    void* aux = nullptr;
+    double* d_xylower = nullptr;   // 2 per slot
+    double* d_dxdy = nullptr;      // 2 per slot
+    double* d_cfl = nullptr;       // per slot
+    double* d_cflacc = nullptr;    // running max of level CFLs (device scalar array)
+    int* d_levels = nullptr;       // level per slot
+    int* d_iota = nullptr;         // 0..cap-1
+    int* d_tags = nullptr;         // per slot
+    double* d_mass = nullptr;      // per slot * meqn
+    int parity[MAXL] = {};
+    LevelPlan plan[MAXL];
+    std::vector<int> has_fine;     // per slot
+    std::vector<int> level_of;     // per slot
+    double swirl_tperiod = 0.0;    // > 0: advection velocities from the swirl stream function
+    int cflacc_count = 0;
+    // tagging parameters
+    double refine_threshold = 0.0, coarsen_threshold = 0.0;
+    int tag_ivar = 0;
+    long long cell_updates = 0;    // sum over patch-steps of mx*my
+
+    explicit Ctx(const hipclaw_physics& p, const hipclaw_domain& d);
+    ~Ctx();
+    void release_pools();
+    int rebuild(bool keep_data, const std::vector<Transfer>* xfer);
+    template <class R> Pools<R> pools() const;
+    int ghost_update(int minlevel, int maxlevel, int time_interp);
+    int update_level(int level, double t, double dt);
+    int timeinterp(int level, double alpha);
+    int advance(double t, double dt, double* maxcfl);
+    int qinit(int kind, const QinitParams& prm);
+    int tag_and_adapt(int init_flag, int* changed);
+    int mass(double* sums);
+    int level_range(int level, int* s0, int* s1) const;
+};
+
+}  // namespace hc

@@ -1,0 +1,216 @@
+"""Host front end of the device-resident AMR solver (hipclaw_ctx_* C ABI).
+
+Mirrors the ForestClaw objects a user of the reference works with:
+  Options              [Options] / [clawpatch] / [clawpack46] groups
+                       (src/fclaw_options.c, fc2d_clawpack46_options.c)
+  Hipclaw46.initialize fclaw2d_initialize (qinit + initial refinement)
+  Hipclaw46.advance    fclaw2d_advance_all_levels
+  Hipclaw46.run        fclaw2d_run outstyle 1 (dt from the CFL number,
+                       step rejection when maxcfl > max_cfl, regrid cadence)
+Every numerical operation runs in libhipclaw.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import hipclaw as H
+
+
+Domain = H.Domain
+QINIT = {"const": 0, "swirl": 1, "radial_dam": 2, "humps": 3, "shock_bubble": 4}
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.c_void_p
+
+
+def _lib():
+    return H.lib()
+
+
+@dataclass
+class Options:
+    """The option names of the reference's .ini sections that the hot path uses."""
+    # [Options]
+    minlevel: int = 0
+    maxlevel: int = 0
+    regrid_interval: int = 1
+    refine_threshold: float = 0.5
+    coarsen_threshold: float = 0.1
+    subcycle: bool = True
+    max_cfl: float = 1.0
+    desired_cfl: float = 0.9
+    use_fixed_dt: bool = False
+    initial_dt: float = 0.01
+    ax: float = 0.0
+    bx: float = 1.0
+    ay: float = 0.0
+    by: float = 1.0
+    mi: int = 1
+    mj: int = 1
+    # [clawpatch]
+    mx: int = 32
+    my: int = 32
+    mbc: int = 2
+    threshold_variable: int = 0
+    # [clawpack46]
+    solver: int = H.SOLVER_SWE_ROE
+    order: tuple = (2, 2)
+    mthlim: tuple | None = None
+    mthbc: tuple = (1, 1, 1, 1)
+    grav: float = 1.0
+    gamma: float = 1.4
+    precision: int = H.FP64
+
+
+def _check(rc):
+    H._check(rc)
+
+
+class Hipclaw46:
+    """One device-resident simulation (one hipclaw_ctx)."""
+
+    def __init__(self, opt: Options, stream: int | None = None):
+        self.opt = opt
+        L = _lib()
+        self.phys = H.physics(opt.solver, opt.mx, opt.my, precision=opt.precision, order=opt.order,
+                              mthlim=opt.mthlim, mthbc=opt.mthbc, grav=opt.grav, gamma=opt.gamma)
+        self.dom = Domain(opt.mi, opt.mj, opt.ax, opt.bx, opt.ay, opt.by, opt.minlevel,
+                          opt.maxlevel, 1 if opt.subcycle else 0)
+        h = _VP()
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(L.hipclaw_ctx_create(ctypes.byref(self.phys), ctypes.byref(self.dom), stream,
+                                    ctypes.byref(h)))
+        self.h = h
+        _check(L.hipclaw_set_tagging(self.h, opt.threshold_variable, opt.refine_threshold,
+                                     opt.coarsen_threshold))
+        self.t = 0.0
+        self.n_inner = 0
+
+    def close(self):
+        if self.h:
+            _lib().hipclaw_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- forest / data ----------------------------------------------------
+    @property
+    def tile_shape(self):
+        return (self.phys.meqn, self.opt.my + 2 * self.opt.mbc, self.opt.mx + 2 * self.opt.mbc)
+
+    def num_patches(self) -> int:
+        n = ctypes.c_int()
+        _check(_lib().hipclaw_num_patches(self.h, ctypes.byref(n)))
+        return n.value
+
+    def patch_info(self):
+        n = self.num_patches()
+        lev, gx, gy = (np.zeros(n, np.int32) for _ in range(3))
+        xl, yl, dx, dy = (np.zeros(n) for _ in range(4))
+        ip = lambda a: a.ctypes.data_as(_IP)  # noqa: E731
+        dp = lambda a: a.ctypes.data_as(_DP)  # noqa: E731
+        _check(_lib().hipclaw_patch_info(self.h, ip(lev), ip(gx), ip(gy), dp(xl), dp(yl), dp(dx),
+                                         dp(dy)))
+        return dict(level=lev, gx=gx, gy=gy, xlower=xl, ylower=yl, dx=dx, dy=dy)
+
+    def init_uniform(self, level: int):
+        _check(_lib().hipclaw_init_uniform(self.h, level))
+
+    def set_q(self, q: np.ndarray):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.shape == (self.num_patches(),) + self.tile_shape
+        _check(_lib().hipclaw_set_q(self.h, q.ctypes.data_as(_DP)))
+
+    def get_q(self) -> np.ndarray:
+        q = np.zeros((self.num_patches(),) + self.tile_shape)
+        _check(_lib().hipclaw_get_q(self.h, q.ctypes.data_as(_DP)))
+        return q
+
+    def set_swirl(self, tperiod: float):
+        _check(_lib().hipclaw_set_swirl(self.h, tperiod))
+
+    def qinit(self, kind: str, params=(), nsub: int = 4):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        _check(_lib().hipclaw_qinit(self.h, QINIT[kind], p.ctypes.data_as(_DP), len(p), nsub))
+
+    def initialize(self, kind: str, params=(), nsub: int = 4):
+        """fclaw2d_initialize: qinit at minlevel, refine while tagged, qinit again."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        _check(_lib().hipclaw_initialize(self.h, QINIT[kind], p.ctypes.data_as(_DP), len(p), nsub))
+
+    # ---- time stepping ----------------------------------------------------
+    def ghost_update(self, minlevel, maxlevel, time_interp=0, t=0.0):
+        _check(_lib().hipclaw_ghost_update(self.h, minlevel, maxlevel, t, time_interp))
+
+    def update_level(self, level, t, dt) -> float:
+        c = ctypes.c_double()
+        _check(_lib().hipclaw_update_level(self.h, level, t, dt, ctypes.byref(c)))
+        return c.value
+
+    def advance(self, t: float, dt: float, want_cfl: bool = True) -> float:
+        c = ctypes.c_double(0.0)
+        _check(_lib().hipclaw_advance(self.h, t, dt, ctypes.byref(c) if want_cfl else None))
+        return c.value
+
+    def regrid(self, init_flag: int = 0) -> bool:
+        ch = ctypes.c_int()
+        _check(_lib().hipclaw_regrid(self.h, init_flag, ctypes.byref(ch)))
+        return bool(ch.value)
+
+    def mass(self) -> np.ndarray:
+        s = np.zeros(self.phys.meqn)
+        _check(_lib().hipclaw_mass(self.h, s.ctypes.data_as(_DP)))
+        return s
+
+    def cell_updates(self) -> int:
+        n = ctypes.c_longlong()
+        _check(_lib().hipclaw_cell_updates(self.h, ctypes.byref(n)))
+        return n.value
+
+    def run(self, tfinal: float, nout: int = 1, dt0: float | None = None, max_steps: int = 10**9,
+            on_step=None):
+        """fclaw2d_run outstyle 1 (src/fclaw2d_run.c:119-297): variable dt from the
+        CFL number unless use_fixed_dt; reject and retake a step whose maxcfl
+        exceeds max_cfl (the solution is restored from a saved copy)."""
+        o = self.opt
+        dt_min = dt0 if dt0 is not None else o.initial_dt
+        t0 = self.t
+        dt_outer = (tfinal - t0) / nout
+        steps = 0
+        for n in range(nout):
+            tend = t0 + (n + 1) * dt_outer
+            while self.t < tend and steps < max_steps:
+                saved = None if o.use_fixed_dt else self.get_q()
+                dt_step = dt_min
+                tol = 1e-2 * dt_step
+                took_small = False
+                if not o.use_fixed_dt:
+                    small = tend - (self.t + dt_step)
+                    if small < tol:
+                        dt_step = tend - self.t
+                        took_small = small < 0
+                cfl = self.advance(self.t, dt_step)
+                if cfl > o.max_cfl and not o.use_fixed_dt:
+                    self.set_q(saved)
+                    dt_min = dt_min * o.desired_cfl / cfl
+                    continue
+                self.n_inner += 1
+                steps += 1
+                self.t += dt_step
+                if not o.use_fixed_dt and not took_small:
+                    dt_min = dt_min * o.desired_cfl / cfl
+                if on_step:
+                    on_step(self, cfl, dt_step)
+                if o.regrid_interval > 0 and self.n_inner % o.regrid_interval == 0 and \
+                        o.maxlevel > o.minlevel:
+                    self.regrid()
+        return dt_min

@@ -38,6 +38,14 @@ class Launch(ctypes.Structure):
     _fields_ = [("cols_per_lane", ctypes.c_int), ("waves_per_group", ctypes.c_int)]
 
 
+class Domain(ctypes.Structure):
+    _fields_ = [("mi", ctypes.c_int), ("mj", ctypes.c_int),
+                ("ax", ctypes.c_double), ("bx", ctypes.c_double),
+                ("ay", ctypes.c_double), ("by", ctypes.c_double),
+                ("minlevel", ctypes.c_int), ("maxlevel", ctypes.c_int),
+                ("subcycle", ctypes.c_int)]
+
+
 class HipclawError(RuntimeError):
     pass
 
@@ -59,6 +67,34 @@ _SIGS = {
     "hipclaw_bc2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP],
                           ctypes.c_int),
 }
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_SIGS.update({
+    "hipclaw_ctx_create": ([ctypes.POINTER(Physics), ctypes.POINTER(Domain), _VP,
+                            ctypes.POINTER(_VP)], ctypes.c_int),
+    "hipclaw_ctx_destroy": ([_VP], ctypes.c_int),
+    "hipclaw_init_uniform": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_num_patches": ([_VP, _IP], ctypes.c_int),
+    "hipclaw_patch_info": ([_VP, _IP, _IP, _IP, _DP, _DP, _DP, _DP], ctypes.c_int),
+    "hipclaw_set_q": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_get_q": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_set_swirl": ([_VP, ctypes.c_double], ctypes.c_int),
+    "hipclaw_set_aux": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_qinit": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "hipclaw_ghost_update": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int],
+                             ctypes.c_int),
+    "hipclaw_update_level": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, _DP],
+                             ctypes.c_int),
+    "hipclaw_timeinterp": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
+    "hipclaw_advance": ([_VP, ctypes.c_double, ctypes.c_double, _DP], ctypes.c_int),
+    "hipclaw_set_tagging": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+    "hipclaw_regrid": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
+    "hipclaw_initialize": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "hipclaw_mass": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_cell_updates": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+})
 
 
 def lib() -> ctypes.CDLL:

@@ -115,6 +115,89 @@ int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const doubl
 int hipclaw_bc2_batch(const hipclaw_physics* phys, int npatch, void* q, const int* patch_ids,
                       const int* mthbc, void* stream);
 
+
+/* ------------------------------------------------------------------------
+ * Device-resident AMR solver context.
+ *
+ * Holds the forest of leaf patches (a quadtree per block of an mi x mj brick,
+ * replacing what the reference obtains from p4est through src/forestclaw2d.c),
+ * all patch data packed per slot in device pools, the precomputed ghost-fill
+ * plans and the subcycling state.  Leaves are ordered by level, then Morton
+ * order within each block; host arrays exchanged with the context follow that
+ * order (slot k = leaf k).
+ * ------------------------------------------------------------------------ */
+typedef struct hipclaw_domain {
+    int mi, mj;            /* brick of mi x mj unit blocks (1 x 1 = single block) */
+    double ax, bx, ay, by; /* physical extent of the brick                        */
+    int minlevel, maxlevel;
+    int subcycle;          /* [Options] subcycle: 1 = level time stepping (default) */
+} hipclaw_domain;
+
+typedef struct hipclaw_ctx hipclaw_ctx;
+
+/* Initial-condition families (the applications' clawpack46_qinit hooks). */
+#define HIPCLAW_QINIT_CONST 0
+#define HIPCLAW_QINIT_SWIRL 1        /* swirl/user_4.6/qinit.f: q = 1 for x < p[0]      */
+#define HIPCLAW_QINIT_RADIAL_DAM 2   /* p = x0, y0, r0, h_in, h_out (cell averages)      */
+#define HIPCLAW_QINIT_HUMPS 3        /* p = h0, n, then n x (cx, cy, A, w)               */
+#define HIPCLAW_QINIT_SHOCK_BUBBLE 4 /* p = gamma, xs, rho, u, p post-shock, x0, y0, r, rho_b */
+
+int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, void* stream,
+                       hipclaw_ctx** out);
+int hipclaw_ctx_destroy(hipclaw_ctx* ctx);
+
+/* Uniform forest at one level (fclaw2d_domain_new_brick / _unitsquare). */
+int hipclaw_init_uniform(hipclaw_ctx* ctx, int level);
+int hipclaw_num_patches(hipclaw_ctx* ctx, int* n);
+/* Per-patch host arrays of length num_patches (any pointer may be NULL). */
+int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* xlower,
+                       double* ylower, double* dx, double* dy);
+/* Whole solution in leaf order, FP64 on the host, T*meqn per patch. */
+int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q);
+int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q);
+/* Swirl velocity field recomputed before every patch step (b4step2), period
+ * tperiod; tperiod = 0 keeps the aux arrays as set. */
+int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod);
+int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux);
+
+/* patch_vt->initialize on every leaf. */
+int hipclaw_qinit(hipclaw_ctx* ctx, int kind, const double* params, int nparams, int nsub);
+
+/* fclaw2d_ghost_update (src/fclaw2d_ghost_fill.c:838-856): average, copy,
+ * physical BC, interpolate, physical BC over levels minlevel..maxlevel, with
+ * level minlevel-1 supplying time-interpolated data when time_interp != 0. */
+int hipclaw_ghost_update(hipclaw_ctx* ctx, int minlevel, int maxlevel, double sync_time,
+                         int time_interp);
+
+/* fclaw2d_update_single_step for one level (src/fclaw2d_update_single_step.c:73-103):
+ * every patch of the level in one batched launch; *cfl = level maximum. */
+int hipclaw_update_level(hipclaw_ctx* ctx, int level, double t, double dt, double* cfl);
+
+/* fclaw2d_timeinterp (src/fclaw2d_timeinterp.c:51-57). */
+int hipclaw_timeinterp(hipclaw_ctx* ctx, int level, double alpha);
+
+/* fclaw2d_advance_all_levels (src/fclaw2d_advance.c:247-357): one coarse step
+ * of size dt with subcycling, intermediate and final ghost updates; returns
+ * the maximum Courant number over all level updates. */
+int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl);
+
+/* Gradient tagging on field ivar (fclaw2d_clawpatch_gradient_exceeds_th.f90). */
+int hipclaw_set_tagging(hipclaw_ctx* ctx, int ivar, double refine_threshold,
+                        double coarsen_threshold);
+/* fclaw2d_regrid (src/fclaw2d_regrid.c:234-335): tag, adapt with 2:1 balance,
+ * repopulate (interpolate2fine / average2coarse), rebuild plans, ghost update.
+ * init_flag = 1 disables coarsening as in the initial refinement. */
+int hipclaw_regrid(hipclaw_ctx* ctx, int init_flag, int* changed);
+/* fclaw2d_initialize: qinit on minlevel, then refine / re-initialise until the
+ * forest stops changing (at most maxlevel-minlevel passes); ghost cells filled. */
+int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, int nparams,
+                       int nsub);
+
+/* FP64 conservation sums sum(q dx dy) per field (meqn doubles). */
+int hipclaw_mass(hipclaw_ctx* ctx, double* sums);
+/* Cell updates performed so far (count_single_step * mx * my). */
+int hipclaw_cell_updates(hipclaw_ctx* ctx, long long* n);
+
 #ifdef __cplusplus
 }
 #endif

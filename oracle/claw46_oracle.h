@@ -60,6 +60,31 @@ int orc_bc2(int mx, int my, int mbc, int meqn, const int mthbc[4], double* q);
 /* Limiter function phi(r) of clawpack46_inlinelimiter for one family. */
 double orc_limiter_phi(int mthlim, double r);
 
+/* Ghost-cell and regrid kernels for one patch pair (claw46_ghost.c).
+ * (sx, sy): same-size index shift; (ox, oy): half-size offset
+ * (xlower_coarse - xlower_fine) * 2^level_fine * mx (resp. my). */
+double orc_compute_slopes(double sl, double sr);
+void orc_copy_face(int mx, int my, int mbc, int meqn, double* qthis, const double* qnbr,
+                   int iface, int sx, int sy);
+void orc_copy_corner(int mx, int my, int mbc, int meqn, double* qthis, const double* qnbr,
+                     int icorner, int sx, int sy);
+void orc_average_face(int mx, int my, int mbc, int meqn, double* qcoarse, const double* qfine,
+                      int iface_coarse, int ox, int oy);
+void orc_average_corner(int mx, int my, int mbc, int meqn, double* qcoarse, const double* qfine,
+                        int icorner_coarse, int ox, int oy);
+void orc_interpolate_face(int mx, int my, int mbc, int meqn, const double* qcoarse, double* qfine,
+                          int iface_coarse, int ox, int oy);
+void orc_interpolate_corner(int mx, int my, int mbc, int meqn, const double* qcoarse, double* qfine,
+                            int icorner_coarse, int ox, int oy);
+void orc_interpolate2fine(int mx, int my, int mbc, int meqn, const double* qcoarse, double* qfine,
+                          int igrid);
+void orc_average2coarse(int mx, int my, int mbc, int meqn, double* qcoarse, const double* qfine,
+                        int igrid);
+int orc_timeinterp(int mx, int my, int mbc, int meqn, const double* qcurr, const double* qlast,
+                   double* qinterp, double alpha);
+int orc_tag_gradient(int mx, int my, int mbc, const double* q, int ivar, double dx, double dy,
+                     double threshold);
+
 #ifdef __cplusplus
 }
 #endif
