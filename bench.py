@@ -1,0 +1,206 @@
+"""Benchmark: cell-updates/s of the wave-propagation patch solver on one MI355X.
+
+Workload (shallow water, large uniform-plus-AMR case): unit square, one block,
+32x32 patches (mbc 2), base level 7 uniform (16,384 patches = 16.8 M cells)
+refined to level 9 around 8 seeded Gaussian humps of amplitude U(0.1, 0.5)
+and width U(0.02, 0.1) on h = 1 at rest; Roe solver with entropy fix, MC
+limiter, transverse corrections, extrapolation boundaries, subcycling.
+
+A step is one coarse step (fclaw2d_advance_all_levels): every level update
+at its subcycled times, time interpolation and all ghost updates.  Regrid is
+not part of a step.  The headline value is the FP32 path; the FP64 path runs
+the same steps from the same data and serves as the FP64 oracle (it is
+bitwise equal to the CPU oracle in the parity tests), which gives the FP32
+error.  The CPU baseline times the oracle's patch step (the restated
+Fortran step2) on a bounded sample of the same patches.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s on 1 MI355X, FP32-batched vs FP64 plain, at <=1e-5 rel L1"
+HBM_PEAK = 8.0e12
+
+
+def humps(seed=7, n=8):
+    rng = np.random.default_rng(seed)
+    p = [1.0, n]
+    for _ in range(n):
+        p += [rng.uniform(0.15, 0.85), rng.uniform(0.15, 0.85), rng.uniform(0.1, 0.5),
+              rng.uniform(0.02, 0.1)]
+    return p
+
+
+def make_sim(precision, args):
+    from forestclaw_amd import forest as F
+    from forestclaw_amd import hipclaw as H
+    opt = F.Options(minlevel=args.minlevel, maxlevel=args.maxlevel, mx=32, my=32,
+                    solver=H.SOLVER_SWE_ROE, precision=precision, grav=1.0,
+                    refine_threshold=args.refine_threshold, coarsen_threshold=0.0,
+                    subcycle=True)
+    sim = F.Hipclaw46(opt)
+    sim.initialize("humps", humps(args.seed))
+    return sim
+
+
+def run_steps(sim, nsteps, dt, t0=0.0):
+    import torch
+    t = t0
+    for _ in range(nsteps):
+        sim.advance(t, dt)
+        t += dt
+    torch.cuda.synchronize()
+    return t
+
+
+def timed(sim, args, dt):
+    import torch
+    run_steps(sim, args.warmup, dt)
+    c0 = sim.cell_updates()
+    sim.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t = run_steps(sim, args.steps, dt, t0=args.warmup * dt)
+    t1 = time.perf_counter()
+    kms, launches, kcells = sim.profile_read()
+    sim.profile(False)
+    cells = sim.cell_updates() - c0
+    return dict(seconds=t1 - t0, cells=cells, kernel_ms=kms, launches=launches, kcells=kcells,
+                t_end=t)
+
+
+def cpu_baseline(sim64, budget_s):
+    """Time the oracle patch step (OpenMP over patches) on a sample of level-7
+    patches of the same workload."""
+    from oracle import oracle as O
+    info = sim64.patch_info()
+    lev = info["level"]
+    q = sim64.get_q()
+    idx = np.nonzero(lev == lev.min())[0][:256]
+    sample = np.ascontiguousarray(q[idx])
+    cfg = O.make_cfg(O.SOLVER_SWE_ROE, 32, 32, 2, grav=1.0)
+    dx, dy = info["dx"][idx], info["dy"][idx]
+    dt = 0.5 * dx.min()
+    cells = 0
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        work = sample.copy()
+        O.step2_batch(cfg, work, None, dx, dy, dt)
+        cells += len(idx) * 32 * 32
+        reps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=cells / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"{len(idx)} level-7 patches (32x32, meqn 3) x {reps} patch-steps each, "
+                       f"oracle step2 FP64, OpenMP over patches, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--minlevel", type=int, default=7)
+    ap.add_argument("--maxlevel", type=int, default=9)
+    ap.add_argument("--refine-threshold", type=float, default=0.5)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
+              "a design in DESIGN.md, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from forestclaw_amd import hipclaw as H
+    assert torch.cuda.is_available(), "no GPU visible"
+
+    sim32 = make_sim(H.FP32, args)
+    info = sim32.patch_info()
+    levels = {int(L): int((info["level"] == L).sum()) for L in np.unique(info["level"])}
+    dx_fine = info["dx"].min()
+    c_max = 1.0 + np.sqrt(1.0 * 1.6)
+    dt = 0.9 * info["dx"].max() / c_max  # CFL 0.9 at the coarsest level (subcycled)
+
+    r32 = timed(sim32, args, dt)
+    out32 = sim32.get_q() if not args.no_fp64 else None
+
+    r64 = None
+    err = None
+    if not args.no_fp64:
+        sim64 = make_sim(H.FP64, args)
+        info64 = sim64.patch_info()
+        assert np.array_equal(info64["level"], info["level"])
+        r64 = timed(sim64, args, dt)
+        out64 = sim64.get_q()
+        a, b = out32[:, :, 2:-2, 2:-2], out64[:, :, 2:-2, 2:-2]
+        err = dict(rel_l1=float(np.abs(a - b).sum() / np.abs(b).sum()),
+                   rel_linf=float(np.abs(a - b).max() / np.abs(b).max()))
+        del out64
+
+    cpu = None
+    if not args.no_cpu:
+        base = sim64 if r64 is not None else make_sim(H.FP64, args)
+        cpu = cpu_baseline(base, args.cpu_seconds)
+
+    w = 4
+    T, N, meqn = 36 * 36, 32 * 32, 3
+    b_upd = w * meqn * (T + N) / N  # algorithmic bytes per cell update (read tile, write interior)
+    k_time = r32["kernel_ms"] / 1e3
+    achieved = b_upd * r32["kcells"] / k_time if k_time > 0 else 0.0
+    value = r32["cells"] / r32["seconds"]
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r32["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian humps, h=1 at rest)",
+        "config": {"workload": "SWE Roe, unit square, 32x32 patches, base level 7 uniform "
+                               "refined to level 9 around 8 seeded humps, subcycled",
+                   "patches_per_level": levels, "dt_coarse": dt,
+                   "cell_updates_per_step": r32["cells"] // args.steps},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_step2<float,Swe>",
+                     "bytes_per_cell_update": b_upd,
+                     "kernel_only_rate": r32["kcells"] / k_time if k_time > 0 else None,
+                     "kernel_launches": r32["launches"]},
+        "cpu_baseline": cpu,
+    }
+    res["roofline"]["achieved"] = achieved / 1e9
+    res["roofline"]["peak"] = HBM_PEAK / 1e9
+    if r64 is not None:
+        v64 = r64["cells"] / r64["seconds"]
+        k64 = r64["kernel_ms"] / 1e3
+        res["fp64_plain"] = {"value": v64, "ms_per_step": 1e3 * r64["seconds"] / args.steps,
+                             "kernel_only_rate": r64["kcells"] / k64 if k64 > 0 else None}
+        res["speedup_vs_fp64_plain"] = value / v64
+        res["fp32_vs_fp64"] = err
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,7 +1,6 @@
 // hc_amr.hpp -- device data structures of the AMR driver: ghost-fill
 // operations, physical-BC and time-interpolation lists, regrid transfers.
 #pragma once
-#include <hip/hip_runtime.h>
 
 namespace hc {
 

@@ -92,12 +92,13 @@ __global__ void k_swirl_aux(int npatch, int mx, int my, const double* xylower,
 // One 64-lane group per patch.
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void k_bc2(int npatch, int mx, int my, int meqn, R* q, const int* ids,
+__global__ void k_bc2(int npatch, int mx, int my, int meqn, R* q, const int* ids, int ntiles,
                       const int* mthbc)
 {
     const int p = blockIdx.x;
     if (p >= npatch) return;
     const int tile = ids ? ids[p] : p;
+    if (tile < 0 || tile >= ntiles) return;
     const int NX = mx + 4, NY = my + 4;
     R* qp = q + (long long)tile * NX * NY * meqn;
     const int* bc = mthbc + 4 * p;
@@ -177,12 +178,14 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
 }
 
 int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
-                        const void* aux, const double* dxdy, const int* patch_ids, double dt,
-                        double* cfl_patch, const hipclaw_launch* launch, void* stream)
+                        const void* aux, const double* dxdy, const int* patch_ids, int ntiles,
+                        double dt, double* cfl_patch, const hipclaw_launch* launch, void* stream)
 {
     int rc = validate_physics(phys);
     if (rc) return rc;
     if (npatch < 0) return set_error(HIPCLAW_ERR_ARG, "npatch < 0");
+    if (!patch_ids && ntiles < npatch)
+        return set_error(HIPCLAW_ERR_ARG, "ntiles < npatch without a patch list");
     if (npatch == 0) return HIPCLAW_OK;
     if (!q_in || !q_out || !dxdy) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
     if (phys->maux > 0 && !aux) return set_error(HIPCLAW_ERR_ARG, "aux is NULL but maux > 0");
@@ -209,6 +212,7 @@ int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_i
     a.dt = dt;
     a.dxdy = dxdy;
     a.patch_ids = patch_ids;
+    a.ntiles = ntiles;
     const long long T = (long long)(phys->mx + 4) * (phys->my + 4);
     a.qstride = T * phys->meqn;
     a.astride = T * phys->maux;
@@ -251,19 +255,21 @@ int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const doubl
 }
 
 int hipclaw_bc2_batch(const hipclaw_physics* phys, int npatch, void* q, const int* patch_ids,
-                      const int* mthbc, void* stream)
+                      int ntiles, const int* mthbc, void* stream)
 {
     int rc = validate_physics(phys);
     if (rc) return rc;
     if (npatch <= 0) return HIPCLAW_OK;
+    if (!patch_ids && ntiles < npatch)
+        return set_error(HIPCLAW_ERR_ARG, "ntiles < npatch without a patch list");
     if (!q || !mthbc) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (phys->precision == HIPCLAW_FP64)
         hipLaunchKernelGGL(k_bc2<double>, dim3(npatch), dim3(64), 0, s, npatch, phys->mx, phys->my,
-                           phys->meqn, static_cast<double*>(q), patch_ids, mthbc);
+                           phys->meqn, static_cast<double*>(q), patch_ids, ntiles, mthbc);
     else
         hipLaunchKernelGGL(k_bc2<float>, dim3(npatch), dim3(64), 0, s, npatch, phys->mx, phys->my,
-                           phys->meqn, static_cast<float*>(q), patch_ids, mthbc);
+                           phys->meqn, static_cast<float*>(q), patch_ids, ntiles, mthbc);
     return check_hip(hipGetLastError(), "k_bc2 launch");
 }
 

@@ -18,6 +18,7 @@
 #include <string>
 
 #include "hc_ctx.hpp"
+#include "hc_plan.hpp"
 #include "hc_internal.hpp"
 
 namespace hc {
@@ -46,7 +47,18 @@ Ctx::Ctx(const hipclaw_physics& p, const hipclaw_domain& d)
 
 Ctx::~Ctx()
 {
+    prof_clear();
     release_pools();
+}
+
+void Ctx::prof_clear()
+{
+    for (auto& e : ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    ev.clear();
+    ev_cells.clear();
 }
 
 void Ctx::release_pools()
@@ -210,54 +222,14 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
     }
 
     // neighbour relations -> ghost plans
-    std::vector<GhostOp> avg[MAXL], copy[MAXL], interp[MAXL];
-    std::vector<BcOp> bc[MAXL];
-    std::vector<TiOp> tio[MAXL];
-    has_fine.assign(nn, 0);
-    for (int k = 0; k < nn; k++) {
-        const Leaf& P = forest.leaves[k];
-        const int L = P.level;
-        BcOp b{};
-        b.slot = k;
-        b.level = L;
-        bool phys_any = false;
-        for (int f = 0; f < 4; f++) {
-            int nb[2];
-            const int kind = forest.face_neighbors(k, f, nb);
-            b.mthbc[f] = -1;
-            if (kind == NB_BOUNDARY) {
-                b.mthbc[f] = phys.mthbc[f];
-                phys_any = true;
-            } else if (kind == NB_SAME) {
-                const Leaf& N = forest.leaves[nb[0]];
-                copy[L].push_back({G_COPY_FACE, f, k, nb[0], L, L, (P.gx - N.gx) * mx, (P.gy - N.gy) * my, 0});
-            } else if (kind == NB_FINE) {
-                has_fine[k] = 1;
-                for (int g = 0; g < 2; g++) {
-                    const Leaf& F = forest.leaves[nb[g]];
-                    const int ox = (2 * P.gx - F.gx) * mx, oy = (2 * P.gy - F.gy) * my;
-                    avg[L].push_back({G_AVG_FACE, f, k, nb[g], L, L + 1, ox, oy, 0});
-                    interp[L].push_back({G_INTERP_FACE, f, nb[g], k, L + 1, L, ox, oy, 0});
-                }
-            }
-        }
-        for (int c = 0; c < 4; c++) {
-            int nb;
-            const int kind = forest.corner_neighbor(k, c, &nb);
-            if (kind == NB_SAME) {
-                const Leaf& N = forest.leaves[nb];
-                copy[L].push_back({G_COPY_CORNER, c, k, nb, L, L, (P.gx - N.gx) * mx, (P.gy - N.gy) * my, 0});
-            } else if (kind == NB_FINE) {
-                has_fine[k] = 1;
-                const Leaf& F = forest.leaves[nb];
-                const int ox = (2 * P.gx - F.gx) * mx, oy = (2 * P.gy - F.gy) * my;
-                avg[L].push_back({G_AVG_CORNER, c, k, nb, L, L + 1, ox, oy, 0});
-                interp[L].push_back({G_INTERP_CORNER, c, nb, k, L + 1, L, ox, oy, 0});
-            }
-        }
-        if (phys_any) bc[L].push_back(b);
-        if (has_fine[k]) tio[L].push_back({k, L});
-    }
+    HostPlans hp;
+    build_host_plans(forest, mx, my, phys.mthbc, hp);
+    has_fine = hp.has_fine;
+    auto& avg = hp.avg;
+    auto& copy = hp.copy;
+    auto& interp = hp.interp;
+    auto& bc = hp.bc;
+    auto& tio = hp.ti;
     for (int L = 0; L < MAXL; L++) {
         LevelPlan& lp = plan[L];
         auto with = [](std::vector<GhostOp> v, int flags) {
@@ -327,9 +299,20 @@ int Ctx::update_level(int level, double t, double dt)
     if (swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV)
         HC_TRY(hipclaw_swirl_aux_batch(&phys, np, d_xylower + 2 * s0, d_dxdy + 2 * s0, t, dt,
                                        swirl_tperiod, a, stream));
-    HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, dt, d_cfl + s0,
-                               nullptr, stream));
-    if (cflacc_count < 4096) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        HC_TRY(check_hip(hipEventCreate(&e0), "event"));
+        HC_TRY(check_hip(hipEventCreate(&e1), "event"));
+        HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
+    }
+    HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
+                               d_cfl + s0, nullptr, stream));
+    if (prof) {
+        HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
+        ev.push_back({e0, e1});
+        ev_cells.push_back((long long)np * phys.mx * phys.my);
+    }
+    if (cflacc_count < 4095) {
         HC_TRY(hipclaw_max_reduce(d_cfl + s0, np, d_cflacc + cflacc_count, stream));
         cflacc_count++;
     }
@@ -766,6 +749,35 @@ int hipclaw_mass(hipclaw_ctx* ctx, double* sums)
     CTX_READY(ctx);
     if (!sums) return set_error(HIPCLAW_ERR_ARG, "sums is NULL");
     return ctx->c->mass(sums);
+}
+
+int hipclaw_profile(hipclaw_ctx* ctx, int enable)
+{
+    CTX_CHECK(ctx);
+    ctx->c->prof_clear();
+    ctx->c->prof = enable != 0;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_profile_read(hipclaw_ctx* ctx, double* kernel_ms, long long* launches, long long* cells)
+{
+    CTX_CHECK(ctx);
+    Ctx& c = *ctx->c;
+    int rc = check_hip(hipStreamSynchronize(c.stream), "profile sync");
+    if (rc) return rc;
+    double tot = 0.0;
+    long long cl = 0;
+    for (size_t k = 0; k < c.ev.size(); k++) {
+        float ms = 0.f;
+        rc = check_hip(hipEventElapsedTime(&ms, c.ev[k].first, c.ev[k].second), "elapsed");
+        if (rc) return rc;
+        tot += ms;
+        cl += c.ev_cells[k];
+    }
+    if (kernel_ms) *kernel_ms = tot;
+    if (launches) *launches = (long long)c.ev.size();
+    if (cells) *cells = cl;
+    return HIPCLAW_OK;
 }
 
 int hipclaw_cell_updates(hipclaw_ctx* ctx, long long* n)

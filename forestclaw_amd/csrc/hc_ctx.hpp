@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <utility>
 #include <vector>
 
 #include "../../include/hipclaw.h"
@@ -64,6 +65,11 @@ struct Ctx {
     double refine_threshold = 0.0, coarsen_threshold = 0.0;
     int tag_ivar = 0;
     long long cell_updates = 0;    // sum over patch-steps of mx*my
+    // HIP-event timing of the patch-step launches (on this context's stream)
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<long long> ev_cells;  // interior cells updated by each launch
+    void prof_clear();
 
     explicit Ctx(const hipclaw_physics& p, const hipclaw_domain& d);
     ~Ctx();

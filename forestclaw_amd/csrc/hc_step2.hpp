@@ -38,6 +38,7 @@ struct StepArgs {
     double dt;
     const double* dxdy;    // per patch (dx, dy)
     const int* patch_ids;  // optional indirection (nullptr = identity)
+    int ntiles;            // tiles addressable in q / aux / dxdy / cflp
     long long qstride;     // elements per patch in q (NX*NY*meqn)
     long long astride;     // elements per patch in aux (NX*NY*maux)
 };
@@ -68,8 +69,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int slot = lane / A.lpp;
     const int lp = lane - slot * A.lpp;
     const int pidx = gwave * A.ppw + slot;
-    const bool pact = (slot < A.ppw) && (pidx < A.npatch);
-    const int patch = pact ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
+    const bool pin = (slot < A.ppw) && (pidx < A.npatch);
+    const int praw = pin ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
+    // a list entry outside the tile range is skipped, never dereferenced
+    const bool pact = pin && praw >= 0 && praw < A.ntiles;
+    const int patch = pact ? praw : 0;
     const int NX = A.mx + 4, NY = A.my + 4;
 
     const R* __restrict__ qp = qin + (long long)patch * A.qstride;

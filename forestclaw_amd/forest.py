@@ -176,6 +176,16 @@ class Hipclaw46:
         _check(_lib().hipclaw_cell_updates(self.h, ctypes.byref(n)))
         return n.value
 
+    def profile(self, enable: bool = True):
+        _check(_lib().hipclaw_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        ms = ctypes.c_double()
+        nl, nc = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib().hipclaw_profile_read(self.h, ctypes.byref(ms), ctypes.byref(nl),
+                                           ctypes.byref(nc)))
+        return ms.value, nl.value, nc.value
+
     def run(self, tfinal: float, nout: int = 1, dt0: float | None = None, max_steps: int = 10**9,
             on_step=None):
         """fclaw2d_run outstyle 1 (src/fclaw2d_run.c:119-297): variable dt from the

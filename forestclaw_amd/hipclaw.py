@@ -59,13 +59,14 @@ _SIGS = {
     "hipclaw_physics_default": ([ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.POINTER(Physics)], ctypes.c_int),
     "hipclaw_step2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP, _VP,
-                             ctypes.c_double, _VP, ctypes.POINTER(Launch), _VP], ctypes.c_int),
+                             ctypes.c_int, ctypes.c_double, _VP, ctypes.POINTER(Launch), _VP],
+                            ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
     "hipclaw_swirl_aux_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP, _VP],
                                 ctypes.c_int),
-    "hipclaw_bc2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP],
-                          ctypes.c_int),
+    "hipclaw_bc2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, ctypes.c_int, _VP,
+                           _VP], ctypes.c_int),
 }
 
 
@@ -94,6 +95,9 @@ _SIGS.update({
     "hipclaw_initialize": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_mass": ([_VP, _DP], ctypes.c_int),
     "hipclaw_cell_updates": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "hipclaw_profile": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_profile_read": ([_VP, _DP, ctypes.POINTER(ctypes.c_longlong),
+                              ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
 })
 
 
@@ -155,13 +159,18 @@ def physics(solver: int, mx: int, my: int, *, precision: int = FP64, order=(2, 2
 
 
 def step2_batch(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, cfl=None,
-                patch_ids=None, launch: tuple[int, int] | None = None, stream=None) -> None:
+                patch_ids=None, launch: tuple[int, int] | None = None, stream=None,
+                ntiles: int | None = None) -> None:
+    """npatch tiles (or the npatch entries of patch_ids) of buffers holding
+    ntiles tiles (default npatch)."""
+    if ntiles is None:
+        ntiles = npatch
     ln = None
     if launch is not None:
         ln = Launch(*launch)
     _check(lib().hipclaw_step2_batch(
         ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy),
-        _ptr(patch_ids), float(dt), _ptr(cfl), ctypes.byref(ln) if ln is not None else None,
+        _ptr(patch_ids), int(ntiles), float(dt), _ptr(cfl), ctypes.byref(ln) if ln is not None else None,
         _stream(stream)))
 
 
@@ -175,6 +184,9 @@ def swirl_aux_batch(phys: Physics, npatch: int, xylower, dxdy, t: float, dt: flo
                                          t, dt, tperiod, _ptr(aux), _stream(stream)))
 
 
-def bc2_batch(phys: Physics, npatch: int, q, mthbc, patch_ids=None, stream=None) -> None:
+def bc2_batch(phys: Physics, npatch: int, q, mthbc, patch_ids=None, stream=None,
+              ntiles: int | None = None) -> None:
+    if ntiles is None:
+        ntiles = npatch
     _check(lib().hipclaw_bc2_batch(ctypes.byref(phys), npatch, _ptr(q), _ptr(patch_ids),
-                                   _ptr(mthbc), _stream(stream)))
+                                   int(ntiles), _ptr(mthbc), _stream(stream)))

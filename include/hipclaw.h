@@ -89,12 +89,15 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out);
  * aux    device, npatch aux tiles (maux fields) or NULL when maux == 0
  * dxdy   device, 2 doubles per patch (dx, dy)
  * patch_ids device, optional list of npatch tile indices (NULL = 0..npatch-1)
- * cfl_patch device, npatch doubles: the per-patch Courant number returned by
- *        clawpack46_update (may be NULL)
+ * ntiles number of tiles addressable in q_in / q_out / aux / dxdy / cfl_patch;
+ *        list entries outside [0, ntiles) are skipped (dxdy and cfl_patch
+ *        are indexed by tile)
+ * cfl_patch device, one double per tile: the per-patch Courant number
+ *        returned by clawpack46_update (may be NULL)
  * ------------------------------------------------------------------------ */
 int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_in,
                         void* q_out, const void* aux, const double* dxdy,
-                        const int* patch_ids, double dt, double* cfl_patch,
+                        const int* patch_ids, int ntiles, double dt, double* cfl_patch,
                         const hipclaw_launch* launch, void* stream);
 
 /* Maximum of n device doubles into *out_dev (device), the level CFL of
@@ -111,9 +114,10 @@ int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const doubl
 /* Physical boundary conditions, clawpack46_bc2_default
  * (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_bc2_default.f:1-213),
  * per patch mthbc[4] (-1 where the face is not on the physical boundary,
- * as clawpack46_bc2 builds it, fc2d_clawpack46.cpp:120-128). */
+ * as clawpack46_bc2 builds it, fc2d_clawpack46.cpp:120-128), indexed by list
+ * entry; tiles outside [0, ntiles) are skipped. */
 int hipclaw_bc2_batch(const hipclaw_physics* phys, int npatch, void* q, const int* patch_ids,
-                      const int* mthbc, void* stream);
+                      int ntiles, const int* mthbc, void* stream);
 
 
 /* ------------------------------------------------------------------------
@@ -197,6 +201,14 @@ int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, i
 int hipclaw_mass(hipclaw_ctx* ctx, double* sums);
 /* Cell updates performed so far (count_single_step * mx * my). */
 int hipclaw_cell_updates(hipclaw_ctx* ctx, long long* n);
+
+/* HIP-event timing of every patch-step launch on the context's stream (the
+ * ADVANCE_STEP2 timer of src/fclaw_timer.h on device events).  enable != 0
+ * starts a fresh record; _read synchronises and returns the summed kernel
+ * time, the number of launches and the interior cells they updated. */
+int hipclaw_profile(hipclaw_ctx* ctx, int enable);
+int hipclaw_profile_read(hipclaw_ctx* ctx, double* kernel_ms, long long* launches,
+                         long long* cells);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,8 @@ def run_gpu(dev, kind, q, aux, dx, dy, dt, *, order=(2, 2), mthlim=None, precisi
         np.float64), dev)
     cfl = torch.full((n,), -1.0, dtype=torch.float64, device=dev)
     ids = None if patch_ids is None else _t(np.asarray(patch_ids, dtype=np.int32), dev)
-    H.step2_batch(p, n, qin, qout, a, dxdy, dt, cfl, patch_ids=ids, launch=launch)
+    npatch = n if patch_ids is None else len(patch_ids)
+    H.step2_batch(p, npatch, qin, qout, a, dxdy, dt, cfl, patch_ids=ids, launch=launch, ntiles=n)
     torch.cuda.synchronize()
     return qout.double().cpu().numpy(), cfl.cpu().numpy()
 
@@ -132,8 +133,9 @@ def test_patch_ids_and_inplace(dev):
     rng = np.random.default_rng(7)
     q, aux, dx, dy, dt = _batch("swe", rng, 12, 32, 32)
     ref, rcfl = _oracle_batch("swe", q, aux, dx, dy, dt)
-    # step only every other tile, in place, in a permuted order
-    ids = [10, 2, 6, 0, 8, 4]
+    # step only every other tile, in place, in a permuted order; list entries
+    # outside the tile range are skipped by the kernel
+    ids = [10, 2, 6, 0, 8, 4, 12, -1]
     got, cfl = run_gpu(dev, "swe", q, aux, dx, dy, dt, inplace=True, patch_ids=ids)
     inner = (slice(None), slice(2, -2), slice(2, -2))
     for t in range(12):
