@@ -549,6 +549,18 @@ int hipclaw_init_uniform(hipclaw_ctx* ctx, int level)
     return c.rebuild(false, nullptr);
 }
 
+int hipclaw_init_leaves(hipclaw_ctx* ctx, int nleaves, const int* level, const int* gx, const int* gy)
+{
+    CTX_CHECK(ctx);
+    Ctx& c = *ctx->c;
+    if (nleaves < 1 || !level || !gx || !gy) return set_error(HIPCLAW_ERR_ARG, "bad leaf list");
+    std::vector<Leaf> in((size_t)nleaves);
+    for (int k = 0; k < nleaves; k++) in[k] = {level[k], gx[k], gy[k]};
+    std::string err = c.forest.set_leaves(in);
+    if (!err.empty()) return set_error(HIPCLAW_ERR_ARG, err);
+    return c.rebuild(false, nullptr);
+}
+
 int hipclaw_num_patches(hipclaw_ctx* ctx, int* n)
 {
     CTX_CHECK(ctx);

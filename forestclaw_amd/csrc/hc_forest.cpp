@@ -90,6 +90,50 @@ void Forest::init_uniform(int level)
     sort_and_index();
 }
 
+std::string Forest::set_leaves(const std::vector<Leaf>& in)
+{
+    if (in.empty()) return "empty leaf list";
+    int top = 0;
+    for (const Leaf& l : in) {
+        if (l.level < minlevel || l.level > maxlevel) return "leaf level outside [minlevel, maxlevel]";
+        if (!inside(l.level, l.gx, l.gy)) return "leaf outside the brick";
+        top = std::max(top, l.level);
+    }
+    // Cover: the areas (in finest-level cells) add up to the brick and no
+    // leaf has an ancestor in the list; together these make an exact cover.
+    unsigned long long area = 0;
+    for (const Leaf& l : in) area += 1ull << (2 * (top - l.level));
+    if (area != (unsigned long long)mi * mj << (2 * top)) return "leaves do not cover the brick exactly";
+    std::vector<Leaf> old;
+    old.swap(leaves);
+    leaves = in;
+    sort_and_index();
+    std::string err;
+    if (index_.size() != leaves.size()) err = "duplicate leaves";
+    for (int k = 0; err.empty() && k < (int)leaves.size(); k++) {
+        const Leaf& l = leaves[k];
+        for (int a = l.level - 1; a >= 0 && err.empty(); a--)
+            if (find(a, l.gx >> (l.level - a), l.gy >> (l.level - a)) >= 0) err = "nested leaves";
+    }
+    // 2:1 balance: the leaf covering each of the eight neighbours of a leaf
+    // at its own level is at most one level coarser.  (A finer neighbour is
+    // checked from its own side.)
+    for (int k = 0; err.empty() && k < (int)leaves.size(); k++) {
+        const Leaf& l = leaves[k];
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                if ((dx == 0 && dy == 0) || !inside(l.level, l.gx + dx, l.gy + dy)) continue;
+                int c = covering(l.level, l.gx + dx, l.gy + dy);
+                if (c >= 0 && leaves[c].level < l.level - 1) err = "leaves are not 2:1 balanced";
+            }
+    }
+    if (!err.empty()) {
+        leaves.swap(old);
+        sort_and_index();
+    }
+    return err;
+}
+
 int Forest::face_neighbors(int leaf, int face, int nb[2]) const
 {
     const Leaf& l = leaves[leaf];

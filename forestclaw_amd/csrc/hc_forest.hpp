@@ -10,6 +10,7 @@
 // src/forestclaw2d.c:655-663, 798-841).
 #pragma once
 #include <cstdint>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -53,6 +54,11 @@ public:
     int block_of(const Leaf& l) const { return (l.gy >> l.level) * mi + (l.gx >> l.level); }
 
     void init_uniform(int level);
+    // Replace the leaves by a given list.  Returns an empty string on success,
+    // else why the list is not a valid forest: every leaf inside the brick and
+    // within [minlevel, maxlevel], no duplicates or nested leaves, an exact
+    // cover of the brick, and 2:1 balance across faces and corners.
+    std::string set_leaves(const std::vector<Leaf>& in);
 
     // face 0 left, 1 right, 2 bottom, 3 top.  Returns the kind; nb[0..1]
     // receive neighbour leaves (two for NB_FINE, ordered by increasing

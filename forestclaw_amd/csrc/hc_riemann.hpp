@@ -89,17 +89,17 @@ template <class R> struct Swe {
         R hsl = MM::sqrt_(qa[0]);
         R hsr = MM::sqrt_(qb[0]);
         R hsq2 = hsl + hsr;
-        R u = MM::div(MM::div(qa[mu], hsl) + MM::div(qb[mu], hsr), hsq2);
-        R v = MM::div(MM::div(qa[mv], hsl) + MM::div(qb[mv], hsr), hsq2);
+        R u = MM::divp(MM::divp(qa[mu], hsl) + MM::divp(qb[mu], hsr), hsq2);
+        R v = MM::divp(MM::divp(qa[mv], hsl) + MM::divp(qb[mv], hsr), hsq2);
         R a = MM::sqrt_(g * h);
         roe.u = u; roe.v = v; roe.a = a;
 
         R d0 = qb[0] - qa[0];
         R d1 = qb[mu] - qa[mu];
         R d2 = qb[mv] - qa[mv];
-        R a1 = MM::div((u + a) * d0 - d1, R(2) * a);
+        R a1 = MM::divp((u + a) * d0 - d1, R(2) * a);
         R a2 = -v * d0 + d2;
-        R a3 = MM::div(-(u - a) * d0 + d1, R(2) * a);
+        R a3 = MM::divp(-(u - a) * d0 + d1, R(2) * a);
 
         f.w[0][0] = a1; f.w[0][mu] = a1 * (u - a); f.w[0][mv] = a1 * v;
         f.s[0] = u - a;
@@ -122,14 +122,14 @@ template <class R> struct Swe {
 
         // entropy fix: amdq from the left-going parts
         do {
-            R s0 = MM::div(qa[mu], qa[0]) - MM::sqrt_(g * qa[0]);
+            R s0 = MM::divp(qa[mu], qa[0]) - MM::sqrt_(g * qa[0]);
             if (s0 > R(0) && f.s[0] > R(0)) {
                 f.am[0] = R(0); f.am[1] = R(0); f.am[2] = R(0);
                 break;
             }
             R h1 = qa[0] + f.w[0][0];
             R hu1 = qa[mu] + f.w[0][mu];
-            R s1 = MM::div(hu1, h1) - MM::sqrt_(g * h1);
+            R s1 = MM::divp(hu1, h1) - MM::sqrt_(g * h1);
             R sfract;
             if (s0 < R(0) && s1 > R(0))
                 sfract = s0 * MM::div(s1 - f.s[0], s1 - s0);
@@ -140,10 +140,10 @@ template <class R> struct Swe {
             for (int m = 0; m < 3; m++) f.am[m] = sfract * f.w[0][m];
             if (f.s[1] > R(0)) break;
             for (int m = 0; m < 3; m++) f.am[m] = f.am[m] + f.s[1] * f.w[1][m];
-            R s03 = MM::div(qb[mu], qb[0]) + MM::sqrt_(g * qb[0]);
+            R s03 = MM::divp(qb[mu], qb[0]) + MM::sqrt_(g * qb[0]);
             R h3 = qb[0] - f.w[2][0];
             R hu3 = qb[mu] - f.w[2][mu];
-            R s3 = MM::div(hu3, h3) + MM::sqrt_(g * h3);
+            R s3 = MM::divp(hu3, h3) + MM::sqrt_(g * h3);
             if (s3 < R(0) && s03 > R(0))
                 sfract = s3 * MM::div(s03 - f.s[2], s03 - s3);
             else if (f.s[2] < R(0))
@@ -167,9 +167,9 @@ template <class R> struct Swe {
         constexpr int mv = DIR == 0 ? 2 : 1;
         using MM = M<R>;
         const R u = roe.u, v = roe.v, a = roe.a;
-        R a1 = MM::div((v + a) * asdq[0] - asdq[mv], R(2) * a);
+        R a1 = MM::divp((v + a) * asdq[0] - asdq[mv], R(2) * a);
         R a2 = asdq[mu] - u * asdq[0];
-        R a3 = MM::div(-(v - a) * asdq[0] + asdq[mv], R(2) * a);
+        R a3 = MM::divp(-(v - a) * asdq[0] + asdq[mv], R(2) * a);
         R wb[3][3], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
@@ -204,23 +204,23 @@ template <class R> struct Euler5 {
         const R g1 = R(pp.gamma - 1.0);
         R rl = MM::sqrt_(qa[0]);
         R rr = MM::sqrt_(qb[0]);
-        R pl = g1 * (qa[3] - MM::div(R(0.5) * (qa[1] * qa[1] + qa[2] * qa[2]), qa[0]));
-        R pr = g1 * (qb[3] - MM::div(R(0.5) * (qb[1] * qb[1] + qb[2] * qb[2]), qb[0]));
+        R pl = g1 * (qa[3] - MM::divp(R(0.5) * (qa[1] * qa[1] + qa[2] * qa[2]), qa[0]));
+        R pr = g1 * (qb[3] - MM::divp(R(0.5) * (qb[1] * qb[1] + qb[2] * qb[2]), qb[0]));
         R rsq2 = rl + rr;
-        R u = MM::div(MM::div(qa[mu], rl) + MM::div(qb[mu], rr), rsq2);
-        R v = MM::div(MM::div(qa[mv], rl) + MM::div(qb[mv], rr), rsq2);
-        R enth = MM::div(MM::div(qa[3] + pl, rl) + MM::div(qb[3] + pr, rr), rsq2);
+        R u = MM::divp(MM::divp(qa[mu], rl) + MM::divp(qb[mu], rr), rsq2);
+        R v = MM::divp(MM::divp(qa[mv], rl) + MM::divp(qb[mv], rr), rsq2);
+        R enth = MM::divp(MM::divp(qa[3] + pl, rl) + MM::divp(qb[3] + pr, rr), rsq2);
         R u2v2 = u * u + v * v;
         R a2c = g1 * (enth - R(.5) * u2v2);
         R a = MM::sqrt_(a2c);
-        R g1a2 = MM::div(g1, a2c);
+        R g1a2 = MM::divp(g1, a2c);
         R euv = enth - u2v2;
         t.u = u; t.v = v; t.enth = enth; t.a = a; t.g1a2 = g1a2; t.euv = euv; t.u2v2 = u2v2;
 
         R d0 = qb[0] - qa[0], d1 = qb[mu] - qa[mu], d2 = qb[mv] - qa[mv], d3 = qb[3] - qa[3];
         R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
         R a2 = d2 - v * d0;
-        R a4 = MM::div(d1 + (a - u) * d0 - a * a3, R(2) * a);
+        R a4 = MM::divp(d1 + (a - u) * d0 - a * a3, R(2) * a);
         R a1 = d0 - a3 - a4;
 
         f.w[0][0] = a1; f.w[0][mu] = a1 * (u - a); f.w[0][mv] = a1 * v;
@@ -250,9 +250,9 @@ template <class R> struct Euler5 {
 
         do {
             R rhoim1 = qa[0];
-            R pim1 = g1 * (qa[3] - MM::div(R(0.5) * (qa[mu] * qa[mu] + qa[mv] * qa[mv]), rhoim1));
-            R cim1 = MM::sqrt_(MM::div(gam * pim1, rhoim1));
-            R s0 = MM::div(qa[mu], rhoim1) - cim1;
+            R pim1 = g1 * (qa[3] - MM::divp(R(0.5) * (qa[mu] * qa[mu] + qa[mv] * qa[mv]), rhoim1));
+            R cim1 = MM::sqrt_(MM::divp(gam * pim1, rhoim1));
+            R s0 = MM::divp(qa[mu], rhoim1) - cim1;
             if (s0 >= R(0) && f.s[0] > R(0)) {
                 for (int m = 0; m < 5; m++) f.am[m] = R(0);
                 break;
@@ -261,9 +261,9 @@ template <class R> struct Euler5 {
             R rhou1 = qa[mu] + f.w[0][mu];
             R rhov1 = qa[mv] + f.w[0][mv];
             R en1 = qa[3] + f.w[0][3];
-            R p1 = g1 * (en1 - MM::div(R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1), rho1));
-            R c1 = MM::sqrt_(MM::div(gam * p1, rho1));
-            R s1 = MM::div(rhou1, rho1) - c1;
+            R p1 = g1 * (en1 - MM::divp(R(0.5) * (rhou1 * rhou1 + rhov1 * rhov1), rho1));
+            R c1 = MM::sqrt_(MM::divp(gam * p1, rho1));
+            R s1 = MM::divp(rhou1, rho1) - c1;
             R sfract;
             if (s0 < R(0) && s1 > R(0))
                 sfract = MM::div(s0 * (s1 - f.s[0]), s1 - s0);
@@ -279,16 +279,16 @@ template <class R> struct Euler5 {
                 f.am[m] = f.am[m] + f.s[4] * f.w[4][m];
             }
             R rhoi = qb[0];
-            R pi_ = g1 * (qb[3] - MM::div(R(0.5) * (qb[mu] * qb[mu] + qb[mv] * qb[mv]), rhoi));
-            R ci = MM::sqrt_(MM::div(gam * pi_, rhoi));
-            R s3 = MM::div(qb[mu], rhoi) + ci;
+            R pi_ = g1 * (qb[3] - MM::divp(R(0.5) * (qb[mu] * qb[mu] + qb[mv] * qb[mv]), rhoi));
+            R ci = MM::sqrt_(MM::divp(gam * pi_, rhoi));
+            R s3 = MM::divp(qb[mu], rhoi) + ci;
             R rho2 = qb[0] - f.w[3][0];
             R rhou2 = qb[mu] - f.w[3][mu];
             R rhov2 = qb[mv] - f.w[3][mv];
             R en2 = qb[3] - f.w[3][3];
-            R p2 = g1 * (en2 - MM::div(R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2), rho2));
-            R c2 = MM::sqrt_(MM::div(gam * p2, rho2));
-            R s2 = MM::div(rhou2, rho2) + c2;
+            R p2 = g1 * (en2 - MM::divp(R(0.5) * (rhou2 * rhou2 + rhov2 * rhov2), rho2));
+            R c2 = MM::sqrt_(MM::divp(gam * p2, rho2));
+            R s2 = MM::divp(rhou2, rho2) + c2;
             if (s2 < R(0) && s3 > R(0))
                 sfract = MM::div(s2 * (s3 - f.s[3]), s3 - s2);
             else if (f.s[3] < R(0))
@@ -315,7 +315,7 @@ template <class R> struct Euler5 {
                 u2v2 = t.u2v2;
         R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
         R a2 = asdq[mu] - u * asdq[0];
-        R a4 = MM::div(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
+        R a4 = MM::divp(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
         R a1 = asdq[0] - a3 - a4;
         R wb[4][5], sb[4];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
@@ -346,7 +346,7 @@ template <class R> __device__ inline R limiter_phi(int mthlim, R r)
     switch (mthlim) {
     case 1: return dmax(R(0), dmin(R(1), r));
     case 2: return dmax(dmax(R(0), dmin(R(1), R(2) * r)), dmin(R(2), r));
-    case 3: return M<R>::div(r + M<R>::abs_(r), R(1) + M<R>::abs_(r));
+    case 3: return M<R>::divp(r + M<R>::abs_(r), R(1) + M<R>::abs_(r));
     case 4: {
         R c = (R(1) + r) / R(2);
         return dmax(R(0), dmin(dmin(c, R(2)), R(2) * r));

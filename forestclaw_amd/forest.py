@@ -125,6 +125,13 @@ class Hipclaw46:
     def init_uniform(self, level: int):
         _check(_lib().hipclaw_init_uniform(self.h, level))
 
+    def init_leaves(self, level, gx, gy):
+        """Forest from an explicit leaf list; patches come back renumbered."""
+        a = [np.ascontiguousarray(v, dtype=np.int32) for v in (level, gx, gy)]
+        if not (len(a[0]) == len(a[1]) == len(a[2])):
+            raise ValueError("level, gx, gy must have equal lengths")
+        _check(_lib().hipclaw_init_leaves(self.h, len(a[0]), *(v.ctypes.data_as(_IP) for v in a)))
+
     def set_q(self, q: np.ndarray):
         q = np.ascontiguousarray(q, dtype=np.float64)
         assert q.shape == (self.num_patches(),) + self.tile_shape

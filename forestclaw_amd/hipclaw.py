@@ -77,6 +77,7 @@ _SIGS.update({
                             ctypes.POINTER(_VP)], ctypes.c_int),
     "hipclaw_ctx_destroy": ([_VP], ctypes.c_int),
     "hipclaw_init_uniform": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
     "hipclaw_num_patches": ([_VP, _IP], ctypes.c_int),
     "hipclaw_patch_info": ([_VP, _IP, _IP, _IP, _DP, _DP, _DP, _DP], ctypes.c_int),
     "hipclaw_set_q": ([_VP, _DP], ctypes.c_int),
@@ -108,6 +109,14 @@ def lib() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise HipclawError(
                 f"{LIB_PATH} is missing: build it with `python -m forestclaw_amd.build`")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Loading torch first lets the dynamic linker bind
+        # this library to that runtime, so the process has exactly one HIP
+        # runtime and device pointers, streams and allocations are shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in _SIGS.items():
             fn = getattr(L, name)

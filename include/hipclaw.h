@@ -152,6 +152,13 @@ int hipclaw_ctx_destroy(hipclaw_ctx* ctx);
 
 /* Uniform forest at one level (fclaw2d_domain_new_brick / _unitsquare). */
 int hipclaw_init_uniform(hipclaw_ctx* ctx, int level);
+/* Forest from an explicit leaf list (level, gx, gy), e.g. to restore a saved
+ * mesh (the reference's restart path, src/fclaw2d_forestclaw.c) or to run two
+ * precisions on one mesh.  The list must cover the brick exactly with levels in
+ * [minlevel, maxlevel] and be 2:1 balanced across faces and corners; patches are
+ * renumbered into level-then-Morton order (read them back with patch_info). */
+int hipclaw_init_leaves(hipclaw_ctx* ctx, int nleaves, const int* level, const int* gx,
+                        const int* gy);
 int hipclaw_num_patches(hipclaw_ctx* ctx, int* n);
 /* Per-patch host arrays of length num_patches (any pointer may be NULL). */
 int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* xlower,

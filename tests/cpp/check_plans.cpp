@@ -209,8 +209,54 @@ static void check_forest(const Forest& F, int mx, int my, const char* name)
     REQUIRE(missing == 0, "%s: %lld ghost cells are never written", name, missing);
 }
 
+// Forest::set_leaves accepts every adapted forest (in any order, giving the
+// same numbering) and rejects gaps, duplicates, nesting and imbalance.
+static void check_set_leaves(const Forest& F)
+{
+    std::vector<Leaf> in(F.leaves.rbegin(), F.leaves.rend());
+    Forest G(F.mi, F.mj, F.minlevel, F.maxlevel);
+    std::string err = G.set_leaves(in);
+    REQUIRE(err.empty(), "set_leaves rejected an adapted forest: %s", err.c_str());
+    bool same = G.nleaves() == F.nleaves();
+    for (int k = 0; same && k < F.nleaves(); k++)
+        same = G.leaves[k].level == F.leaves[k].level && G.leaves[k].gx == F.leaves[k].gx &&
+               G.leaves[k].gy == F.leaves[k].gy;
+    REQUIRE(same, "set_leaves numbering differs from the adapted forest");
+    std::vector<Leaf> gap(F.leaves.begin() + 1, F.leaves.end());
+    REQUIRE(!G.set_leaves(gap).empty(), "gap accepted");
+    REQUIRE(G.nleaves() == F.nleaves(), "a rejected list changed the forest");
+    std::vector<Leaf> dup(F.leaves);
+    dup.back() = dup.front();
+    REQUIRE(!G.set_leaves(dup).empty(), "duplicate accepted");
+    // split a leaf into its four children but keep the parent: nested (area off too)
+    const Leaf p = F.leaves.front();
+    if (p.level < F.maxlevel) {
+        std::vector<Leaf> nest(F.leaves);
+        for (int c = 0; c < 4; c++) nest.push_back({p.level + 1, 2 * p.gx + (c & 1), 2 * p.gy + (c >> 1)});
+        REQUIRE(!G.set_leaves(nest).empty(), "nested leaves accepted");
+    }
+}
+
 int main()
 {
+    {
+        // imbalance: a level-1 leaf next to level-3 leaves
+        Forest F(1, 1, 1, 3);
+        std::vector<Leaf> in = {{1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
+        for (int y = 0; y < 2; y++)
+            for (int x = 0; x < 2; x++)
+                if (x == 1 && y == 1) {
+                    for (int c = 0; c < 4; c++) in.push_back({3, 2 + (c & 1), 2 + (c >> 1)});
+                } else {
+                    in.push_back({2, x, y});
+                }
+        std::string err = F.set_leaves(in);
+        REQUIRE(err.find("balanced") != std::string::npos, "imbalanced forest accepted (%s)", err.c_str());
+        in.resize(3);
+        for (int y = 0; y < 2; y++)
+            for (int x = 0; x < 2; x++) in.push_back({2, x, y});
+        REQUIRE(F.set_leaves(in).empty(), "balanced forest rejected");
+    }
     const int mx = 8, my = 8;
     {
         Forest F(1, 1, 3, 3);
@@ -241,6 +287,7 @@ int main()
             char name[64];
             std::snprintf(name, sizeof(name), "random trial %d round %d", trial, round);
             check_forest(F, mx, my, name);
+            check_set_leaves(F);
         }
     }
     std::printf("checked %lld conditions, %d failures\n", g_checked, g_err);

@@ -47,6 +47,7 @@ def interiors(arrs):
 
 
 DAM = (0.5, 0.5, 0.23, 2.0, 1.0)
+SMALL_DAM = (0.3, 0.35, 0.15, 2.0, 1.0)  # touches 4 of the 16 level-2 patches
 
 
 def test_uniform_swe_bitwise(dev):
@@ -67,10 +68,10 @@ def test_uniform_swe_bitwise(dev):
 
 @pytest.mark.parametrize("subcycle", [True, False])
 def test_amr_swe_subcycled_bitwise(dev, subcycle):
-    sim = make(H.SOLVER_SWE_ROE, 8, 1, 3, refine=0.5, subcycle=subcycle)
-    sim.initialize("radial_dam", (0.4, 0.45, 0.2, 2.0, 1.0), nsub=4)
+    sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, refine=0.5, subcycle=subcycle)
+    sim.initialize("radial_dam", SMALL_DAM, nsub=4)
     lev = sim.patch_info()["level"]
-    assert lev.min() == 1 and lev.max() == 3, "test needs three levels"
+    assert lev.min() == 2 and lev.max() == 4, f"test needs three levels, got {sorted(set(lev))}"
     amr = oracle_from(sim, O.SOLVER_SWE_ROE)
     dt = 0.01
     for s in range(3):
@@ -111,8 +112,8 @@ def test_swirl_advection_device_velocities(dev):
 
 
 def test_regrid_transfers_match_oracle_kernels(dev):
-    sim = make(H.SOLVER_SWE_ROE, 8, 1, 3, refine=0.5, coarsen=0.2)
-    sim.initialize("radial_dam", (0.4, 0.45, 0.2, 2.0, 1.0), nsub=4)
+    sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, refine=0.5, coarsen=0.2)
+    sim.initialize("radial_dam", SMALL_DAM, nsub=4)
     for s in range(3):
         sim.advance(s * 0.01, 0.01)
     old_info = sim.patch_info()
@@ -146,8 +147,8 @@ def test_regrid_transfers_match_oracle_kernels(dev):
 
 
 def test_fp32_amr_error_vs_fp64_oracle(dev):
-    sim = make(H.SOLVER_SWE_ROE, 8, 1, 3, precision=H.FP32, refine=0.5)
-    sim.initialize("radial_dam", (0.4, 0.45, 0.2, 2.0, 1.0), nsub=4)
+    sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, precision=H.FP32, refine=0.5)
+    sim.initialize("radial_dam", SMALL_DAM, nsub=4)
     amr = oracle_from(sim, O.SOLVER_SWE_ROE)  # starts from the FP32-rounded data
     dt = 0.01
     for s in range(3):

@@ -213,8 +213,11 @@ def test_swirl_aux_device(dev):
     got = aux.cpu().numpy()
     for k in range(n):
         ref = O.swirl_aux(mx, my, 2, xy[k, 0], xy[k, 1], dxdy[k, 0], dxdy[k, 1], 1.3, 0.002, 8.0)
-        scale = np.abs(ref).max()
-        assert np.abs(got[k] - ref).max() <= 1e-14 * scale
+        # device sin() and glibc sin() may differ by an ulp; an edge velocity is a
+        # difference of two stream-function values (|psi| <= 1/pi) divided by dx,
+        # so a few ulps of psi become eps/pi/dx ~ 2e-14 here: allow 16 of them
+        tol = 16 * np.finfo(np.float64).eps * (1 / np.pi) / dxdy[k, 0]
+        assert np.abs(got[k] - ref).max() <= tol
 
 
 def test_max_reduce_exact(dev):
