@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 plain device copies of 1 GiB (float32), the reference "
+                         "kernel for scaling hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
@@ -149,6 +152,14 @@ def main():
     levels = {int(L): int((info["level"] == L).sum()) for L in np.unique(info["level"])}
     c_max = 1.0 + np.sqrt(1.0 * 1.6)
     dt = 0.9 * info["dx"].max() / c_max  # CFL 0.9 at the coarsest level (subcycled)
+
+    # launch shape of the patch-step kernel picked by the autotuner on the
+    # level with the most patches (timed into a scratch buffer)
+    tuned = {}
+    for name, sim in (("fp32", sim32), ("fp64", sim64)):
+        best, sweep = sim.autotune(-1, 3)
+        tuned[name] = {"cols_per_lane": best[0], "waves_per_group": best[1],
+                       "sweep_ms": {f"cpl{c}_wpg{w}": round(v, 4) for (c, w), v in sweep.items()}}
 
     r32 = timed(sim32, args, dt)
     r64 = None
@@ -188,7 +199,8 @@ def main():
         "config": {"workload": "SWE Roe, unit square, 32x32 patches, base level 7 uniform "
                                "refined to level 9 around 8 seeded humps, subcycled",
                    "patches_per_level": levels, "dt_coarse": dt,
-                   "cell_updates_per_step": r32["cells"] // args.steps},
+                   "cell_updates_per_step": r32["cells"] // args.steps,
+                   "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "k_step2<float,Swe>",
@@ -206,6 +218,14 @@ def main():
                              "kernel_only_rate": r64["kcells"] / k64 if k64 > 0 else None}
         res["speedup_vs_fp64_plain"] = value / v64
         res["fp32_vs_fp64"] = err
+    if args.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        res["copy_calibration_bytes"] = {"read": 1 << 30, "write": 1 << 30}
+        del x, y
     print(json.dumps(res))
 
 

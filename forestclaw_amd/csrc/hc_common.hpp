@@ -16,29 +16,58 @@ namespace hc {
 //         2*sound speed, 1+|r|): the FP32 path uses the hardware reciprocal
 //         (about 1 ulp), far inside its stated error budget
 //   sqrt_ FP32: hardware square root (about 1 ulp) of depths / pressures
+//   lim_ok / divl  the limiter ratio r = dot / |wave|^2.  FP64 limits every
+//         wave with a nonzero norm, as the reference does.  FP32 limits waves
+//         whose squared norm is a normal number and divides with the hardware
+//         reciprocal; a wave with a denormal squared norm (|wave| < 1.1e-19)
+//         is left unlimited, which moves q by less than dt/dx*|s|*1.1e-19.
 template <class R> struct M;
 template <> struct M<double> {
     __device__ static inline double sqrt_(double x) { return __builtin_sqrt(x); }
     __device__ static inline double div(double a, double b) { return a / b; }
     __device__ static inline double divp(double a, double b) { return a / b; }
     __device__ static inline double abs_(double x) { return __builtin_fabs(x); }
+    __device__ static inline bool lim_ok(double wn) { return wn != 0.0; }
+    __device__ static inline double divl(double a, double wn) { return a / wn; }
 };
 template <> struct M<float> {
     __device__ static inline float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ static inline float div(float a, float b) { return a / b; }
     __device__ static inline float divp(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
     __device__ static inline float abs_(float x) { return __builtin_fabsf(x); }
+    __device__ static inline bool lim_ok(float wn) { return wn >= 1.17549435e-38f; }
+    __device__ static inline float divl(float a, float wn) { return a * __builtin_amdgcn_rcpf(wn); }
 };
 
 template <class R> __device__ inline R dmin(R a, R b) { return a < b ? a : b; }
 template <class R> __device__ inline R dmax(R a, R b) { return a > b ? a : b; }
 
-// Whole-wave neighbour exchange (lane l receives lane l-1 / l+1).  Lanes at
-// the wave edges get their own value back; callers never consume those.
-__device__ inline float shl_up(float v) { return __shfl_up(v, 1, HC_WAVE); }
-__device__ inline float shl_dn(float v) { return __shfl_down(v, 1, HC_WAVE); }
-__device__ inline double shl_up(double v) { return __shfl_up(v, 1, HC_WAVE); }
-__device__ inline double shl_dn(double v) { return __shfl_down(v, 1, HC_WAVE); }
+// Whole-wave neighbour exchange (lane l receives lane l-1 / l+1) with the
+// GFX9 DPP wave_shr:1 / wave_shl:1 row controls: one VALU move, no LDS round
+// trip.  Lanes at the wave edges get their own value back (bound_ctrl off,
+// old = source); callers never consume those.  Called only in wave-uniform
+// control flow, so every source lane is active.
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+template <int CTRL> __device__ inline int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL> __device__ inline float dpp_f(float v)
+{
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ inline double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)(b & 0xffffffffll));
+    const int hi = dpp_i<CTRL>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline float shl_up(float v) { return dpp_f<DPP_WAVE_SHR1>(v); }
+__device__ inline float shl_dn(float v) { return dpp_f<DPP_WAVE_SHL1>(v); }
+__device__ inline double shl_up(double v) { return dpp_d<DPP_WAVE_SHR1>(v); }
+__device__ inline double shl_dn(double v) { return dpp_d<DPP_WAVE_SHL1>(v); }
 
 // Non-negative doubles order like their bit patterns.
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }

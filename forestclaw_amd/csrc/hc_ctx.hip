@@ -306,7 +306,7 @@ int Ctx::update_level(int level, double t, double dt)
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
     HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
-                               d_cfl + s0, nullptr, stream));
+                               d_cfl + s0, launch.cols_per_lane > 0 ? &launch : nullptr, stream));
     if (prof) {
         HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
         ev.push_back({e0, e1});
@@ -318,6 +318,63 @@ int Ctx::update_level(int level, double t, double dt)
     }
     parity[level] ^= 1;
     cell_updates += (long long)np * phys.mx * phys.my;
+    return HIPCLAW_OK;
+}
+
+// Time every patch-step launch shape on one level's current data (output into
+// a temporary buffer, so the solution is untouched) and keep the fastest.
+int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
+{
+    if (level < 0) {  // the level with the most patches
+        int bn = -1;
+        for (int L = 0; L < MAXL; L++)
+            if (plan[L].s1 - plan[L].s0 > bn) bn = plan[L].s1 - plan[L].s0, level = L;
+    }
+    if (level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    const int s0 = plan[level].s0, np = plan[level].s1 - s0;
+    if (np <= 0) return set_error(HIPCLAW_ERR_ARG, "level has no patches");
+    if (reps < 1) reps = 1;
+    const char* qin = static_cast<const char*>(pool[parity[level]]) + (size_t)s0 * tile * esz;
+    const char* a = aux ? static_cast<const char*>(aux) + (size_t)s0 * atile * esz : nullptr;
+    void* tmp = nullptr;
+    double* cfl = nullptr;
+    HC_TRY(dmalloc(&tmp, (size_t)np * tile * esz));
+    int rc = dmalloc(reinterpret_cast<void**>(&cfl), sizeof(double) * np);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = check_hip(hipEventCreate(&e0), "event");
+    if (!rc) rc = check_hip(hipEventCreate(&e1), "event");
+    // a small step (Courant number ~0.1 for unit wave speeds) keeps the data valid
+    const double dx = (dom.bx - dom.ax) / dom.mi / (double)(1 << level) / phys.mx;
+    const double dt = 0.1 * dx;
+    float bestms = 1e30f;
+    int k = 0;
+    for (int cpl = 1; cpl <= 2 && !rc; cpl++)
+        for (int wpg : {1, 2, 4}) {
+            if (rc) break;
+            hipclaw_launch l{cpl, wpg};
+            // one untimed launch, then reps timed ones
+            rc = hipclaw_step2_batch(&phys, np, qin, tmp, a, d_dxdy + 2 * s0, nullptr, np, dt, cfl, &l, stream);
+            if (!rc) rc = check_hip(hipEventRecord(e0, stream), "event");
+            for (int r = 0; r < reps && !rc; r++)
+                rc = hipclaw_step2_batch(&phys, np, qin, tmp, a, d_dxdy + 2 * s0, nullptr, np, dt, cfl, &l, stream);
+            if (!rc) rc = check_hip(hipEventRecord(e1, stream), "event");
+            if (!rc) rc = check_hip(hipEventSynchronize(e1), "event");
+            float t = 0;
+            if (!rc) rc = check_hip(hipEventElapsedTime(&t, e0, e1), "event");
+            t /= reps;
+            if (ms) ms[k] = t;
+            k++;
+            if (!rc && t < bestms) {
+                bestms = t;
+                launch = l;
+            }
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(tmp);
+    if (cfl) (void)hipFree(cfl);
+    if (rc) return rc;
+    if (best) *best = launch;
     return HIPCLAW_OK;
 }
 
@@ -559,6 +616,26 @@ int hipclaw_init_leaves(hipclaw_ctx* ctx, int nleaves, const int* level, const i
     std::string err = c.forest.set_leaves(in);
     if (!err.empty()) return set_error(HIPCLAW_ERR_ARG, err);
     return c.rebuild(false, nullptr);
+}
+
+int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch)
+{
+    CTX_CHECK(ctx);
+    if (!launch) {
+        ctx->c->launch = hipclaw_launch{0, 0};
+        return HIPCLAW_OK;
+    }
+    if (launch->cols_per_lane < 1 || launch->cols_per_lane > 2 || launch->waves_per_group < 1 ||
+        launch->waves_per_group > 4)
+        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
+    ctx->c->launch = *launch;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_autotune(hipclaw_ctx* ctx, int level, int reps, hipclaw_launch* best, float* ms)
+{
+    CTX_READY(ctx);
+    return ctx->c->autotune(level, reps, best, ms);
 }
 
 int hipclaw_num_patches(hipclaw_ctx* ctx, int* n)

@@ -61,6 +61,7 @@ struct Ctx {
     std::vector<int> level_of;     // per slot
     double swirl_tperiod = 0.0;    // > 0: advection velocities from the swirl stream function
     int cflacc_count = 0;
+    hipclaw_launch launch{0, 0};   // patch-step launch shape; 0 = library default
     // tagging parameters
     double refine_threshold = 0.0, coarsen_threshold = 0.0;
     int tag_ivar = 0;
@@ -84,6 +85,7 @@ struct Ctx {
     int tag_and_adapt(int init_flag, int* changed);
     int mass(double* sums);
     int level_range(int level, int* s0, int* s1) const;
+    int autotune(int level, int reps, hipclaw_launch* best, float* ms);
 };
 
 }  // namespace hc

@@ -54,7 +54,10 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
     return k < CPL - 1 ? v[k < CPL - 1 ? k + 1 : 0] : s;
 }
 
-template <class R, template <class> class SolverT, int CPL>
+// MC22: compile-time specialisation for the configuration every shipped
+// application uses (order 2, transverse 2, MC limiter on every wave family);
+// the generic instantiation reads these from the arguments.
+template <class R, template <class> class SolverT, int CPL, bool MC22>
 __global__ void __launch_bounds__(256)
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
@@ -87,9 +90,12 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const R hdtdx = R(0.5) * dtdx;
     const R hdtdy = R(0.5) * dtdy;
     const PhysParams pp = A.pp;
-    const bool second = A.order2 == 2;
-    const bool trans = A.transverse > 0;
-    const bool cqsplit = (A.order2 > 1) && (A.transverse == 2);
+    const bool second = MC22 || A.order2 == 2;
+    const bool trans = MC22 || A.transverse > 0;
+    const bool cqsplit = MC22 || ((A.order2 > 1) && (A.transverse == 2));
+    int mth[MW];
+#pragma unroll
+    for (int w = 0; w < MW; w++) mth[w] = MC22 ? 4 : A.mthlim[w];
 
     int col[CPL];
     bool cv[CPL];
@@ -213,13 +219,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                     for (int k = 0; k < CPL; k++) {
                         R dotl = from_left<CPL>(t, k);
-                        if (A.mthlim[w] != 0 && col[k] >= 2 && col[k] <= NX - 2) {
+                        if (mth[w] != 0 && col[k] >= 2 && col[k] <= NX - 2) {
                             R wn = R(0);
 #pragma unroll
                             for (int m = 0; m < ME; m++) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
-                            if (wn != R(0)) {
-                                R r = fx[k].s[w] > R(0) ? M<R>::div(dotl, wn) : M<R>::div(dotr[k][w], wn);
-                                R phi = limiter_phi<R>(A.mthlim[w], r);
+                            if (M<R>::lim_ok(wn)) {
+                                R r = M<R>::divl(fx[k].s[w] > R(0) ? dotl : dotr[k][w], wn);
+                                R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
                                 for (int m = 0; m < ME; m++) fx[k].w[w][m] = phi * fx[k].w[w][m];
                             }
@@ -371,14 +377,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 if (second) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
-                        if (A.mthlim[w] != 0) {
+                        if (mth[w] != 0) {
                             R wn = R(0);
 #pragma unroll
                             for (int m = 0; m < ME; m++) wn = wn + f.w[w][m] * f.w[w][m];
-                            if (wn != R(0)) {
-                                R r = f.s[w] > R(0) ? M<R>::div(dotly[k][w], wn)
-                                                    : M<R>::div(dotry[k][w], wn);
-                                R phi = limiter_phi<R>(A.mthlim[w], r);
+                            if (M<R>::lim_ok(wn)) {
+                                R r = M<R>::divl(f.s[w] > R(0) ? dotly[k][w] : dotry[k][w], wn);
+                                R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
                                 for (int m = 0; m < ME; m++) f.w[w][m] = phi * f.w[w][m];
                             }

@@ -10,6 +10,7 @@ template <class R, template <class> class SolverT, int CPL>
 static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
                       const void* aux, double* cfl, hipStream_t s)
 {
+    constexpr int MW = SolverT<R>::MW;
     StepArgs a = a0;
     const int NX = a.mx + 4;
     a.lpp = (NX + CPL - 1) / CPL;
@@ -18,9 +19,16 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
     const long long blocks = (waves + wpg - 1) / wpg;
     if (blocks == 0) return HIPCLAW_OK;
-    hipLaunchKernelGGL((k_step2<R, SolverT, CPL>), dim3((unsigned)blocks), dim3(HC_WAVE * wpg),
-                       0, s, a, static_cast<const R*>(qin), static_cast<R*>(qout),
-                       static_cast<const R*>(aux), cfl);
+    bool mc22 = a.order2 == 2 && a.transverse == 2;
+    for (int w = 0; w < MW; w++) mc22 = mc22 && a.mthlim[w] == 4;
+    if (mc22)
+        hipLaunchKernelGGL((k_step2<R, SolverT, CPL, true>), dim3((unsigned)blocks),
+                           dim3(HC_WAVE * wpg), 0, s, a, static_cast<const R*>(qin),
+                           static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
+    else
+        hipLaunchKernelGGL((k_step2<R, SolverT, CPL, false>), dim3((unsigned)blocks),
+                           dim3(HC_WAVE * wpg), 0, s, a, static_cast<const R*>(qin),
+                           static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
     return check_hip(hipGetLastError(), "k_step2 launch");
 }
 

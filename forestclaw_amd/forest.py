@@ -132,6 +132,23 @@ class Hipclaw46:
             raise ValueError("level, gx, gy must have equal lengths")
         _check(_lib().hipclaw_init_leaves(self.h, len(a[0]), *(v.ctypes.data_as(_IP) for v in a)))
 
+    def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None):
+        """Patch-step launch shape for this context (None: library default)."""
+        if cols_per_lane is None:
+            _check(_lib().hipclaw_set_launch(self.h, None))
+        else:
+            lc = H.Launch(cols_per_lane, waves_per_group)
+            _check(_lib().hipclaw_set_launch(self.h, ctypes.byref(lc)))
+
+    def autotune(self, level: int = -1, reps: int = 3):
+        """Time the six launch shapes on one level and keep the fastest.
+        Returns ((cols_per_lane, waves_per_group), {shape: ms})."""
+        best = H.Launch()
+        ms = (ctypes.c_float * 6)()
+        _check(_lib().hipclaw_autotune(self.h, level, reps, ctypes.byref(best), ms))
+        shapes = [(c, w) for c in (1, 2) for w in (1, 2, 4)]
+        return (best.cols_per_lane, best.waves_per_group), dict(zip(shapes, list(ms)))
+
     def set_q(self, q: np.ndarray):
         q = np.ascontiguousarray(q, dtype=np.float64)
         assert q.shape == (self.num_patches(),) + self.tile_shape

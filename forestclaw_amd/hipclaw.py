@@ -78,6 +78,9 @@ _SIGS.update({
     "hipclaw_ctx_destroy": ([_VP], ctypes.c_int),
     "hipclaw_init_uniform": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
+    "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
+    "hipclaw_autotune": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+                         ctypes.c_int),
     "hipclaw_num_patches": ([_VP, _IP], ctypes.c_int),
     "hipclaw_patch_info": ([_VP, _IP, _IP, _IP, _DP, _DP, _DP, _DP], ctypes.c_int),
     "hipclaw_set_q": ([_VP, _DP], ctypes.c_int),

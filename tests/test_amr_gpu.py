@@ -169,3 +169,23 @@ def test_wall_boundaries_conserve_mass(dev):
         sim.advance(s * 0.004, 0.004)
     m1 = sim.mass()
     assert abs(m1[0] - m0[0]) <= 1e-13 * abs(m0[0])
+
+
+def test_autotune_and_launch_shapes_bitwise(dev):
+    """Every launch shape gives the same FP64 bits; the autotuner keeps one of them."""
+    sim = make(H.SOLVER_SWE_ROE, 16, 2, 3, refine=0.5)
+    sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+    q0 = sim.get_q()
+    ref = None
+    for cpl in (1, 2):
+        for wpg in (1, 2, 4):
+            sim.set_q(q0)
+            sim.set_launch(cpl, wpg)
+            cfl = [sim.advance(s * 0.005, 0.005) for s in range(2)]
+            got = (cfl, interiors(sim.get_q()))
+            if ref is None:
+                ref = got
+            assert got[0] == ref[0] and np.array_equal(got[1], ref[1]), (cpl, wpg)
+    best, sweep = sim.autotune(-1, 2)
+    assert best in sweep and len(sweep) == 6 and all(v > 0 for v in sweep.values())
+    sim.set_launch(None)
