@@ -46,6 +46,16 @@ struct TiOp {           // time interpolation of one coarse patch
     int slot, level;
 };
 
+// Conservative time-sync corrections of one patch (fclaw2d_time_sync.c:118-154
+// through cb_face_fill's TIME_SYNC_SAMESIZE / TIME_SYNC_F2C dispatch,
+// src/fclaw2d_face_neighbors.c:320-390).
+enum TsKind { TS_NONE = 0, TS_SAME = 1, TS_FINE = 2 };
+struct TsOp {
+    int slot;
+    int kind[4];        // per face
+    int nbr[4][2];      // same size: [0]; finer: igrid 0, 1
+};
+
 struct XferOp {         // regrid transfer into the new pools
     int kind;           // 0 keep (copy), 1 interpolate2fine, 2 average2coarse
     int dst;            // new slot

@@ -34,5 +34,9 @@ template <class R>
 void launch_mass(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double*,
                  hipStream_t);
 template <class R> void launch_copy_tiles(const int*, int, const R*, R*, long long, hipStream_t);
+template <class R>
+void launch_ts_correct(const TsOp*, int, R*, const double*, long long, const double*, int, int, int,
+                       hipStream_t);
+void launch_ts_reset(const int*, int, double*, long long, int, int, int, hipStream_t);
 
 }  // namespace hc

@@ -177,9 +177,35 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
     return HIPCLAW_OK;
 }
 
+static int step2_batch_impl(const hipclaw_physics* phys, int npatch, const void* q_in,
+                            void* q_out, const void* aux, const double* dxdy,
+                            const int* patch_ids, int ntiles, double dt, double* cfl_patch,
+                            double* reg, const hipclaw_launch* launch, void* stream);
+
 int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                         const void* aux, const double* dxdy, const int* patch_ids, int ntiles,
                         double dt, double* cfl_patch, const hipclaw_launch* launch, void* stream)
+{
+    return step2_batch_impl(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt,
+                            cfl_patch, nullptr, launch, stream);
+}
+
+int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void* q_in,
+                             void* q_out, const void* aux, const double* dxdy,
+                             const int* patch_ids, int ntiles, double dt, double* cfl_patch,
+                             double* registers, const hipclaw_launch* launch, void* stream)
+{
+    if (!registers) return set_error(HIPCLAW_ERR_ARG, "registers is NULL");
+    if (phys && phys->solver != HIPCLAW_SOLVER_SWE_ROE)
+        return set_error(HIPCLAW_ERR_UNSUPPORTED, "time-sync registers need the SWE solver");
+    return step2_batch_impl(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt,
+                            cfl_patch, registers, launch, stream);
+}
+
+static int step2_batch_impl(const hipclaw_physics* phys, int npatch, const void* q_in,
+                            void* q_out, const void* aux, const double* dxdy,
+                            const int* patch_ids, int ntiles, double dt, double* cfl_patch,
+                            double* reg, const hipclaw_launch* launch, void* stream)
 {
     int rc = validate_physics(phys);
     if (rc) return rc;
@@ -216,6 +242,9 @@ int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_i
     const long long T = (long long)(phys->mx + 4) * (phys->my + 4);
     a.qstride = T * phys->meqn;
     a.astride = T * phys->maux;
+    a.reg = reg;
+    a.regstride = 8LL * phys->meqn * (phys->mx + phys->my);
+    if (reg && phys->mbc != 2) return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs mbc = 2");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (phys->precision == HIPCLAW_FP64)
         return launch_step2_fp64(a, phys->solver, cpl, wpg, q_in, q_out, aux, cfl_patch, s);

@@ -72,7 +72,7 @@ void Ctx::release_pools()
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
-    double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass};
+    double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass, &d_reg};
     for (auto b : dbufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -85,6 +85,7 @@ void Ctx::release_pools()
     for (auto& lp : plan) {
         lp.avg.free_(); lp.avg_ti.free_(); lp.copy.free_(); lp.copy_ti.free_();
         lp.interp.free_(); lp.interp_ti.free_(); lp.bc.free_(); lp.bc_ti.free_(); lp.ti.free_();
+        lp.ts.free_(); lp.ts_same.free_(); lp.ts_f2c.free_(); lp.ts_phys.free_();
     }
     n = cap = 0;
 }
@@ -247,7 +248,15 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
         for (auto& o : bti) o.flags = 1;
         HC_TRY(lp.bc_ti.upload(bti));
         HC_TRY(lp.ti.upload(tio[L]));
+        HC_TRY(lp.ts.upload(hp.ts[L]));
+        HC_TRY(lp.ts_same.upload(hp.ts_same[L]));
+        HC_TRY(lp.ts_f2c.upload(hp.ts_f2c[L]));
+        HC_TRY(lp.ts_phys.upload(hp.ts_phys[L]));
     }
+    // registers start at zero on a new mesh (a regrid happens after a full
+    // coarse step, when every synchronised register has been reset)
+    regstride = 8LL * phys.meqn * (phys.mx + phys.my);
+    if (tsync) HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)nn * regstride));
     return check_hip(hipDeviceSynchronize(), "rebuild");
 }
 
@@ -305,8 +314,13 @@ int Ctx::update_level(int level, double t, double dt)
         HC_TRY(check_hip(hipEventCreate(&e1), "event"));
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
-    HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
-                               d_cfl + s0, launch.cols_per_lane > 0 ? &launch : nullptr, stream));
+    const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
+    if (tsync)
+        HC_TRY(hipclaw_step2_batch_sync(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
+                                        d_cfl + s0, d_reg + (size_t)s0 * regstride, lc, stream));
+    else
+        HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
+                                   d_cfl + s0, lc, stream));
     if (prof) {
         HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
         ev.push_back({e0, e1});
@@ -378,6 +392,44 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     return HIPCLAW_OK;
 }
 
+int Ctx::set_time_sync(bool on)
+{
+    if (on && phys.solver != HIPCLAW_SOLVER_SWE_ROE)
+        return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs a conservative flux (SWE only)");
+    tsync = on;
+    if (on && !d_reg && n > 0) HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)n * regstride));
+    return HIPCLAW_OK;
+}
+
+// fclaw2d_time_sync(minlevel, maxlevel): corrections of every synchronised
+// level, then the register resets of correct_coarse_cells
+// (src/fclaw2d_time_sync.c:118-154): same-size faces of levels
+// [minlevel, maxlevel], coarse/fine faces between levels L and L+1 for L in
+// [minlevel, maxlevel-1], physical faces of [minlevel, maxlevel].
+int Ctx::time_sync(int minlevel, int maxlevel)
+{
+    if (!tsync) return HIPCLAW_OK;
+    if (minlevel < 0 || maxlevel >= MAXL || minlevel > maxlevel)
+        return set_error(HIPCLAW_ERR_ARG, "time_sync: bad level range");
+    const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
+    for (int L = minlevel; L <= maxlevel; L++) {
+        const LevelPlan& lp = plan[L];
+        if (esz == 8)
+            launch_ts_correct<double>(lp.ts.p, lp.ts.n, pools<double>().q[L], d_reg, regstride, d_dxdy,
+                                      mx, my, meqn, stream);
+        else
+            launch_ts_correct<float>(lp.ts.p, lp.ts.n, pools<float>().q[L], d_reg, regstride, d_dxdy,
+                                     mx, my, meqn, stream);
+    }
+    for (int L = minlevel; L <= maxlevel; L++) {
+        launch_ts_reset(plan[L].ts_same.p, plan[L].ts_same.n, d_reg, regstride, mx, my, meqn, stream);
+        if (L < maxlevel)
+            launch_ts_reset(plan[L].ts_f2c.p, plan[L].ts_f2c.n, d_reg, regstride, mx, my, meqn, stream);
+        launch_ts_reset(plan[L].ts_phys.p, plan[L].ts_phys.n, d_reg, regstride, mx, my, meqn, stream);
+    }
+    return check_hip(hipGetLastError(), "time sync");
+}
+
 int Ctx::timeinterp(int level, double alpha)
 {
     const LevelPlan& lp = plan[level];
@@ -442,12 +494,15 @@ int Ctx::advance(double t, double dt, double* maxcfl)
                 int tl = hi - 1;
                 while (c[hi].last_step % c[tl].step_inc == 0) tl--;
                 HC_TRY(ghost_update(tl + 1, hi, 1));
+                HC_TRY(time_sync(tl + 1, hi));
             } else {
                 HC_TRY(ghost_update(lo, hi, 0));
+                HC_TRY(time_sync(lo, hi));
             }
         }
     }
     HC_TRY(ghost_update(lo, hi, 0));
+    HC_TRY(time_sync(lo, hi));
     // maximum over the level CFLs of this coarse step
     if (maxcfl) {
         HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
@@ -630,6 +685,30 @@ int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch)
         return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
     ctx->c->launch = *launch;
     return HIPCLAW_OK;
+}
+
+int hipclaw_set_time_sync(hipclaw_ctx* ctx, int on)
+{
+    CTX_CHECK(ctx);
+    return ctx->c->set_time_sync(on != 0);
+}
+
+int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel)
+{
+    CTX_READY(ctx);
+    return ctx->c->time_sync(minlevel, maxlevel);
+}
+
+int hipclaw_get_registers(hipclaw_ctx* ctx, double* host)
+{
+    CTX_READY(ctx);
+    Ctx& c = *ctx->c;
+    if (!host) return set_error(HIPCLAW_ERR_ARG, "host is NULL");
+    if (!c.d_reg) return set_error(HIPCLAW_ERR_ARG, "time sync is off");
+    int rc = check_hip(hipStreamSynchronize(c.stream), "sync");
+    if (rc) return rc;
+    return check_hip(hipMemcpy(host, c.d_reg, sizeof(double) * (size_t)c.n * c.regstride,
+                               hipMemcpyDeviceToHost), "registers");
 }
 
 int hipclaw_autotune(hipclaw_ctx* ctx, int level, int reps, hipclaw_launch* best, float* ms)

@@ -32,6 +32,8 @@ struct LevelPlan {
     DevArray<GhostOp> interp, interp_ti; // coarse = this level
     DevArray<BcOp> bc, bc_ti;
     DevArray<TiOp> ti;                   // patches with finer neighbours
+    DevArray<TsOp> ts;                   // time-sync corrections of this level's patches
+    DevArray<int> ts_same, ts_f2c, ts_phys;  // register faces reset by a sync
     int s0 = 0, s1 = 0;                  // slot range of the level
 };
 
@@ -62,6 +64,9 @@ struct Ctx {
     double swirl_tperiod = 0.0;    // > 0: advection velocities from the swirl stream function
     int cflacc_count = 0;
     hipclaw_launch launch{0, 0};   // patch-step launch shape; 0 = library default
+    bool tsync = false;            // conservative time synchronisation on
+    double* d_reg = nullptr;       // time-sync registers, regstride doubles per slot (FP64)
+    long long regstride = 0;
     // tagging parameters
     double refine_threshold = 0.0, coarsen_threshold = 0.0;
     int tag_ivar = 0;
@@ -86,6 +91,8 @@ struct Ctx {
     int mass(double* sums);
     int level_range(int level, int* s0, int* s1) const;
     int autotune(int level, int reps, hipclaw_launch* best, float* ms);
+    int set_time_sync(bool on);
+    int time_sync(int minlevel, int maxlevel);
 };
 
 }  // namespace hc

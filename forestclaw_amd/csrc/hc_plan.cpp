@@ -21,6 +21,7 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
     for (int L = 0; L < MAXL; L++) {
         hp.avg[L].clear(); hp.copy[L].clear(); hp.interp[L].clear();
         hp.bc[L].clear(); hp.ti[L].clear();
+        hp.ts[L].clear(); hp.ts_same[L].clear(); hp.ts_f2c[L].clear(); hp.ts_phys[L].clear();
     }
     hp.has_fine.assign(nn, 0);
     for (int k = 0; k < nn; k++) {
@@ -30,9 +31,30 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
         b.slot = k;
         b.level = L;
         bool phys_any = false;
+        TsOp ts{};
+        ts.slot = k;
+        bool ts_any = false;
         for (int f = 0; f < 4; f++) {
             int nb[2];
             const int kind = forest.face_neighbors(k, f, nb);
+            ts.kind[f] = TS_NONE;
+            ts.nbr[f][0] = ts.nbr[f][1] = -1;
+            if (kind == NB_SAME) {
+                ts.kind[f] = TS_SAME;
+                ts.nbr[f][0] = nb[0];
+                hp.ts_same[L].push_back(4 * k + f);
+                ts_any = true;
+            } else if (kind == NB_FINE) {
+                ts.kind[f] = TS_FINE;
+                ts.nbr[f][0] = nb[0];
+                ts.nbr[f][1] = nb[1];
+                hp.ts_f2c[L].push_back(4 * k + f);
+                ts_any = true;
+            } else if (kind == NB_COARSE) {
+                if (L >= 1) hp.ts_f2c[L - 1].push_back(4 * k + f);
+            } else if (kind == NB_BOUNDARY) {
+                hp.ts_phys[L].push_back(4 * k + f);
+            }
             b.mthbc[f] = -1;
             if (kind == NB_BOUNDARY) {
                 b.mthbc[f] = mthbc[f];
@@ -67,6 +89,7 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
             }
         }
         if (phys_any) hp.bc[L].push_back(b);
+        if (ts_any) hp.ts[L].push_back(ts);
         if (hp.has_fine[k]) hp.ti[L].push_back({k, L});
     }
 }

@@ -13,6 +13,14 @@ struct HostPlans {
     std::vector<BcOp> bc[MAXL];
     std::vector<TiOp> ti[MAXL];
     std::vector<int> has_fine;  // per leaf: a finer face or corner neighbour
+    // time sync: corrections of level-L patches, and register faces (slot*4 +
+    // face) reset after a sync that includes level L
+    // (fclaw2d_clawpatch_time_sync_reset, fclaw2d_clawpatch_conservation.c:207-280):
+    //   ts_same: SAMESIZE faces of level-L patches
+    //   ts_f2c : HALFSIZE faces of level-L patches + DOUBLESIZE faces of level L+1
+    //   ts_phys: physical-boundary faces of level-L patches
+    std::vector<TsOp> ts[MAXL];
+    std::vector<int> ts_same[MAXL], ts_f2c[MAXL], ts_phys[MAXL];
 };
 
 // Turns the per-pair dispatch of cb_face_fill / cb_corner_fill

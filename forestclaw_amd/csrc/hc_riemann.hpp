@@ -42,6 +42,7 @@ struct PhysParams {
 // ---------------------------------------------------------------------------
 template <class R> struct Adv {
     static constexpr int ME = 1, MW = 1, MAUX = 2;
+    static constexpr bool HAS_CONS = false;
     using Face = FaceT<R, 1, 1>;
     // transverse speeds for the amdq split (cell on the low side) and the
     // apdq split (cell on the high side): {down, up}
@@ -74,8 +75,25 @@ template <class R> struct Adv {
 // ---------------------------------------------------------------------------
 template <class R> struct Swe {
     static constexpr int ME = 3, MW = 3, MAUX = 0;
+    static constexpr bool HAS_CONS = true;
     using Face = FaceT<R, 3, 3>;
     struct Trans { R u, v, a; };
+
+    // Conservative flux f(q) in direction DIR, always in FP64, for the
+    // time-sync edge registers (rpn2_cons_update of the radial-dam
+    // application, applications/clawpack/shallow/2d/radialdam/rpn2cons_update.f:24-48).
+    template <int DIR>
+    __device__ static inline void cons_flux(const double* q, double grav, double* f)
+    {
+#pragma clang fp contract(off)
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        const double hun = q[mu], hut = q[mv], h = q[0];
+        const double un = hun / h;
+        f[0] = hun;
+        f[mu] = hun * hun / h + 0.5 * grav * (h * h);
+        f[mv] = un * hut;
+    }
 
     template <int DIR>
     __device__ static inline void rpn2(const R* qa, const R* qb, R, const PhysParams& pp,
@@ -190,6 +208,7 @@ template <class R> struct Swe {
 // ---------------------------------------------------------------------------
 template <class R> struct Euler5 {
     static constexpr int ME = 5, MW = 5, MAUX = 0;
+    static constexpr bool HAS_CONS = false;
     using Face = FaceT<R, 5, 5>;
     struct Trans { R u, v, enth, a, g1a2, euv, u2v2; };
 

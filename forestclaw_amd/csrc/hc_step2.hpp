@@ -41,7 +41,33 @@ struct StepArgs {
     int ntiles;            // tiles addressable in q / aux / dxdy / cflp
     long long qstride;     // elements per patch in q (NX*NY*meqn)
     long long astride;     // elements per patch in aux (NX*NY*maux)
+    double* reg;           // time-sync registers (SYNC kernels only), FP64
+    long long regstride;   // doubles per patch in reg = 8*meqn*(mx+my)
 };
+
+// Time-sync register blocks of one face: P (fp/gp), M (fm/gm), E0/E1 (edge
+// flux at the interior / ghost cell), each meqn x len with len fastest, faces
+// 0 left, 1 right, 2 bottom, 3 top (the clawpatch registers fp[], fm[], gp[],
+// gm[], edge_fluxes[] of src/patches/clawpatch/fclaw2d_clawpatch_conservation.c:80-110).
+__device__ inline long long reg_off(int mx, int my, int meqn, int face, int block)
+{
+    long long off = 0;
+    for (int f = 0; f < face; f++) off += 4LL * meqn * (f < 2 ? my : mx);
+    return off + (long long)block * meqn * (face < 2 ? my : mx);
+}
+
+// r += s * v and r -= s * v in FP64 without contraction, as
+// clawpack46_time_sync.f:37-156 and :163-226 accumulate.
+__device__ inline void reg_add(double* r, double s, double v)
+{
+#pragma clang fp contract(off)
+    *r = *r + s * v;
+}
+__device__ inline void reg_sub(double* r, double s, double v)
+{
+#pragma clang fp contract(off)
+    *r = *r - s * v;
+}
 
 template <int CPL, class T> __device__ inline T from_left(const T (&v)[CPL], int k)
 {
@@ -57,7 +83,11 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
 // MC22: compile-time specialisation for the configuration every shipped
 // application uses (order 2, transverse 2, MC limiter on every wave family);
 // the generic instantiation reads these from the arguments.
-template <class R, template <class> class SolverT, int CPL, bool MC22>
+// SYNC: also accumulate the conservative time-sync registers (solvers with a
+// conservative flux function only): store_flux on the data before the update
+// and accumulate_waves on the final face fluctuations
+// (src/solvers/fc2d_clawpack4.6/fc2d_clawpack46.cpp:317-341, 376-390).
+template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC>
 __global__ void __launch_bounds__(256)
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
@@ -145,6 +175,35 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + J + 1) * NX + col[k]] : R(0);
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
+            }
+        }
+
+        // ---- time-sync edge fluxes of the data before the update ----------
+        if constexpr (SYNC) {
+            double* rg = A.reg + (long long)patch * A.regstride;
+            const double dtel_y = A.dt * dy, dtel_x = A.dt * dx;
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int c = col[k];
+                double qd[ME], fl[ME];
+#pragma unroll
+                for (int m = 0; m < ME; m++) qd[m] = (double)qc[k][m];
+                if (cv[k] && J >= 2 && J <= NY - 3 && (c == 1 || c == 2 || c == NX - 3 || c == NX - 2)) {
+                    const int face = c <= 2 ? 0 : 1;
+                    const int blk = (c == 2 || c == NX - 3) ? 2 : 3;
+                    S::template cons_flux<0>(qd, pp.grav, fl);
+                    double* r = rg + reg_off(A.mx, A.my, ME, face, blk) + (J - 2);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) reg_add(r + m * A.my, dtel_y, fl[m]);
+                }
+                if (cv[k] && c >= 2 && c <= NX - 3 && (J == 1 || J == 2 || J == NY - 3 || J == NY - 2)) {
+                    const int face = J <= 2 ? 2 : 3;
+                    const int blk = (J == 2 || J == NY - 3) ? 2 : 3;
+                    S::template cons_flux<1>(qd, pp.grav, fl);
+                    double* r = rg + reg_off(A.mx, A.my, ME, face, blk) + (c - 2);
+#pragma unroll
+                    for (int m = 0; m < ME; m++) reg_add(r + m * A.mx, dtel_x, fl[m]);
+                }
             }
         }
 
@@ -466,6 +525,52 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
                         qo[((long long)m * NY + Rw) * NX + col[k]] = qn;
                     }
+                    if constexpr (SYNC) {
+                        // fluctuations on the patch edges (accumulate_waves)
+                        double* rg = A.reg + (long long)patch * A.regstride;
+                        const int c = col[k];
+                        const double dtel_y = A.dt * dy, dtel_x = A.dt * dx;
+                        if (cv[k] && (c == 1 || c == NX - 3)) {
+                            const R fmr = (fmx2[k][m] + e2[k][m]) + e1r;
+                            reg_add(rg + reg_off(A.mx, A.my, ME, c == 1 ? 0 : 1, 1) + m * A.my + (Rw - 2),
+                                    dtel_y, (double)fmr);
+                        }
+                        if (cv[k] && (c == 2 || c == NX - 2)) {
+                            const R fpc = (fpx2[k][m] + e2l) + e1[k][m];
+                            reg_sub(rg + reg_off(A.mx, A.my, ME, c == 2 ? 0 : 1, 0) + m * A.my + (Rw - 2),
+                                    dtel_y, (double)fpc);
+                        }
+                        if (cv[k] && c >= 2 && c <= NX - 3 && Rw == 2) {
+                            const R gpc = G2[k][m] + fdp_prev[k][m];
+                            reg_sub(rg + reg_off(A.mx, A.my, ME, 2, 0) + m * A.mx + (c - 2), dtel_x,
+                                    (double)gpc);
+                        }
+                        if (cv[k] && c >= 2 && c <= NX - 3 && Rw == NY - 3) {
+                            const R gmu = G1[k][m] + fdm[k][m];
+                            reg_add(rg + reg_off(A.mx, A.my, ME, 3, 1) + m * A.mx + (c - 2), dtel_x,
+                                    (double)gmu);
+                        }
+                    }
+                }
+            }
+        }
+
+        // ---- time-sync fluctuations on the two ghost-row edge faces ---------
+        if constexpr (SYNC) {
+            double* rg = A.reg + (long long)patch * A.regstride;
+            const double dtel_x = A.dt * dx;
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int c = col[k];
+                if (!(cv[k] && c >= 2 && c <= NX - 3)) continue;
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    if (J == 3)  // gm on the bottom edge face (top face of ghost row 1)
+                        reg_add(rg + reg_off(A.mx, A.my, ME, 2, 1) + m * A.mx + (c - 2), dtel_x,
+                                (double)(G1[k][m] + fdm[k][m]));
+                    if (J == NY - 1)  // gp on the top edge face (bottom face of ghost row NY-2)
+                        reg_sub(rg + reg_off(A.mx, A.my, ME, 3, 0) + m * A.mx + (c - 2), dtel_x,
+                                (double)(G1[k][m] + fdp[k][m]));
                 }
             }
         }

@@ -40,6 +40,7 @@ class Options:
     refine_threshold: float = 0.5
     coarsen_threshold: float = 0.1
     subcycle: bool = True
+    time_sync: bool = False
     max_cfl: float = 1.0
     desired_cfl: float = 0.9
     use_fixed_dt: bool = False
@@ -88,6 +89,8 @@ class Hipclaw46:
         self.h = h
         _check(L.hipclaw_set_tagging(self.h, opt.threshold_variable, opt.refine_threshold,
                                      opt.coarsen_threshold))
+        if opt.time_sync:
+            _check(L.hipclaw_set_time_sync(self.h, 1))
         self.t = 0.0
         self.n_inner = 0
 
@@ -131,6 +134,20 @@ class Hipclaw46:
         if not (len(a[0]) == len(a[1]) == len(a[2])):
             raise ValueError("level, gx, gy must have equal lengths")
         _check(_lib().hipclaw_init_leaves(self.h, len(a[0]), *(v.ctypes.data_as(_IP) for v in a)))
+
+    def set_time_sync(self, on: bool = True):
+        """Conservative time synchronisation (SWE): FP64 registers per patch,
+        corrections after every synchronising ghost update of advance()."""
+        _check(_lib().hipclaw_set_time_sync(self.h, int(bool(on))))
+
+    def time_sync(self, minlevel: int, maxlevel: int):
+        _check(_lib().hipclaw_time_sync(self.h, minlevel, maxlevel))
+
+    def get_registers(self) -> np.ndarray:
+        n = self.num_patches()
+        r = np.zeros((n, 8 * self.phys.meqn * (self.opt.mx + self.opt.my)))
+        _check(_lib().hipclaw_get_registers(self.h, r.ctypes.data_as(_DP)))
+        return r
 
     def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None):
         """Patch-step launch shape for this context (None: library default)."""

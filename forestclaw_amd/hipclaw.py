@@ -61,6 +61,9 @@ _SIGS = {
     "hipclaw_step2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP, _VP,
                              ctypes.c_int, ctypes.c_double, _VP, ctypes.POINTER(Launch), _VP],
                             ctypes.c_int),
+    "hipclaw_step2_batch_sync": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP,
+                                  _VP, ctypes.c_int, ctypes.c_double, _VP, _VP,
+                                  ctypes.POINTER(Launch), _VP], ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
     "hipclaw_swirl_aux_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP, _VP],
@@ -79,6 +82,9 @@ _SIGS.update({
     "hipclaw_init_uniform": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
     "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
+    "hipclaw_set_time_sync": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_time_sync": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "hipclaw_get_registers": ([_VP, _DP], ctypes.c_int),
     "hipclaw_autotune": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
                          ctypes.c_int),
     "hipclaw_num_patches": ([_VP, _IP], ctypes.c_int),
@@ -184,6 +190,17 @@ def step2_batch(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, c
         ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy),
         _ptr(patch_ids), int(ntiles), float(dt), _ptr(cfl), ctypes.byref(ln) if ln is not None else None,
         _stream(stream)))
+
+
+def step2_batch_sync(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, cfl,
+                     registers, launch: tuple[int, int] | None = None, stream=None) -> None:
+    """step2_batch that also accumulates the FP64 time-sync registers
+    (8*meqn*(mx+my) doubles per tile)."""
+    ln = Launch(*launch) if launch is not None else None
+    _check(lib().hipclaw_step2_batch_sync(
+        ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy), None,
+        int(npatch), float(dt), _ptr(cfl), _ptr(registers),
+        ctypes.byref(ln) if ln is not None else None, _stream(stream)))
 
 
 def max_reduce(x, n: int, out, stream=None) -> None:

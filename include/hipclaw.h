@@ -100,6 +100,22 @@ int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_i
                         const int* patch_ids, int ntiles, double dt, double* cfl_patch,
                         const hipclaw_launch* launch, void* stream);
 
+/* The same update that also accumulates the conservative time-sync
+ * registers of every patch (SWE only), as clawpack46_step2 does when
+ * fclaw_opt->time_sync is set: clawpack46_time_sync_store_flux on the data
+ * before the update and clawpack46_time_sync_accumulate_waves on the final
+ * edge fluctuations (reference src/solvers/fc2d_clawpack4.6/fc2d_clawpack46.cpp:317-341,
+ * 376-390; fortran_source/clawpack46_time_sync.f:37-226).  registers: device
+ * FP64, 8*meqn*(mx+my) doubles per tile addressed like q; per face f = 0 left,
+ * 1 right, 2 bottom, 3 top (length len = my or mx) four blocks of meqn*len
+ * (len fastest): fp|gp, fm|gm, edge flux at the interior cell, edge flux at
+ * the ghost cell (the fclaw2d_clawpatch_registers arrays,
+ * src/patches/clawpatch/fclaw2d_clawpatch_conservation.h:56-77). */
+int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void* q_in,
+                             void* q_out, const void* aux, const double* dxdy,
+                             const int* patch_ids, int ntiles, double dt, double* cfl_patch,
+                             double* registers, const hipclaw_launch* launch, void* stream);
+
 /* Maximum of n device doubles into *out_dev (device), the level CFL of
  * fclaw2d_update_single_step (src/fclaw2d_update_single_step.c:68). */
 int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream);
@@ -160,6 +176,16 @@ int hipclaw_init_uniform(hipclaw_ctx* ctx, int level);
 int hipclaw_init_leaves(hipclaw_ctx* ctx, int nleaves, const int* level, const int* gx,
                         const int* gy);
 int hipclaw_num_patches(hipclaw_ctx* ctx, int* n);
+/* Conservative time synchronisation (fclaw_opt->time_sync; SWE only): the
+ * context keeps FP64 registers per patch, accumulated by every level update,
+ * and hipclaw_advance applies fclaw2d_time_sync after each synchronising ghost
+ * update (reference src/fclaw2d_advance.c:280-341). */
+int hipclaw_set_time_sync(hipclaw_ctx* ctx, int on);
+/* fclaw2d_time_sync(glob, minlevel, maxlevel) (src/fclaw2d_time_sync.c:157-202):
+ * same-size and fine-to-coarse corrections, then the register resets. */
+int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel);
+/* Copy of all registers (num_patches * 8*meqn*(mx+my) doubles, slot order). */
+int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
 /* Patch-step launch shape used by this context (NULL = library default). */
 int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch);
 /* Autotuner: times every launch shape (cols_per_lane 1..2 x waves_per_group

@@ -11,6 +11,9 @@ Follows, loop for loop:
                                src/fclaw2d_corner_neighbors.c:334-535
   advance                      src/fclaw2d_advance.c:54-357
   timeinterp                   src/fclaw2d_timeinterp.c:30-57
+  time_sync                    src/fclaw2d_time_sync.c:118-202 with the reset
+                               rules of fclaw2d_clawpatch_conservation.c:207-280
+                               and the dispatch of fclaw2d_face_neighbors.c:320-390
 Neighbour relations are derived here independently from the leaf
 coordinates (level, gx, gy) on the brick's global index grid.
 """
@@ -44,6 +47,13 @@ def _setup_sigs():
         fn.argtypes = i4 + [_dp, _dp, ctypes.c_int]
         fn.restype = None
     L.orc_timeinterp.argtypes = i4 + [_dp, _dp, _dp, ctypes.c_double]
+    L.orc_time_sync_f2c.argtypes = i4 + [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp,
+                                         _dp]
+    L.orc_time_sync_f2c.restype = None
+    L.orc_time_sync_samesize.argtypes = i4 + [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+    L.orc_time_sync_samesize.restype = None
+    L.orc_time_sync_reset_face.argtypes = [ctypes.c_int] * 4 + [_dp]
+    L.orc_time_sync_reset_face.restype = None
     L.orc_tag_gradient.argtypes = [ctypes.c_int] * 3 + [_dp, ctypes.c_int, ctypes.c_double,
                                                          ctypes.c_double, ctypes.c_double]
     L._amr_sigs = True
@@ -52,7 +62,7 @@ def _setup_sigs():
 
 class AmrOracle:
     def __init__(self, cfg: O.OrcCfg, leaves, geom, mthbc, brick=(1, 1), subcycle=True,
-                 swirl_tperiod=0.0):
+                 swirl_tperiod=0.0, time_sync=False):
         """leaves: list of (level, gx, gy); geom: array (n, 4) of xlower, ylower, dx, dy."""
         self.L = _setup_sigs()
         self.cfg = cfg
@@ -71,6 +81,8 @@ class AmrOracle:
         self.qti = [np.zeros(shape) for _ in range(n)]
         self.levels = sorted(set(l[0] for l in self.leaves))
         self.cell_updates = 0
+        self.time_sync = time_sync
+        self.reg = [np.zeros(O.reg_size(self.mx, self.my, self.meqn)) for _ in range(n)]
 
     # ---------------- topology ------------------------------------------
     def inside(self, L, gx, gy):
@@ -221,6 +233,13 @@ class AmrOracle:
             return 0.0
         for k in ks:
             self.qlast[k][...] = self.q[k]
+        if self.time_sync:
+            cfl = 0.0
+            for k in ks:
+                cfl = max(cfl, O.step2_ts(self.cfg, self.q[k], self.geom[k, 2], self.geom[k, 3],
+                                          dt, self.reg[k]))
+            self.cell_updates += len(ks) * self.mx * self.my
+            return cfl
         qb = np.stack([self.q[k] for k in ks])
         aux = None
         if self.cfg.solver == O.SOLVER_ADV:
@@ -231,6 +250,42 @@ class AmrOracle:
             self.q[k][...] = qb[i]
         self.cell_updates += len(ks) * self.mx * self.my
         return float(cfl.max())
+
+    # ---------------- conservative time synchronisation ----------------
+    def _reset(self, k, kinds):
+        for f in range(4):
+            if self.face(k, f)[0] in kinds:
+                self.L.orc_time_sync_reset_face(self.mx, self.my, self.meqn, f, _p(self.reg[k]))
+
+    def sync(self, minlevel, maxlevel):
+        """fclaw2d_time_sync(minlevel, maxlevel) -> correct_coarse_cells."""
+        a = self._args()
+        for L in range(maxlevel, minlevel - 1, -1):
+            for k in self.patches_at(L):
+                area = self.geom[k, 2] * self.geom[k, 3]
+                for f in range(4):
+                    kind, nb = self.face(k, f)
+                    if kind == "same":
+                        self.L.orc_time_sync_samesize(*a, f, area, _p(self.q[k]),
+                                                      _p(self.reg[k]), _p(self.reg[nb]))
+            for k in self.patches_at(L):
+                self._reset(k, ("same",))
+            if L < maxlevel:
+                for k in self.patches_at(L):
+                    area = self.geom[k, 2] * self.geom[k, 3]
+                    for f in range(4):
+                        kind, nb = self.face(k, f)
+                        if kind == "fine":
+                            for g in (0, 1):
+                                self.L.orc_time_sync_f2c(*a, f, g, area, _p(self.q[k]),
+                                                         _p(self.reg[k]), _p(self.reg[nb[g]]))
+                for k in self.patches_at(L):
+                    self._reset(k, ("fine",))
+                for k in self.patches_at(L + 1):
+                    self._reset(k, ("coarse",))
+        for L in range(maxlevel, minlevel - 1, -1):
+            for k in self.patches_at(L):
+                self._reset(k, ("bdry",))
 
     def timeinterp(self, L, alpha):
         for k in self.patches_at(L):
@@ -269,7 +324,13 @@ class AmrOracle:
                     while c[hi]["last"] % c[tl]["inc"] == 0:
                         tl -= 1
                     self.ghost_update(tl + 1, hi, 1)
+                    if self.time_sync:
+                        self.sync(tl + 1, hi)
                 else:
                     self.ghost_update(lo, hi, 0)
+                    if self.time_sync:
+                        self.sync(lo, hi)
         self.ghost_update(lo, hi, 0)
+        if self.time_sync:
+            self.sync(lo, hi)
         return maxcfl

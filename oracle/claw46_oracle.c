@@ -585,8 +585,119 @@ static int check_cfg(const orc_cfg* c)
     return 0;
 }
 
+/* Conservative flux f(q) of the shallow-water equations in direction idir
+ * (0 = x, 1 = y), rpn2_cons_update of the radial-dam application
+ * (applications/clawpack/shallow/2d/radialdam/rpn2cons_update.f:7-51). */
+void orc_swe_flux(int idir, double grav, const double* q, double* flux)
+{
+    const int mu = idir == 0 ? 1 : 2, mv = idir == 0 ? 2 : 1;
+    const double hun = q[mu], hut = q[mv], h = q[0];
+    const double un = hun / h;
+    flux[0] = hun;
+    flux[mu] = hun * hun / h + 0.5 * grav * (h * h);
+    flux[mv] = un * hut;
+}
+
+long orc_reg_offset(int mx, int my, int meqn, int face, int block)
+{
+    long off = 0;
+    for (int f = 0; f < face; f++) off += 4L * meqn * (f < 2 ? my : mx);
+    return off + (long)block * meqn * (face < 2 ? my : mx);
+}
+
+long orc_reg_size(int mx, int my, int meqn) { return 8L * meqn * (mx + my); }
+
+/* Edge-flux registers before the update
+ * (clawpack46_time_sync_store_flux, clawpack46_time_sync.f:37-156):
+ * E_k += dt*el*f(q) at the first interior (k = 0) and first ghost (k = 1)
+ * cell of every edge cell; el = dy on x faces, dx on y faces (Cartesian
+ * clawpatch_time_sync_setup, fclaw2d_clawpatch_utils.f:296-312). */
+static void store_flux(const orc_cfg* c, const double* q, double dx, double dy, double dt,
+                       double* reg)
+{
+    const int mx = c->mx, my = c->my, mbc = c->mbc, meqn = c->meqn;
+    const int NX = mx + 2 * mbc, NY = my + 2 * mbc;
+    double qv[8], fl[8];
+#define QV(i, j) for (int m = 0; m < meqn; m++) qv[m] = q[((size_t)m * NY + (j)) * NX + (i)]
+    for (int j = 0; j < my; j++)
+        for (int k = 0; k < 2; k++) {
+            double* e0 = reg + orc_reg_offset(mx, my, meqn, 0, 2 + k);
+            double* e1 = reg + orc_reg_offset(mx, my, meqn, 1, 2 + k);
+            QV(mbc - k, mbc + j);
+            orc_swe_flux(0, c->grav, qv, fl);
+            for (int m = 0; m < meqn; m++) e0[m * my + j] = e0[m * my + j] + dt * dy * fl[m];
+            QV(mbc + mx - 1 + k, mbc + j);
+            orc_swe_flux(0, c->grav, qv, fl);
+            for (int m = 0; m < meqn; m++) e1[m * my + j] = e1[m * my + j] + dt * dy * fl[m];
+        }
+    for (int i = 0; i < mx; i++)
+        for (int k = 0; k < 2; k++) {
+            double* e2 = reg + orc_reg_offset(mx, my, meqn, 2, 2 + k);
+            double* e3 = reg + orc_reg_offset(mx, my, meqn, 3, 2 + k);
+            QV(mbc + i, mbc - k);
+            orc_swe_flux(1, c->grav, qv, fl);
+            for (int m = 0; m < meqn; m++) e2[m * mx + i] = e2[m * mx + i] + dt * dx * fl[m];
+            QV(mbc + i, mbc + my - 1 + k);
+            orc_swe_flux(1, c->grav, qv, fl);
+            for (int m = 0; m < meqn; m++) e3[m * mx + i] = e3[m * mx + i] + dt * dx * fl[m];
+        }
+#undef QV
+}
+
+/* Fluctuation registers after the update
+ * (clawpack46_time_sync_accumulate_waves, clawpack46_time_sync.f:163-226). */
+static void accumulate_waves(const orc_cfg* c, const double* fp, const double* fm,
+                             const double* gp, const double* gm, double dx, double dy,
+                             double dt, double* reg)
+{
+    const int mx = c->mx, my = c->my, mbc = c->mbc, meqn = c->meqn;
+    const int NX = mx + 2 * mbc, NY = my + 2 * mbc;
+#define F2(arr, i, j, m) arr[((size_t)(m) * NY + (j)) * NX + (i)]
+    double* P0 = reg + orc_reg_offset(mx, my, meqn, 0, 0);
+    double* M0 = reg + orc_reg_offset(mx, my, meqn, 0, 1);
+    double* P1 = reg + orc_reg_offset(mx, my, meqn, 1, 0);
+    double* M1 = reg + orc_reg_offset(mx, my, meqn, 1, 1);
+    for (int j = 0; j < my; j++)
+        for (int m = 0; m < meqn; m++) {
+            const int r = m * my + j;
+            P0[r] = P0[r] - dt * dy * F2(fp, mbc, mbc + j, m);
+            M0[r] = M0[r] + dt * dy * F2(fm, mbc, mbc + j, m);
+            P1[r] = P1[r] - dt * dy * F2(fp, mbc + mx, mbc + j, m);
+            M1[r] = M1[r] + dt * dy * F2(fm, mbc + mx, mbc + j, m);
+        }
+    double* P2 = reg + orc_reg_offset(mx, my, meqn, 2, 0);
+    double* M2 = reg + orc_reg_offset(mx, my, meqn, 2, 1);
+    double* P3 = reg + orc_reg_offset(mx, my, meqn, 3, 0);
+    double* M3 = reg + orc_reg_offset(mx, my, meqn, 3, 1);
+    for (int i = 0; i < mx; i++)
+        for (int m = 0; m < meqn; m++) {
+            const int r = m * mx + i;
+            P2[r] = P2[r] - dt * dx * F2(gp, mbc + i, mbc, m);
+            M2[r] = M2[r] + dt * dx * F2(gm, mbc + i, mbc, m);
+            P3[r] = P3[r] - dt * dx * F2(gp, mbc + i, mbc + my, m);
+            M3[r] = M3[r] + dt * dx * F2(gm, mbc + i, mbc + my, m);
+        }
+#undef F2
+}
+
+static int step2_impl(const orc_cfg* c, double* q, const double* aux, double dx, double dy,
+                      double dt, double* cflout, double* reg);
+
 int orc_step2(const orc_cfg* c, double* q, const double* aux, double dx, double dy,
               double dt, double* cflout)
+{
+    return step2_impl(c, q, aux, dx, dy, dt, cflout, NULL);
+}
+
+int orc_step2_ts(const orc_cfg* c, double* q, const double* aux, double dx, double dy,
+                 double dt, double* cflout, double* reg)
+{
+    if (c->solver != ORC_SOLVER_SWE_ROE) return 1;  /* only SWE has a conservative flux */
+    return step2_impl(c, q, aux, dx, dy, dt, cflout, reg);
+}
+
+static int step2_impl(const orc_cfg* c, double* q, const double* aux, double dx, double dy,
+                      double dt, double* cflout, double* reg)
 {
     if (check_cfg(c)) return 1;
     const int mx = c->mx, my = c->my, mbc = c->mbc, meqn = c->meqn;
@@ -677,6 +788,11 @@ int orc_step2(const orc_cfg* c, double* q, const double* aux, double dx, double 
                 F2(fm, i + 1, j, m) = F2(fm, i + 1, j, m) + GM(1, m, j);
                 F2(fp, i + 1, j, m) = F2(fp, i + 1, j, m) + GP(1, m, j);
             }
+    }
+
+    if (reg) {
+        store_flux(c, q, dx, dy, dt, reg);  /* q still holds the data before the update */
+        accumulate_waves(c, fp, fm, gp, gm, dx, dy, dt, reg);
     }
 
     /* flux-difference update over [1-mbc, mx+mbc-1] x [1-mbc, my+mbc-1] */

@@ -60,6 +60,20 @@ int orc_bc2(int mx, int my, int mbc, int meqn, const int mthbc[4], double* q);
 /* Limiter function phi(r) of clawpack46_inlinelimiter for one family. */
 double orc_limiter_phi(int mthlim, double r);
 
+/* Conservative time synchronisation (SWE only; claw46_oracle.c,
+ * claw46_timesync.c).  Registers: orc_reg_size doubles per patch. */
+void orc_swe_flux(int idir, double grav, const double* q, double* flux);
+long orc_reg_offset(int mx, int my, int meqn, int face, int block);
+long orc_reg_size(int mx, int my, int meqn);
+/* orc_step2 plus store_flux (before) and accumulate_waves (after) into reg. */
+int orc_step2_ts(const orc_cfg* cfg, double* q, const double* aux, double dx, double dy,
+                 double dt, double* cfl, double* reg);
+void orc_time_sync_f2c(int mx, int my, int mbc, int meqn, int iface, int igrid, double area,
+                       double* qc, const double* rc, const double* rf);
+void orc_time_sync_samesize(int mx, int my, int mbc, int meqn, int iface, double area,
+                            double* q, const double* rt, const double* rn);
+void orc_time_sync_reset_face(int mx, int my, int meqn, int iface, double* reg);
+
 /* Ghost-cell and regrid kernels for one patch pair (claw46_ghost.c).
  * (sx, sy): same-size index shift; (ox, oy): half-size offset
  * (xlower_coarse - xlower_fine) * 2^level_fine * mx (resp. my). */

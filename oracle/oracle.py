@@ -61,8 +61,11 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
+_DPT = ctypes.POINTER(ctypes.c_double)
+
+
 def _dp(a: np.ndarray):
-    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return a.ctypes.data_as(_DPT)
 
 
 def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = None,
@@ -110,6 +113,37 @@ def step2_batch(cfg: OrcCfg, q: np.ndarray, aux: np.ndarray | None, dx, dy, dt: 
     if rc != 0:
         raise ValueError("oracle rejected the configuration")
     return cfl
+
+
+def reg_size(mx, my, meqn) -> int:
+    """Doubles of time-sync registers per patch (4 faces x 4 blocks x meqn x length)."""
+    return 8 * meqn * (mx + my)
+
+
+def swe_flux(idir: int, grav: float, qv) -> np.ndarray:
+    L = lib()
+    L.orc_swe_flux.argtypes = [ctypes.c_int, ctypes.c_double, _DPT, _DPT]
+    L.orc_swe_flux.restype = None
+    qv = np.ascontiguousarray(qv, dtype=np.float64)
+    out = np.zeros(qv.shape[0])
+    L.orc_swe_flux(idir, grav, _dp(qv), _dp(out))
+    return out
+
+
+def step2_ts(cfg: OrcCfg, q: np.ndarray, dx: float, dy: float, dt: float, reg: np.ndarray):
+    """One SWE patch step in place that also accumulates the time-sync
+    registers (store_flux before, accumulate_waves after); returns the CFL."""
+    L = lib()
+    L.orc_step2_ts.argtypes = [ctypes.c_void_p, _DPT, _DPT, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _DPT, _DPT]
+    L.orc_step2_ts.restype = ctypes.c_int
+    assert q.dtype == np.float64 and q.flags.c_contiguous
+    assert reg.dtype == np.float64 and reg.flags.c_contiguous
+    cfl = np.zeros(1)
+    rc = L.orc_step2_ts(ctypes.byref(cfg), _dp(q), None, dx, dy, dt, _dp(cfl), _dp(reg))
+    if rc != 0:
+        raise ValueError("oracle rejected the configuration (time sync needs SWE)")
+    return float(cfl[0])
 
 
 def swirl_aux(mx, my, mbc, xlower, ylower, dx, dy, t, dt, tperiod):
