@@ -781,6 +781,70 @@ static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
     return HIPCLAW_OK;
 }
 
+// Per-patch host arrays of one level <-> the level's slots of the current pool.
+static int level_copy(Ctx& c, int level, int n, double* const* q, const double* const* aux, bool up)
+{
+    if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    const int s0 = c.plan[level].s0, np = c.plan[level].s1 - s0;
+    if (n != np) return set_error(HIPCLAW_ERR_ARG, "n differs from the number of patches of the level");
+    if (np == 0) return HIPCLAW_OK;
+    if (!q) return set_error(HIPCLAW_ERR_ARG, "q is NULL");
+    for (int k = 0; k < n; k++)
+        if (!q[k]) return set_error(HIPCLAW_ERR_ARG, "a patch pointer is NULL");
+    auto move = [&](double* const* ptrs, long long per, void* dev_base) -> int {
+        char* dev = static_cast<char*>(dev_base) + (size_t)s0 * per * c.esz;
+        const size_t cnt = (size_t)np * per;
+        if (c.esz == 8) {
+            std::vector<double> h(cnt);
+            if (up) {
+                for (int k = 0; k < n; k++) std::memcpy(&h[(size_t)k * per], ptrs[k], sizeof(double) * per);
+                return check_hip(hipMemcpy(dev, h.data(), cnt * 8, hipMemcpyHostToDevice), "level upload");
+            }
+            int rc = check_hip(hipMemcpy(h.data(), dev, cnt * 8, hipMemcpyDeviceToHost), "level download");
+            if (rc) return rc;
+            for (int k = 0; k < n; k++) std::memcpy(ptrs[k], &h[(size_t)k * per], sizeof(double) * per);
+            return HIPCLAW_OK;
+        }
+        std::vector<float> h(cnt);
+        if (up) {
+            for (int k = 0; k < n; k++)
+                for (long long i = 0; i < per; i++) h[(size_t)k * per + i] = (float)ptrs[k][i];
+            return check_hip(hipMemcpy(dev, h.data(), cnt * 4, hipMemcpyHostToDevice), "level upload");
+        }
+        int rc = check_hip(hipMemcpy(h.data(), dev, cnt * 4, hipMemcpyDeviceToHost), "level download");
+        if (rc) return rc;
+        for (int k = 0; k < n; k++)
+            for (long long i = 0; i < per; i++) ptrs[k][i] = (double)h[(size_t)k * per + i];
+        return HIPCLAW_OK;
+    };
+    if (!up) {
+        int rc = check_hip(hipStreamSynchronize(c.stream), "sync");
+        if (rc) return rc;
+    }
+    int rc = move(q, c.tile, c.pool[c.parity[level]]);
+    if (rc) return rc;
+    if (up && aux && c.phys.maux > 0) {
+        for (int k = 0; k < n; k++)
+            if (!aux[k]) return set_error(HIPCLAW_ERR_ARG, "an aux pointer is NULL");
+        c.swirl_tperiod = 0;
+        rc = move(const_cast<double* const*>(aux), c.atile, c.aux);
+    }
+    return rc;
+}
+
+int hipclaw_level_upload(hipclaw_ctx* ctx, int level, int n, const double* const* q,
+                         const double* const* aux)
+{
+    CTX_READY(ctx);
+    return level_copy(*ctx->c, level, n, const_cast<double* const*>(q), aux, true);
+}
+
+int hipclaw_level_download(hipclaw_ctx* ctx, int level, int n, double* const* q)
+{
+    CTX_READY(ctx);
+    return level_copy(*ctx->c, level, n, q, nullptr, false);
+}
+
 int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q)
 {
     CTX_READY(ctx);

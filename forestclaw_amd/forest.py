@@ -166,6 +166,20 @@ class Hipclaw46:
         shapes = [(c, w) for c in (1, 2) for w in (1, 2, 4)]
         return (best.cols_per_lane, best.waves_per_group), dict(zip(shapes, list(ms)))
 
+    def level_upload(self, level: int, tiles, aux_tiles=None):
+        """tiles: one C-contiguous float64 array per patch of the level, slot order."""
+        n = len(tiles)
+        arr = (_DP * n)(*[t.ctypes.data_as(_DP) for t in tiles])
+        aarr = None
+        if aux_tiles is not None:
+            aarr = (_DP * n)(*[t.ctypes.data_as(_DP) for t in aux_tiles])
+        _check(_lib().hipclaw_level_upload(self.h, level, n, arr, aarr))
+
+    def level_download(self, level: int, tiles):
+        n = len(tiles)
+        arr = (_DP * n)(*[t.ctypes.data_as(_DP) for t in tiles])
+        _check(_lib().hipclaw_level_download(self.h, level, n, arr))
+
     def set_q(self, q: np.ndarray):
         q = np.ascontiguousarray(q, dtype=np.float64)
         assert q.shape == (self.num_patches(),) + self.tile_shape

@@ -53,6 +53,8 @@ class HipclawError(RuntimeError):
 _lib = None
 
 _VP = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
 _SIGS = {
     "hipclaw_abi_version": ([], ctypes.c_int),
     "hipclaw_last_error": ([], ctypes.c_char_p),
@@ -65,6 +67,16 @@ _SIGS = {
                                   _VP, ctypes.c_int, ctypes.c_double, _VP, _VP,
                                   ctypes.POINTER(Launch), _VP], ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_batch_create": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, ctypes.POINTER(_VP)],
+                             ctypes.c_int),
+    "hipclaw_batch_destroy": ([_VP], ctypes.c_int),
+    "hipclaw_single_step_update": ([_VP, _DP, _DP, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    _DP], ctypes.c_int),
+    "hipclaw_level_upload": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_DP),
+                              ctypes.POINTER(_DP)], ctypes.c_int),
+    "hipclaw_level_download": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_DP)],
+                               ctypes.c_int),
     "hipclaw_swirl_aux_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, _VP, _VP],
                                 ctypes.c_int),
@@ -73,8 +85,6 @@ _SIGS = {
 }
 
 
-_DP = ctypes.POINTER(ctypes.c_double)
-_IP = ctypes.POINTER(ctypes.c_int)
 _SIGS.update({
     "hipclaw_ctx_create": ([ctypes.POINTER(Physics), ctypes.POINTER(Domain), _VP,
                             ctypes.POINTER(_VP)], ctypes.c_int),
@@ -201,6 +211,36 @@ def step2_batch_sync(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: flo
         ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy), None,
         int(npatch), float(dt), _ptr(cfl), _ptr(registers),
         ctypes.byref(ln) if ln is not None else None, _stream(stream)))
+
+
+class Batch:
+    """Deferred single-step update for host-resident patches
+    (hipclaw_batch_create / hipclaw_single_step_update)."""
+
+    def __init__(self, phys: Physics, capacity: int, stream=None):
+        h = _VP()
+        _check(lib().hipclaw_batch_create(ctypes.byref(phys), capacity, _stream(stream),
+                                          ctypes.byref(h)))
+        self.h = h
+
+    def single_step_update(self, q, aux, dx, dy, t, dt, it, total) -> float:
+        """q, aux: C-contiguous float64 numpy tiles, updated in place on a flush."""
+        c = ctypes.c_double()
+        _check(lib().hipclaw_single_step_update(
+            self.h, q.ctypes.data_as(_DP), None if aux is None else aux.ctypes.data_as(_DP),
+            dx, dy, t, dt, it, total, ctypes.byref(c)))
+        return c.value
+
+    def close(self):
+        if self.h:
+            lib().hipclaw_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def max_reduce(x, n: int, out, stream=None) -> None:

@@ -116,6 +116,29 @@ int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void
                              const int* patch_ids, int ntiles, double dt, double* cfl_patch,
                              double* registers, const hipclaw_launch* launch, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Host-resident patches: deferred single-step update.
+ *
+ * For a host that keeps its clawpatch arrays in host memory (an unmodified
+ * ForestClaw): register a patch_vt->single_step_update callback that calls
+ * hipclaw_single_step_update with the patch's q / aux arrays and the
+ * fclaw2d_single_step_buffer_data_t counters it receives (reference
+ * src/fclaw2d_patch.h:937-943, src/fclaw2d_update_single_step.c:45-103).
+ * Calls stage patches into pinned memory; the call that fills the batch
+ * (capacity patches) or carries iter == total_count-1 runs one batched launch
+ * and copies the updated interior cells back (ghost cells left as given).
+ * *maxcfl is the batch Courant number on a flushing call, 0 otherwise.
+ * All calls of one level must share t and dt.  No time-sync registers on
+ * this path (use the device-resident context for that).
+ * ------------------------------------------------------------------------ */
+typedef struct hipclaw_batch hipclaw_batch;
+int hipclaw_batch_create(const hipclaw_physics* phys, int capacity, void* stream,
+                         hipclaw_batch** out);
+int hipclaw_batch_destroy(hipclaw_batch* b);
+int hipclaw_single_step_update(hipclaw_batch* b, double* q, const double* aux, double dx,
+                               double dy, double t, double dt, int iter, int total_count,
+                               double* maxcfl);
+
 /* Maximum of n device doubles into *out_dev (device), the level CFL of
  * fclaw2d_update_single_step (src/fclaw2d_update_single_step.c:68). */
 int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream);
@@ -198,6 +221,14 @@ int hipclaw_autotune(hipclaw_ctx* ctx, int level, int reps, hipclaw_launch* best
 /* Per-patch host arrays of length num_patches (any pointer may be NULL). */
 int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* xlower,
                        double* ylower, double* dx, double* dy);
+/* Per-patch host arrays of one level (n = patches of the level, in slot
+ * order; each array T*meqn doubles, aux T*maux or aux = NULL) to / from the
+ * context: the clawpatch soln / aux arrays (fclaw2d_clawpatch_soln_data,
+ * src/patches/clawpatch/fclaw2d_clawpatch.cpp:166-189).  Uploading aux
+ * switches off the swirl velocity field. */
+int hipclaw_level_upload(hipclaw_ctx* ctx, int level, int n, const double* const* q,
+                         const double* const* aux);
+int hipclaw_level_download(hipclaw_ctx* ctx, int level, int n, double* const* q);
 /* Whole solution in leaf order, FP64 on the host, T*meqn per patch. */
 int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q);
 int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q);
