@@ -35,6 +35,7 @@ SOURCES = {
     "hc_amr_kernels.hip": STRICT,
     "hc_timesync.hip": STRICT,
     "hc_batch.hip": STRICT,
+    "hc_cnn.hip": STRICT,
     "hc_ctx.hip": STRICT,
     "hc_forest.cpp": STRICT,
     "hc_plan.cpp": STRICT,

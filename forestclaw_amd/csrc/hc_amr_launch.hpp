@@ -38,5 +38,9 @@ template <class R>
 void launch_ts_correct(const TsOp*, int, R*, const double*, long long, const double*, int, int, int,
                        hipStream_t);
 void launch_ts_reset(const int*, int, double*, long long, int, int, int, hipStream_t);
+// Network input tiles of n patches (bf16 [n][36][36][8]); returns HIPCLAW_*.
+template <class R>
+int launch_make_tiles(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double,
+                      void*, hipStream_t);
 
 }  // namespace hc

@@ -49,6 +49,8 @@ Ctx::~Ctx()
 {
     prof_clear();
     release_pools();
+    if (d_tiles) (void)hipFree(d_tiles);
+    if (d_logits) (void)hipFree(d_logits);
 }
 
 void Ctx::prof_clear()
@@ -526,6 +528,25 @@ int Ctx::qinit(int kind, const QinitParams& prm)
 
 // Tag every leaf (refine) and every complete family (coarsen), adapt the
 // forest, repopulate the new leaves and refill ghost cells.
+int Ctx::make_tiles(void* tiles, int* tags)
+{
+    if (esz == 8)
+        HC_TRY(launch_make_tiles<double>(d_iota, d_levels, n, pools<double>(), d_dxdy, phys.mx, phys.my,
+                                         phys.meqn, refine_threshold, tiles, stream));
+    else
+        HC_TRY(launch_make_tiles<float>(d_iota, d_levels, n, pools<float>(), d_dxdy, phys.mx, phys.my,
+                                        phys.meqn, refine_threshold, tiles, stream));
+    if (tags) {
+        if (esz == 8)
+            launch_tag<double>(d_iota, d_levels, n, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                               tag_ivar, refine_threshold, 0, tags, stream);
+        else
+            launch_tag<float>(d_iota, d_levels, n, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                              tag_ivar, refine_threshold, 0, tags, stream);
+    }
+    return check_hip(hipGetLastError(), "make tiles");
+}
+
 int Ctx::tag_and_adapt(int init_flag, int* changed)
 {
     const int nn = n;
@@ -537,6 +558,42 @@ int Ctx::tag_and_adapt(int init_flag, int* changed)
                           tag_ivar, refine_threshold, 0, d_tags, stream);
     std::vector<int> rtag(nn), ctag(nn);
     HC_TRY(check_hip(hipMemcpy(rtag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
+    if (cnn) {
+        // learned refinement tags; the plain tag is the fallback and the judge
+        if (tiles_cap < nn) {
+            if (d_tiles) (void)hipFree(d_tiles);
+            if (d_logits) (void)hipFree(d_logits);
+            d_tiles = nullptr;
+            d_logits = nullptr;
+            tiles_cap = 0;
+            HC_TRY(check_hip(hipMalloc(&d_tiles, (size_t)nn * 36 * 36 * 8 * 2), "tiles"));
+            HC_TRY(check_hip(hipMalloc((void**)&d_logits, sizeof(float) * 2 * (size_t)nn), "logits"));
+            tiles_cap = nn;
+        }
+        HC_TRY(make_tiles(d_tiles, nullptr));
+        HC_TRY(hipclaw_cnn_forward(cnn, d_tiles, nullptr, nn, d_logits));
+        std::vector<float> lg(2 * (size_t)nn);
+        HC_TRY(check_hip(hipStreamSynchronize(stream), "cnn"));
+        HC_TRY(check_hip(hipMemcpy(lg.data(), d_logits, sizeof(float) * lg.size(), hipMemcpyDeviceToHost),
+                         "logits"));
+        int st[6] = {nn, 0, 0, 0, 0, 0};
+        for (int k = 0; k < nn; k++) {
+            const double d = (double)lg[2 * k + 1] - (double)lg[2 * k];
+            const double pr = 1.0 / (1.0 + std::exp(-d));
+            int t = pr >= 0.5 ? 1 : 0;
+            st[2] += rtag[k] != 0;
+            if (std::fabs(pr - 0.5) < cnn_margin) {
+                t = rtag[k] != 0;
+                st[5]++;
+            } else {
+                st[3] += (rtag[k] != 0 && !t);
+                st[4] += (rtag[k] == 0 && t);
+            }
+            st[1] += t;
+            rtag[k] = t;
+        }
+        std::memcpy(tag_stats, st, sizeof(st));
+    }
     if (!init_flag) {
         if (esz == 8)
             launch_tag<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
@@ -709,6 +766,32 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host)
     if (rc) return rc;
     return check_hip(hipMemcpy(host, c.d_reg, sizeof(double) * (size_t)c.n * c.regstride,
                                hipMemcpyDeviceToHost), "registers");
+}
+
+int hipclaw_make_tiles(hipclaw_ctx* ctx, void* tiles_dev, int* tags_dev)
+{
+    CTX_READY(ctx);
+    if (!tiles_dev) return set_error(HIPCLAW_ERR_ARG, "tiles_dev is NULL");
+    return ctx->c->make_tiles(tiles_dev, tags_dev);
+}
+
+int hipclaw_set_tagging_cnn(hipclaw_ctx* ctx, hipclaw_cnn* net, float margin)
+{
+    CTX_CHECK(ctx);
+    if (net && (ctx->c->phys.mx != 32 || ctx->c->phys.my != 32))
+        return set_error(HIPCLAW_ERR_UNSUPPORTED, "the network takes 32x32 patches");
+    if (!(margin >= 0.f && margin <= 0.5f)) return set_error(HIPCLAW_ERR_ARG, "margin must lie in [0, 0.5]");
+    ctx->c->cnn = net;
+    ctx->c->cnn_margin = margin;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_tag_stats(hipclaw_ctx* ctx, int* stats6)
+{
+    CTX_CHECK(ctx);
+    if (!stats6) return set_error(HIPCLAW_ERR_ARG, "stats6 is NULL");
+    std::memcpy(stats6, ctx->c->tag_stats, sizeof(ctx->c->tag_stats));
+    return HIPCLAW_OK;
 }
 
 int hipclaw_autotune(hipclaw_ctx* ctx, int level, int reps, hipclaw_launch* best, float* ms)

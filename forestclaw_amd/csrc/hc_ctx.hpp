@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/hipclaw.h"
+#include "../../include/hipclaw_cnn.h"
 #include "hc_amr.hpp"
 #include "hc_amr_launch.hpp"
 #include "hc_forest.hpp"
@@ -64,6 +65,12 @@ struct Ctx {
     double swirl_tperiod = 0.0;    // > 0: advection velocities from the swirl stream function
     int cflacc_count = 0;
     hipclaw_launch launch{0, 0};   // patch-step launch shape; 0 = library default
+    hipclaw_cnn* cnn = nullptr;    // learned refinement tags (not owned)
+    float cnn_margin = 0.0f;
+    void* d_tiles = nullptr;       // network input of every slot (bf16)
+    float* d_logits = nullptr;
+    int tiles_cap = 0;
+    int tag_stats[6] = {};         // patches, net refine, plain refine, misses, false refines, fallbacks
     bool tsync = false;            // conservative time synchronisation on
     double* d_reg = nullptr;       // time-sync registers, regstride doubles per slot (FP64)
     long long regstride = 0;
@@ -92,6 +99,7 @@ struct Ctx {
     int level_range(int level, int* s0, int* s1) const;
     int autotune(int level, int reps, hipclaw_launch* best, float* ms);
     int set_time_sync(bool on);
+    int make_tiles(void* tiles, int* tags);
     int time_sync(int minlevel, int maxlevel);
 };
 

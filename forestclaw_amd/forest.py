@@ -149,6 +149,22 @@ class Hipclaw46:
         _check(_lib().hipclaw_get_registers(self.h, r.ctypes.data_as(_DP)))
         return r
 
+    def make_tiles(self, tiles, tags=None):
+        """Network input of every patch (torch bf16 [n, 36, 36, 8] on the device);
+        tags (torch int32 [n], optional) receives the plain refinement tags."""
+        _check(_lib().hipclaw_make_tiles(self.h, H._ptr(tiles), H._ptr(tags)))
+
+    def set_tagging_cnn(self, net, margin: float = 0.0):
+        """Refinement tags of regrid from the network (None: plain tags)."""
+        _check(_lib().hipclaw_set_tagging_cnn(self.h, None if net is None else net.h, margin))
+
+    def tag_stats(self):
+        """patches, network refines, plain refines, misses, false refines, fallbacks."""
+        st = (ctypes.c_int * 6)()
+        _check(_lib().hipclaw_tag_stats(self.h, st))
+        return dict(zip(("patches", "net_refine", "plain_refine", "misses", "false_refines",
+                         "fallbacks"), list(st)))
+
     def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None):
         """Patch-step launch shape for this context (None: library default)."""
         if cols_per_lane is None:

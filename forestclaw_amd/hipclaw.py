@@ -67,6 +67,19 @@ _SIGS = {
                                   _VP, ctypes.c_int, ctypes.c_double, _VP, _VP,
                                   ctypes.POINTER(Launch), _VP], ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_cnn_create": ([_VP, ctypes.POINTER(_VP)], ctypes.c_int),
+    "hipclaw_cnn_destroy": ([_VP], ctypes.c_int),
+    "hipclaw_cnn_num_params": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
+    "hipclaw_cnn_init_params": ([_VP, ctypes.c_ulonglong], ctypes.c_int),
+    "hipclaw_cnn_set_params": ([_VP, _VP], ctypes.c_int),
+    "hipclaw_cnn_get_params": ([_VP, _VP], ctypes.c_int),
+    "hipclaw_cnn_forward": ([_VP, _VP, _VP, ctypes.c_int, _VP], ctypes.c_int),
+    "hipclaw_cnn_gradients": ([_VP, _VP, _VP, _VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_cnn_train_step": ([_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_float, _VP],
+                               ctypes.c_int),
+    "hipclaw_make_tiles": ([_VP, _VP, _VP], ctypes.c_int),
+    "hipclaw_set_tagging_cnn": ([_VP, _VP, ctypes.c_float], ctypes.c_int),
+    "hipclaw_tag_stats": ([_VP, _IP], ctypes.c_int),
     "hipclaw_batch_create": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, ctypes.POINTER(_VP)],
                              ctypes.c_int),
     "hipclaw_batch_destroy": ([_VP], ctypes.c_int),
@@ -211,6 +224,61 @@ def step2_batch_sync(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: flo
         ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy), None,
         int(npatch), float(dt), _ptr(cfl), _ptr(registers),
         ctypes.byref(ln) if ln is not None else None, _stream(stream)))
+
+
+class Cnn:
+    """The learned refinement indicator (hipclaw_cnn_*): parameters live on
+    the device; tiles are device bf16 [n, 36, 36, 8] (torch.bfloat16)."""
+
+    def __init__(self, stream=None, seed: int | None = None):
+        h = _VP()
+        _check(lib().hipclaw_cnn_create(_stream(stream), ctypes.byref(h)))
+        self.h = h
+        n = ctypes.c_longlong()
+        _check(lib().hipclaw_cnn_num_params(self.h, ctypes.byref(n)))
+        self.num_params = n.value
+        if seed is not None:
+            _check(lib().hipclaw_cnn_init_params(self.h, seed))
+
+    def get_params(self):
+        import numpy as np
+        out = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().hipclaw_cnn_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params
+        _check(lib().hipclaw_cnn_set_params(self.h, p.ctypes.data))
+
+    def forward(self, tiles, logits, idx=None, n=None):
+        n = tiles.shape[0] if n is None else n
+        _check(lib().hipclaw_cnn_forward(self.h, _ptr(tiles), _ptr(idx), n, _ptr(logits)))
+
+    def gradients(self, tiles, labels, idx=None, n=None):
+        import numpy as np
+        n = tiles.shape[0] if n is None else n
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        _check(lib().hipclaw_cnn_gradients(self.h, _ptr(tiles), _ptr(labels), _ptr(idx), n,
+                                           g.ctypes.data, loss.ctypes.data))
+        return float(loss[0]), g
+
+    def train_step(self, tiles, labels, idx, n, lr, loss_dev=None):
+        _check(lib().hipclaw_cnn_train_step(self.h, _ptr(tiles), _ptr(labels), _ptr(idx), n,
+                                            float(lr), _ptr(loss_dev)))
+
+    def close(self):
+        if self.h:
+            lib().hipclaw_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
