@@ -1,0 +1,19 @@
+"""FP32 precision mode on every configuration: relative L1 error of the FP32
+interior against the FP64 run on the same mesh from the same data stays
+<= 1e-5 (C1 swirl 200 steps, C2 dam break and C3 shock-bubble on their AMR
+meshes); the errors are printed as the per-configuration report."""
+import pytest
+
+from forestclaw_amd import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name,checkpoints", [("C1", (1, 50, 200)), ("C2", (1, 10, 40)),
+                                              ("C3", (1, 10, 30))])
+def test_fp32_error_per_config(dev, name, checkpoints):
+    rep = C.precision_report(name, checkpoints)
+    for r in rep:
+        print(f"\n{name} after {r['steps']} coarse steps on {r['patches']} patches: "
+              f"rel L1 {r['rel_l1']:.3e}, rel Linf {r['rel_linf']:.3e}")
+        assert r["rel_l1"] <= 1e-5
