@@ -207,6 +207,19 @@ static int step2_batch_impl(const hipclaw_physics* phys, int npatch, const void*
                             const int* patch_ids, int ntiles, double dt, double* cfl_patch,
                             double* reg, const hipclaw_launch* launch, void* stream)
 {
+    return hc::step2_launch(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt, cfl_patch,
+                            reg, launch, stream, nullptr);
+}
+
+}  // extern "C"
+
+namespace hc {
+
+int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
+                 const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,
+                 double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,
+                 const MultiLevel* ml)
+{
     int rc = validate_physics(phys);
     if (rc) return rc;
     if (npatch < 0) return set_error(HIPCLAW_ERR_ARG, "npatch < 0");
@@ -245,11 +258,25 @@ static int step2_batch_impl(const hipclaw_physics* phys, int npatch, const void*
     a.reg = reg;
     a.regstride = 8LL * phys->meqn * (phys->mx + phys->my);
     if (reg && phys->mbc != 2) return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs mbc = 2");
+    if (ml) {
+        if (patch_ids) return set_error(HIPCLAW_ERR_ARG, "multi-level launches take no patch list");
+        a.levels = ml->levels;
+        a.slot0 = ml->slot0;
+        for (int L = 0; L < STEP_MAXL; L++) {
+            a.dtl[L] = ml->dtl[L];
+            a.qin_l[L] = ml->qin_l[L];
+            a.qout_l[L] = ml->qout_l[L];
+        }
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (phys->precision == HIPCLAW_FP64)
         return launch_step2_fp64(a, phys->solver, cpl, wpg, q_in, q_out, aux, cfl_patch, s);
     return launch_step2_fp32(a, phys->solver, cpl, wpg, q_in, q_out, aux, cfl_patch, s);
 }
+
+}  // namespace hc
+
+extern "C" {
 
 int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream)
 {

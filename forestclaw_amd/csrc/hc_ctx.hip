@@ -432,6 +432,54 @@ int Ctx::time_sync(int minlevel, int maxlevel)
     return check_hip(hipGetLastError(), "time sync");
 }
 
+int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
+{
+    if (lo == hi || !merge_levels) {
+        for (int L = hi; L >= lo; L--) HC_TRY(update_level(L, tl[L], dtl[L]));
+        return HIPCLAW_OK;
+    }
+    const int s0 = plan[lo].s0, s1 = plan[hi].s1;
+    for (int L = lo; L <= hi; L++)
+        if (plan[L].s1 <= plan[L].s0 || (L > lo && plan[L].s0 != plan[L - 1].s1))
+            return set_error(HIPCLAW_ERR_STATE, "merged update needs contiguous non-empty levels");
+    const int np = s1 - s0;
+    MultiLevel ml{};
+    ml.levels = d_levels;
+    ml.slot0 = s0;
+    for (int L = lo; L <= hi; L++) {
+        ml.dtl[L] = dtl[L];
+        ml.qin_l[L] = pool[parity[L]];
+        ml.qout_l[L] = pool[parity[L] ^ 1];
+        if (swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV) {
+            const int a0 = plan[L].s0, na = plan[L].s1 - a0;
+            char* a = static_cast<char*>(aux) + (size_t)a0 * atile * esz;
+            HC_TRY(hipclaw_swirl_aux_batch(&phys, na, d_xylower + 2 * a0, d_dxdy + 2 * a0, tl[L], dtl[L],
+                                           swirl_tperiod, a, stream));
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) {
+        HC_TRY(check_hip(hipEventCreate(&e0), "event"));
+        HC_TRY(check_hip(hipEventCreate(&e1), "event"));
+        HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
+    }
+    const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
+    HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
+                        tsync ? d_reg : nullptr, lc, stream, &ml));
+    if (prof) {
+        HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
+        ev.push_back({e0, e1});
+        ev_cells.push_back((long long)np * phys.mx * phys.my);
+    }
+    if (cflacc_count < 4095) {
+        HC_TRY(hipclaw_max_reduce(d_cfl + s0, np, d_cflacc + cflacc_count, stream));
+        cflacc_count++;
+    }
+    for (int L = lo; L <= hi; L++) parity[L] ^= 1;
+    cell_updates += (long long)np * phys.mx * phys.my;
+    return HIPCLAW_OK;
+}
+
 int Ctx::timeinterp(int level, double alpha)
 {
     const LevelPlan& lp = plan[level];
@@ -470,27 +518,37 @@ int Ctx::advance(double t, double dt, double* maxcfl)
             d /= 2;
         }
     }
-    // recursive level advance (advance_level)
-    auto advance_level = [&](auto& self, int level, int curr_fine_step) -> int {
-        HC_TRY(update_level(level, c[level].t, c[level].dt_step));
+    // recursive level advance (advance_level).  The recursion is walked on the
+    // host first: the levels it updates at one fine step are independent of
+    // each other (each reads its own pool and writes its other one), so they
+    // form one launch; the time interpolations follow in the recursion's order.
+    double tl[MAXL], dtl[MAXL];
+    int ulo = hi;
+    std::vector<std::pair<int, double>> tis;
+    auto advance_level = [&](auto& self, int level, int curr_fine_step) -> void {
+        tl[level] = c[level].t;
+        dtl[level] = c[level].dt_step;
+        ulo = level;
         c[level].last_step += c[level].step_inc;
         c[level].t += c[level].dt_step;
         if (level > lo) {
             const int coarser = level - 1;
             if (c[coarser].last_step == curr_fine_step) {
-                HC_TRY(self(self, coarser, c[coarser].last_step));
+                self(self, coarser, c[coarser].last_step);
                 if (sub) {
                     const int coarse_inc = c[level - 1].step_inc;
-                    const double alpha = (double)(c[level].last_step % coarse_inc) / coarse_inc;
-                    HC_TRY(timeinterp(coarser, alpha));
+                    tis.push_back({coarser, (double)(c[level].last_step % coarse_inc) / coarse_inc});
                 }
             }
         }
-        return HIPCLAW_OK;
     };
     const int nfine = c[hi].total_steps;
     for (int nf = 0; nf < nfine; nf++) {
-        HC_TRY(advance_level(advance_level, hi, nf));
+        ulo = hi;
+        tis.clear();
+        advance_level(advance_level, hi, nf);
+        HC_TRY(update_levels(ulo, hi, tl, dtl));
+        for (const auto& ti : tis) HC_TRY(timeinterp(ti.first, ti.second));
         if (c[hi].last_step < c[hi].total_steps) {
             if (sub) {
                 int tl = hi - 1;

@@ -91,6 +91,10 @@ struct Ctx {
     template <class R> Pools<R> pools() const;
     int ghost_update(int minlevel, int maxlevel, int time_interp);
     int update_level(int level, double t, double dt);
+    // one merged launch for the contiguous levels [lo, hi] due at the same
+    // subcycled time (t[L], dt[L] per level)
+    int update_levels(int lo, int hi, const double* t, const double* dt);
+    bool merge_levels = true;
     int timeinterp(int level, double alpha);
     int advance(double t, double dt, double* maxcfl);
     int qinit(int kind, const QinitParams& prm);

@@ -24,4 +24,19 @@ int launch_step2_fp32(const StepArgs& a, int solver, int cpl, int wpg, const voi
 // the message set.
 int validate_physics(const hipclaw_physics* p);
 
+// Patch-step launch behind hipclaw_step2_batch(_sync); with ml, one launch
+// over the slots [ml->slot0, ml->slot0 + npatch) of several levels (q_in /
+// q_out unused, dxdy / aux / cfl / reg addressed by slot).
+struct MultiLevel {
+    const int* levels;
+    int slot0;
+    double dtl[STEP_MAXL];
+    const void* qin_l[STEP_MAXL];
+    void* qout_l[STEP_MAXL];
+};
+int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
+                 const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,
+                 double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,
+                 const MultiLevel* ml);
+
 }  // namespace hc

@@ -28,6 +28,8 @@
 
 namespace hc {
 
+constexpr int STEP_MAXL = 22;  // levels 0..21, as the AMR driver
+
 struct StepArgs {
     int npatch;
     int mx, my;
@@ -43,6 +45,14 @@ struct StepArgs {
     long long astride;     // elements per patch in aux (NX*NY*maux)
     double* reg;           // time-sync registers (SYNC kernels only), FP64
     long long regstride;   // doubles per patch in reg = 8*meqn*(mx+my)
+    // Multi-level launch (levels != nullptr): tile p of the launch is slot
+    // slot0 + p of the slot-indexed arrays (dxdy, aux, cfl, reg and the q
+    // pools), and its level levels[slot] selects the q pool pair and dt.
+    const int* levels;
+    int slot0;
+    double dtl[STEP_MAXL];
+    const void* qin_l[STEP_MAXL];
+    void* qout_l[STEP_MAXL];
 };
 
 // Time-sync register blocks of one face: P (fp/gp), M (fm/gm), E0/E1 (edge
@@ -109,14 +119,21 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int patch = pact ? praw : 0;
     const int NX = A.mx + 4, NY = A.my + 4;
 
-    const R* __restrict__ qp = qin + (long long)patch * A.qstride;
-    R* __restrict__ qo = qout + (long long)patch * A.qstride;
-    const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)patch * A.astride : nullptr;
+    // tile index into the tile-indexed arrays; multi-level launches address
+    // slots and take q pools and dt from the slot's level
+    const int tix = A.levels ? A.slot0 + patch : patch;
+    const int lev = (A.levels && pact) ? A.levels[tix] : 0;
+    const R* qbase = A.levels ? static_cast<const R*>(A.qin_l[lev]) : qin;
+    R* obase = A.levels ? static_cast<R*>(A.qout_l[lev]) : qout;
+    const double dtp = A.levels ? A.dtl[lev] : A.dt;
+    const R* __restrict__ qp = qbase + (long long)tix * A.qstride;
+    R* __restrict__ qo = obase + (long long)tix * A.qstride;
+    const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)tix * A.astride : nullptr;
 
-    const double dx = pact ? A.dxdy[2 * patch] : 1.0;
-    const double dy = pact ? A.dxdy[2 * patch + 1] : 1.0;
-    const R dtdx = R(A.dt / dx);
-    const R dtdy = R(A.dt / dy);
+    const double dx = pact ? A.dxdy[2 * tix] : 1.0;
+    const double dy = pact ? A.dxdy[2 * tix + 1] : 1.0;
+    const R dtdx = R(dtp / dx);
+    const R dtdy = R(dtp / dy);
     const R hdtdx = R(0.5) * dtdx;
     const R hdtdy = R(0.5) * dtdy;
     const PhysParams pp = A.pp;
@@ -180,8 +197,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
         // ---- time-sync edge fluxes of the data before the update ----------
         if constexpr (SYNC) {
-            double* rg = A.reg + (long long)patch * A.regstride;
-            const double dtel_y = A.dt * dy, dtel_x = A.dt * dx;
+            double* rg = A.reg + (long long)tix * A.regstride;
+            const double dtel_y = dtp * dy, dtel_x = dtp * dx;
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
                 const int c = col[k];
@@ -527,9 +544,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     }
                     if constexpr (SYNC) {
                         // fluctuations on the patch edges (accumulate_waves)
-                        double* rg = A.reg + (long long)patch * A.regstride;
+                        double* rg = A.reg + (long long)tix * A.regstride;
                         const int c = col[k];
-                        const double dtel_y = A.dt * dy, dtel_x = A.dt * dx;
+                        const double dtel_y = dtp * dy, dtel_x = dtp * dx;
                         if (cv[k] && (c == 1 || c == NX - 3)) {
                             const R fmr = (fmx2[k][m] + e2[k][m]) + e1r;
                             reg_add(rg + reg_off(A.mx, A.my, ME, c == 1 ? 0 : 1, 1) + m * A.my + (Rw - 2),
@@ -557,8 +574,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
         // ---- time-sync fluctuations on the two ghost-row edge faces ---------
         if constexpr (SYNC) {
-            double* rg = A.reg + (long long)patch * A.regstride;
-            const double dtel_x = A.dt * dx;
+            double* rg = A.reg + (long long)tix * A.regstride;
+            const double dtel_x = dtp * dx;
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
                 const int c = col[k];
@@ -612,7 +629,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         double t = __shfl_up(cd, d, HC_WAVE);
         if (lp - d >= 0) cd = dmax(cd, t);
     }
-    if (pact && lp == A.lpp - 1 && cflp) cflp[patch] = cd;
+    if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
 }
 
 }  // namespace hc
