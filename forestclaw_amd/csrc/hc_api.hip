@@ -262,6 +262,7 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         if (patch_ids) return set_error(HIPCLAW_ERR_ARG, "multi-level launches take no patch list");
         a.levels = ml->levels;
         a.slot0 = ml->slot0;
+        a.push = ml->push;
         for (int L = 0; L < STEP_MAXL; L++) {
             a.dtl[L] = ml->dtl[L];
             a.qin_l[L] = ml->qin_l[L];

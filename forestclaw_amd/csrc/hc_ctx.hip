@@ -79,7 +79,7 @@ void Ctx::release_pools()
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
-    int** ibufs[] = {&d_levels, &d_iota, &d_tags};
+    int** ibufs[] = {&d_levels, &d_iota, &d_tags, &d_push};
     for (auto b : ibufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -255,6 +255,16 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
         HC_TRY(lp.ts_f2c.upload(hp.ts_f2c[L]));
         HC_TRY(lp.ts_phys.upload(hp.ts_phys[L]));
     }
+    // same-level push table: the copy op (dst k, face / corner c, src nb)
+    // becomes "nb writes into k across its opposite face / corner"
+    {
+        std::vector<int> push;
+        build_push_table(hp, nn, push);
+        HC_TRY(dmalloc((void**)&d_push, sizeof(int) * push.size()));
+        HC_TRY(check_hip(hipMemcpy(d_push, push.data(), sizeof(int) * push.size(), hipMemcpyHostToDevice),
+                         "push table"));
+    }
+    clear_pushed();
     // registers start at zero on a new mesh (a regrid happens after a full
     // coarse step, when every synchronised register has been reset)
     regstride = 8LL * phys.meqn * (phys.mx + phys.my);
@@ -283,7 +293,7 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
             launch_ghost<R>(plan[L].avg.p, plan[L].avg.n, P, mx, my, meqn, stream);
         if (use_ti) launch_ghost<R>(plan[tl].avg_ti.p, plan[tl].avg_ti.n, P, mx, my, meqn, stream);
         for (int L = maxlevel; L >= minlevel; L--)
-            launch_ghost<R>(plan[L].copy.p, plan[L].copy.n, P, mx, my, meqn, stream);
+            if (!pushed[L]) launch_ghost<R>(plan[L].copy.p, plan[L].copy.n, P, mx, my, meqn, stream);
         if (use_ti) launch_ghost<R>(plan[tl].copy_ti.p, plan[tl].copy_ti.n, P, mx, my, meqn, stream);
         for (int L = maxcoarse; L >= mincoarse; L--)
             launch_bc<R>(plan[L].bc.p, plan[L].bc.n, P, mx, my, meqn, stream);
@@ -333,6 +343,7 @@ int Ctx::update_level(int level, double t, double dt)
         cflacc_count++;
     }
     parity[level] ^= 1;
+    pushed[level] = false;
     cell_updates += (long long)np * phys.mx * phys.my;
     return HIPCLAW_OK;
 }
@@ -411,6 +422,7 @@ int Ctx::set_time_sync(bool on)
 int Ctx::time_sync(int minlevel, int maxlevel)
 {
     if (!tsync) return HIPCLAW_OK;
+    clear_pushed();
     if (minlevel < 0 || maxlevel >= MAXL || minlevel > maxlevel)
         return set_error(HIPCLAW_ERR_ARG, "time_sync: bad level range");
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
@@ -434,7 +446,7 @@ int Ctx::time_sync(int minlevel, int maxlevel)
 
 int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
 {
-    if (lo == hi || !merge_levels) {
+    if (!merge_levels) {
         for (int L = hi; L >= lo; L--) HC_TRY(update_level(L, tl[L], dtl[L]));
         return HIPCLAW_OK;
     }
@@ -464,8 +476,11 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
     const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
+    const bool push = push_ghosts && !tsync && phys.mx >= 4 && phys.my >= 4;
+    ml.push = push ? d_push : nullptr;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
+    for (int L = lo; L <= hi; L++) pushed[L] = push;
     if (prof) {
         HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
         ev.push_back({e0, e1});
@@ -575,6 +590,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
 
 int Ctx::qinit(int kind, const QinitParams& prm)
 {
+    clear_pushed();
     if (esz == 8)
         launch_qinit<double>(kind, prm, d_iota, d_levels, n, pools<double>(), d_xylower, d_dxdy, phys.mx,
                              phys.my, phys.meqn, stream);
@@ -891,6 +907,7 @@ int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* x
 
 static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
 {
+    if (to_device) c.clear_pushed();
     const long long per = auxdata ? c.atile : c.tile;
     const size_t tot = (size_t)c.n * per;
     if (tot == 0) return HIPCLAW_OK;
@@ -925,6 +942,7 @@ static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
 // Per-patch host arrays of one level <-> the level's slots of the current pool.
 static int level_copy(Ctx& c, int level, int n, double* const* q, const double* const* aux, bool up)
 {
+    if (up) c.clear_pushed();
     if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
     const int s0 = c.plan[level].s0, np = c.plan[level].s1 - s0;
     if (n != np) return set_error(HIPCLAW_ERR_ARG, "n differs from the number of patches of the level");

@@ -95,6 +95,15 @@ struct Ctx {
     // subcycled time (t[L], dt[L] per level)
     int update_levels(int lo, int hi, const double* t, const double* dt);
     bool merge_levels = true;
+    // same-level ghost push from the step kernel (off with time sync, whose
+    // corrections change edge cells between a step and the next ghost fill)
+    bool push_ghosts = false;
+    int* d_push = nullptr;          // 8 ints per slot
+    bool pushed[MAXL] = {};         // level data unchanged since a pushing step
+    void clear_pushed()
+    {
+        for (bool& b : pushed) b = false;
+    }
     int timeinterp(int level, double alpha);
     int advance(double t, double dt, double* maxcfl);
     int qinit(int kind, const QinitParams& prm);

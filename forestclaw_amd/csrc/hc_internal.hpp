@@ -33,6 +33,7 @@ struct MultiLevel {
     double dtl[STEP_MAXL];
     const void* qin_l[STEP_MAXL];
     void* qout_l[STEP_MAXL];
+    const int* push;  // optional same-level ghost push table (8 ints per slot)
 };
 int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                  const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,

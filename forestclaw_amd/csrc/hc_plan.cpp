@@ -94,4 +94,15 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
     }
 }
 
+void build_push_table(const HostPlans& hp, int nslots, std::vector<int>& push)
+{
+    static const int opp_face[4] = {1, 0, 3, 2};
+    push.assign((size_t)nslots * 8, -1);
+    for (int L = 0; L < MAXL; L++)
+        for (const GhostOp& o : hp.copy[L]) {
+            if (o.kind == G_COPY_FACE) push[(size_t)o.src * 8 + opp_face[o.sub]] = o.dst;
+            else if (o.kind == G_COPY_CORNER) push[(size_t)o.src * 8 + 4 + (3 - o.sub)] = o.dst;
+        }
+}
+
 }  // namespace hc

@@ -28,4 +28,10 @@ struct HostPlans {
 // into flat per-level operation lists.  Slot k is leaf k.
 void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], HostPlans& out);
 
+// Same-level ghost push table of the step kernel: 8 ints per slot, the
+// neighbours that receive its edge cells across faces 0..3 and corners 0..3
+// (-1: none).  A copy op (dst k, face / corner c, src nb) becomes "nb writes
+// into k across its opposite face / corner".
+void build_push_table(const HostPlans& hp, int nslots, std::vector<int>& push);
+
 }  // namespace hc

@@ -53,6 +53,10 @@ struct StepArgs {
     double dtl[STEP_MAXL];
     const void* qin_l[STEP_MAXL];
     void* qout_l[STEP_MAXL];
+    // PUSH kernels: per slot, the same-level neighbours across faces 0..3 and
+    // corners 0..3 (-1: none); updated edge cells are also written into their
+    // ghost cells (the same-level copy of the next ghost update).
+    const int* push;
 };
 
 // Time-sync register blocks of one face: P (fp/gp), M (fm/gm), E0/E1 (edge
@@ -97,7 +101,11 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
 // conservative flux function only): store_flux on the data before the update
 // and accumulate_waves on the final face fluctuations
 // (src/solvers/fc2d_clawpack4.6/fc2d_clawpack46.cpp:317-341, 376-390).
-template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC>
+// PUSH: also write every updated cell within two cells of a patch edge into
+// the ghost cells of the same-level neighbour across that face or corner
+// (what fclaw2d_clawpatch46_fort_copy_face / _copy_corner would copy from it,
+// src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_copy.f:17-133).
+template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH>
 __global__ void __launch_bounds__(256)
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
@@ -616,6 +624,43 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             typ[k] = tyn[k];
             um2[k] = um1[k];
             um1[k] = ucur[k];
+        }
+    }
+
+    // ---- same-level ghost push --------------------------------------------
+    // Each lane reads back the edge cells of its own columns it has just
+    // written (same-thread ordering) and stores them into the ghost cells of
+    // the same-level neighbours across faces and corners.
+    if constexpr (PUSH) {
+        if (pact) {
+            const int* tb = A.push + 8 * tix;
+            const int nb_b = tb[2], nb_t = tb[3];
+            auto put = [&](int nb, int cc, int rr, int m, R v) {
+                obase[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
+            };
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int c = col[k];
+                if (!(cv[k] && c >= 2 && c <= NX - 3)) continue;
+                const bool lx = c <= 3, hx = c >= NX - 4;
+                const int nbx = lx ? tb[0] : (hx ? tb[1] : -1);
+                const int nbcl = lx ? tb[4] : (hx ? tb[5] : -1);
+                const int nbch = lx ? tb[6] : (hx ? tb[7] : -1);
+                const int xo = lx ? A.mx : -A.mx;
+                for (int m = 0; m < ME; m++) {
+                    const R* col_m = qo + (long long)m * NY * NX + c;
+                    if (nbx >= 0)
+                        for (int rr = 2; rr <= NY - 3; rr++) put(nbx, c + xo, rr, m, col_m[(long long)rr * NX]);
+                    if (nbcl >= 0)
+                        for (int rr = 2; rr <= 3; rr++) put(nbcl, c + xo, rr + A.my, m, col_m[(long long)rr * NX]);
+                    if (nbch >= 0)
+                        for (int rr = NY - 4; rr <= NY - 3; rr++) put(nbch, c + xo, rr - A.my, m, col_m[(long long)rr * NX]);
+                    if (nb_b >= 0)
+                        for (int rr = 2; rr <= 3; rr++) put(nb_b, c, rr + A.my, m, col_m[(long long)rr * NX]);
+                    if (nb_t >= 0)
+                        for (int rr = NY - 4; rr <= NY - 3; rr++) put(nb_t, c, rr - A.my, m, col_m[(long long)rr * NX]);
+                }
+            }
         }
     }
 

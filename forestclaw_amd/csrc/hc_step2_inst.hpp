@@ -6,11 +6,11 @@
 
 namespace hc {
 
-template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC>
+template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH>
 static void launch_k(const StepArgs& a, long long blocks, int wpg, const void* qin, void* qout,
                      const void* aux, double* cfl, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC>), dim3((unsigned)blocks),
+    hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH>), dim3((unsigned)blocks),
                        dim3(HC_WAVE * wpg), 0, s, a, static_cast<const R*>(qin),
                        static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
 }
@@ -30,20 +30,27 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     if (blocks == 0) return HIPCLAW_OK;
     bool mc22 = a.order2 == 2 && a.transverse == 2;
     for (int w = 0; w < MW; w++) mc22 = mc22 && a.mthlim[w] == 4;
+    if (a.reg && a.push) return set_error(HIPCLAW_ERR_ARG, "ghost push and time-sync registers exclude each other");
+    if (a.push && !a.levels) return set_error(HIPCLAW_ERR_ARG, "ghost push needs slot addressing");
     if (a.reg) {
         if constexpr (SolverT<R>::HAS_CONS) {
             if (mc22)
-                launch_k<R, SolverT, CPL, true, true>(a, blocks, wpg, qin, qout, aux, cfl, s);
+                launch_k<R, SolverT, CPL, true, true, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
             else
-                launch_k<R, SolverT, CPL, false, true>(a, blocks, wpg, qin, qout, aux, cfl, s);
+                launch_k<R, SolverT, CPL, false, true, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
         } else {
             return set_error(HIPCLAW_ERR_UNSUPPORTED,
                              "time-sync registers need a conservative flux (SWE only)");
         }
+    } else if (a.push) {
+        if (mc22)
+            launch_k<R, SolverT, CPL, true, false, true>(a, blocks, wpg, qin, qout, aux, cfl, s);
+        else
+            launch_k<R, SolverT, CPL, false, false, true>(a, blocks, wpg, qin, qout, aux, cfl, s);
     } else if (mc22) {
-        launch_k<R, SolverT, CPL, true, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
+        launch_k<R, SolverT, CPL, true, false, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
     } else {
-        launch_k<R, SolverT, CPL, false, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
+        launch_k<R, SolverT, CPL, false, false, false>(a, blocks, wpg, qin, qout, aux, cfl, s);
     }
     return check_hip(hipGetLastError(), "k_step2 launch");
 }

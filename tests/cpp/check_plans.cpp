@@ -12,7 +12,9 @@
 //       forestclaw_amd/csrc/hc_forest.cpp forestclaw_amd/csrc/hc_plan.cpp
 #include <cstdio>
 #include <cstdlib>
+#include <map>
 #include <random>
+#include <tuple>
 #include <vector>
 
 #include "hc_forest.hpp"
@@ -165,6 +167,85 @@ static void check_xfer(const std::vector<Transfer>& xf, int nold, int nnew, int 
     }
 }
 
+// The step kernel's same-level ghost push (PUSH epilogue of k_step2,
+// forestclaw_amd/csrc/hc_step2.hpp) must write exactly the cells, from exactly
+// the sources, that the same-level copy ops of the plans would.
+using CellMap = std::map<std::tuple<int, int, int>, std::tuple<int, int, int>>;
+
+static void copy_sources(const HostPlans& hp, int mx, int my, CellMap& out)
+{
+    for (int L = 0; L < MAXL; L++)
+        for (const GhostOp& o : hp.copy[L]) {
+            auto rec = [&](int i, int j) {
+                auto key = std::make_tuple(o.dst, i, j);
+                REQUIRE(!out.count(key), "ghost (%d,%d,%d) copied twice", o.dst, i, j);
+                out[key] = std::make_tuple(o.src, i + o.ox, j + o.oy);
+            };
+            if (o.kind == G_COPY_FACE) {
+                const int len = o.sub < 2 ? my : mx;
+                for (int layer = 0; layer < MBC; layer++)
+                    for (int a = MBC; a < MBC + len; a++) {
+                        switch (o.sub) {
+                        case 0: rec(layer, a); break;
+                        case 1: rec(MBC + mx + layer, a); break;
+                        case 2: rec(a, layer); break;
+                        default: rec(a, MBC + my + layer); break;
+                        }
+                    }
+            } else if (o.kind == G_COPY_CORNER) {
+                for (int jb = 1; jb <= MBC; jb++)
+                    for (int ib = 1; ib <= MBC; ib++) {
+                        const int i = (o.sub & 1) ? MBC + mx - 1 + ib : MBC - ib;
+                        const int j = (o.sub & 2) ? MBC + my - 1 + jb : MBC - jb;
+                        rec(i, j);
+                    }
+            }
+        }
+}
+
+static void push_sources(const std::vector<int>& tab, int n, int mx, int my, CellMap& out)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    auto put = [&](int nb, int cc, int rr, int src, int c, int r) {
+        REQUIRE(nb >= 0 && nb < n, "push target %d out of range", nb);
+        REQUIRE(cc >= 0 && cc < NX && rr >= 0 && rr < NY, "push cell (%d,%d) outside the tile", cc, rr);
+        auto key = std::make_tuple(nb, cc, rr);
+        REQUIRE(!out.count(key), "ghost (%d,%d,%d) pushed twice", nb, cc, rr);
+        out[key] = std::make_tuple(src, c, r);
+    };
+    for (int s = 0; s < n; s++) {
+        const int* tb = &tab[(size_t)s * 8];
+        for (int c = 2; c <= NX - 3; c++) {
+            const bool lx = c <= 3, hx = c >= NX - 4;
+            const int nbx = lx ? tb[0] : (hx ? tb[1] : -1);
+            const int nbcl = lx ? tb[4] : (hx ? tb[5] : -1);
+            const int nbch = lx ? tb[6] : (hx ? tb[7] : -1);
+            const int xo = lx ? mx : -mx;
+            if (nbx >= 0)
+                for (int rr = 2; rr <= NY - 3; rr++) put(nbx, c + xo, rr, s, c, rr);
+            if (nbcl >= 0)
+                for (int rr = 2; rr <= 3; rr++) put(nbcl, c + xo, rr + my, s, c, rr);
+            if (nbch >= 0)
+                for (int rr = NY - 4; rr <= NY - 3; rr++) put(nbch, c + xo, rr - my, s, c, rr);
+            if (tb[2] >= 0)
+                for (int rr = 2; rr <= 3; rr++) put(tb[2], c, rr + my, s, c, rr);
+            if (tb[3] >= 0)
+                for (int rr = NY - 4; rr <= NY - 3; rr++) put(tb[3], c, rr - my, s, c, rr);
+        }
+    }
+}
+
+static void check_push(const HostPlans& hp, int n, int mx, int my, const char* name)
+{
+    std::vector<int> tab;
+    build_push_table(hp, n, tab);
+    CellMap a, b;
+    copy_sources(hp, mx, my, a);
+    push_sources(tab, n, mx, my, b);
+    REQUIRE(a == b, "%s: push writes differ from the same-level copies (%zu vs %zu cells)", name, b.size(),
+            a.size());
+}
+
 static void check_forest(const Forest& F, int mx, int my, const char* name)
 {
     const int n = F.nleaves();
@@ -176,6 +257,7 @@ static void check_forest(const Forest& F, int mx, int my, const char* name)
     HostPlans hp;
     const int mthbc[4] = {1, 1, 1, 1};
     build_host_plans(F, mx, my, mthbc, hp);
+    check_push(hp, n, mx, my, name);
     const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
     std::vector<std::vector<char>> written(n, std::vector<char>(NX * NY, 0));
     long long nops = 0;
