@@ -842,6 +842,21 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host)
                                hipMemcpyDeviceToHost), "registers");
 }
 
+int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on)
+{
+    CTX_CHECK(ctx);
+    Ctx& c = *ctx->c;
+    switch (feature) {
+    case HIPCLAW_FEATURE_MERGE_LEVELS: c.merge_levels = on != 0; break;
+    case HIPCLAW_FEATURE_GHOST_PUSH:
+        c.push_ghosts = on != 0;
+        c.clear_pushed();
+        break;
+    default: return set_error(HIPCLAW_ERR_ARG, "unknown feature");
+    }
+    return HIPCLAW_OK;
+}
+
 int hipclaw_make_tiles(hipclaw_ctx* ctx, void* tiles_dev, int* tags_dev)
 {
     CTX_READY(ctx);

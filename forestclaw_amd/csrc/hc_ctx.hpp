@@ -97,7 +97,7 @@ struct Ctx {
     bool merge_levels = true;
     // same-level ghost push from the step kernel (off with time sync, whose
     // corrections change edge cells between a step and the next ghost fill)
-    bool push_ghosts = false;
+    bool push_ghosts = true;
     int* d_push = nullptr;          // 8 ints per slot
     bool pushed[MAXL] = {};         // level data unchanged since a pushing step
     void clear_pushed()

@@ -138,6 +138,12 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     R* __restrict__ qo = obase + (long long)tix * A.qstride;
     const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)tix * A.astride : nullptr;
 
+    // PUSH: this patch's edge cells are staged in LDS during the sweep, per
+    // patch 4*meqn*(mx+my) values: x edges [side][col][m][row], then y edges
+    // [side][row][m][col] (sides: low, high; two cells deep)
+    extern __shared__ char hc_smem[];
+    const int pstage = 4 * ME * (A.mx + A.my);
+    R* stage = reinterpret_cast<R*>(hc_smem) + ((threadIdx.x >> 6) * A.ppw + (slot < A.ppw ? slot : 0)) * pstage;
     const double dx = pact ? A.dxdy[2 * tix] : 1.0;
     const double dy = pact ? A.dxdy[2 * tix + 1] : 1.0;
     const R dtdx = R(dtp / dx);
@@ -549,6 +555,17 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R gpc = G2[k][m] + fdp_prev[k][m];
                         R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
                         qo[((long long)m * NY + Rw) * NX + col[k]] = qn;
+                        if constexpr (PUSH) {
+                            const int c = col[k];
+                            if (c <= 3)
+                                stage[((c - 2) * ME + m) * A.my + (Rw - 2)] = qn;
+                            else if (c >= NX - 4)
+                                stage[((2 + c - (NX - 4)) * ME + m) * A.my + (Rw - 2)] = qn;
+                            if (Rw <= 3)
+                                stage[4 * ME * A.my + ((Rw - 2) * ME + m) * A.mx + (c - 2)] = qn;
+                            else if (Rw >= NY - 4)
+                                stage[4 * ME * A.my + ((2 + Rw - (NY - 4)) * ME + m) * A.mx + (c - 2)] = qn;
+                        }
                     }
                     if constexpr (SYNC) {
                         // fluctuations on the patch edges (accumulate_waves)
@@ -628,38 +645,38 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
 
     // ---- same-level ghost push --------------------------------------------
-    // Each lane reads back the edge cells of its own columns it has just
-    // written (same-thread ordering) and stores them into the ghost cells of
-    // the same-level neighbours across faces and corners.
+    // After the sweep every lane of the wave stores a share of the staged
+    // edge cells of the wave's patches into the ghost cells of their
+    // same-level neighbours across faces and corners.
     if constexpr (PUSH) {
-        if (pact) {
-            const int* tb = A.push + 8 * tix;
-            const int nb_b = tb[2], nb_t = tb[3];
+        __syncthreads();  // every thread of the block reaches this point
+        for (int sp = 0; sp < A.ppw; sp++) {
+            const int pi = gwave * A.ppw + sp;
+            if (pi >= A.npatch || pi >= A.ntiles) break;  // uniform across the wave
+            const int st = A.slot0 + pi;
+            const int* tb = A.push + 8 * st;
+            R* ob = static_cast<R*>(A.qout_l[A.levels[st]]);
+            const R* sg = reinterpret_cast<const R*>(hc_smem) + ((threadIdx.x >> 6) * A.ppw + sp) * pstage;
             auto put = [&](int nb, int cc, int rr, int m, R v) {
-                obase[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
+                ob[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
             };
-#pragma unroll
-            for (int k = 0; k < CPL; k++) {
-                const int c = col[k];
-                if (!(cv[k] && c >= 2 && c <= NX - 3)) continue;
-                const bool lx = c <= 3, hx = c >= NX - 4;
-                const int nbx = lx ? tb[0] : (hx ? tb[1] : -1);
-                const int nbcl = lx ? tb[4] : (hx ? tb[5] : -1);
-                const int nbch = lx ? tb[6] : (hx ? tb[7] : -1);
-                const int xo = lx ? A.mx : -A.mx;
-                for (int m = 0; m < ME; m++) {
-                    const R* col_m = qo + (long long)m * NY * NX + c;
-                    if (nbx >= 0)
-                        for (int rr = 2; rr <= NY - 3; rr++) put(nbx, c + xo, rr, m, col_m[(long long)rr * NX]);
-                    if (nbcl >= 0)
-                        for (int rr = 2; rr <= 3; rr++) put(nbcl, c + xo, rr + A.my, m, col_m[(long long)rr * NX]);
-                    if (nbch >= 0)
-                        for (int rr = NY - 4; rr <= NY - 3; rr++) put(nbch, c + xo, rr - A.my, m, col_m[(long long)rr * NX]);
-                    if (nb_b >= 0)
-                        for (int rr = 2; rr <= 3; rr++) put(nb_b, c, rr + A.my, m, col_m[(long long)rr * NX]);
-                    if (nb_t >= 0)
-                        for (int rr = NY - 4; rr <= NY - 3; rr++) put(nb_t, c, rr - A.my, m, col_m[(long long)rr * NX]);
-                }
+            const int nx_items = 4 * ME * A.my;
+            for (int e = lane; e < nx_items; e += HC_WAVE) {
+                // lanes pair up on the two edge columns of one row (8 contiguous bytes)
+                const int cb = e & 1, u = e >> 1;
+                const int r = u % A.my, t = u / A.my, m = t % ME, side = t / ME;
+                const int c = side ? NX - 4 + cb : 2 + cb, Rw = r + 2;
+                const int xo = side ? -A.mx : A.mx;
+                const R v = sg[((side * 2 + cb) * ME + m) * A.my + r];
+                if (tb[side] >= 0) put(tb[side], c + xo, Rw, m, v);
+                if (Rw <= 3 && tb[4 + side] >= 0) put(tb[4 + side], c + xo, Rw + A.my, m, v);
+                if (Rw >= NY - 4 && tb[6 + side] >= 0) put(tb[6 + side], c + xo, Rw - A.my, m, v);
+            }
+            const int ny_items = 4 * ME * A.mx;
+            for (int e = lane; e < ny_items; e += HC_WAVE) {
+                const int i = e % A.mx, t = e / A.mx, m = t % ME, sr = t / ME;  // sr = side*2 + row
+                const int side = sr >> 1, Rw = side ? NY - 4 + (sr & 1) : 2 + (sr & 1);
+                if (tb[2 + side] >= 0) put(tb[2 + side], i + 2, Rw + (side ? -A.my : A.my), m, sg[nx_items + e]);
             }
         }
     }

@@ -140,6 +140,10 @@ class Hipclaw46:
         corrections after every synchronising ghost update of advance()."""
         _check(_lib().hipclaw_set_time_sync(self.h, int(bool(on))))
 
+    def set_feature(self, feature: int, on: bool):
+        """H.FEATURE_MERGE_LEVELS / H.FEATURE_GHOST_PUSH (both on by default)."""
+        _check(_lib().hipclaw_set_feature(self.h, feature, int(bool(on))))
+
     def time_sync(self, minlevel: int, maxlevel: int):
         _check(_lib().hipclaw_time_sync(self.h, minlevel, maxlevel))
 

@@ -209,6 +209,15 @@ int hipclaw_set_time_sync(hipclaw_ctx* ctx, int on);
 int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel);
 /* Copy of all registers (num_patches * 8*meqn*(mx+my) doubles, slot order). */
 int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
+/* Execution features of hipclaw_advance (both on by default; results are
+ * bitwise the same either way):
+ *   MERGE_LEVELS  one patch-step launch for all levels due at a subcycled time
+ *   GHOST_PUSH    the step writes updated edge cells into the ghost cells of
+ *                 same-level neighbours, replacing the same-level copies of the
+ *                 next ghost fill (not used while time sync is on) */
+#define HIPCLAW_FEATURE_MERGE_LEVELS 1
+#define HIPCLAW_FEATURE_GHOST_PUSH 2
+int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on);
 /* Patch-step launch shape used by this context (NULL = library default). */
 int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch);
 /* Autotuner: times every launch shape (cols_per_lane 1..2 x waves_per_group

@@ -189,3 +189,19 @@ def test_autotune_and_launch_shapes_bitwise(dev):
     best, sweep = sim.autotune(-1, 2)
     assert best in sweep and len(sweep) == 6 and all(v > 0 for v in sweep.values())
     sim.set_launch(None)
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_merged_launch_and_ghost_push_change_no_bits(dev, precision):
+    """hipclaw_advance with one launch per subcycled time and the same-level
+    ghost push gives the same bits as per-level launches with ghost copies."""
+    res = []
+    for on in (True, False):
+        sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, precision=precision, refine=0.5)
+        sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        sim.set_feature(H.FEATURE_MERGE_LEVELS, on)
+        sim.set_feature(H.FEATURE_GHOST_PUSH, on)
+        cfl = [sim.advance(s * 0.01, 0.01) for s in range(3)]
+        res.append((cfl, sim.get_q()))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
