@@ -42,14 +42,22 @@ def humps(seed=7, n=8):
     return p
 
 
+_STREAM = []
+
+
 def make_sim(precision, args, leaves=None, q=None):
+    """Simulation on a dedicated (default-priority) stream: graph capture
+    needs a stream other than the null stream."""
+    import torch
     from forestclaw_amd import forest as F
     from forestclaw_amd import hipclaw as H
+    if not _STREAM:
+        _STREAM.append(torch.cuda.Stream())
     opt = F.Options(minlevel=args.minlevel, maxlevel=args.maxlevel, mx=32, my=32,
                     solver=H.SOLVER_SWE_ROE, precision=precision, grav=1.0,
                     refine_threshold=args.refine_threshold, coarsen_threshold=0.0,
                     subcycle=True)
-    sim = F.Hipclaw46(opt)
+    sim = F.Hipclaw46(opt, stream=_STREAM[0].cuda_stream)
     if leaves is None:
         sim.initialize("humps", humps(args.seed))
     else:
@@ -68,34 +76,96 @@ def run_steps(sim, nsteps, dt, t0=0.0):
     return t
 
 
-def timed(sim, args, dt):
+def timed(sim, args, dt, graph):
+    """Warm-up steps, then args.steps timed coarse steps (graph replays when
+    graph is set), then the same number of eager steps with HIP events around
+    every patch-step launch for the kernel time."""
     import torch
+    from forestclaw_amd import hipclaw as H
+    sim.set_feature(H.FEATURE_GRAPH, graph)
     run_steps(sim, args.warmup, dt)
     c0 = sim.cell_updates()
-    sim.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     t = run_steps(sim, args.steps, dt, t0=args.warmup * dt)
     t1 = time.perf_counter()
+    cells = sim.cell_updates() - c0
+    caps, reps = sim.graph_stats()
+    # kernel timing pass (eager, events on the context's stream)
+    sim.profile(True)
+    run_steps(sim, args.steps, dt, t0=t)
     kms, launches, kcells = sim.profile_read()
     sim.profile(False)
-    cells = sim.cell_updates() - c0
     return dict(seconds=t1 - t0, cells=cells, kernel_ms=kms, launches=launches, kcells=kcells,
-                t_end=t)
+                t_end=t, graph_captures=caps, graph_replays=reps)
+
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libref_swe.so")
+
+
+def _ref_worker(args):
+    """One process: the reference's own Fortran patch step (CLAWPACK46_STEP2_WRAP
+    of src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_step2_wrap.f, built
+    unchanged from the reference sources into oracle/_ref) over its share of
+    the sample until the time budget is spent.  Separate processes, since the
+    Fortran solvers keep their state in common blocks."""
+    import ctypes
+    path, tiles, dx, dy, dt, budget = args
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    ci, cd = ctypes.c_int, ctypes.c_double
+    cd.in_dll(L, "cparam_").value = 1.0
+    mx = my = 32
+    mbc, meqn, mwaves, maux = 2, 3, 3, 0
+    maxm = 32
+    mwork = (maxm + 2 * mbc) * (12 * meqn + (meqn + 1) * mwaves + 3 * maux + 2)
+    work = np.zeros(mwork)
+    size = meqn * (mx + 2 * mbc) * (my + 2 * mbc)
+    fp, fm, gp, gm = (np.zeros(size) for _ in range(4))
+    method = (ci * 7)(0, 2, 2, 0, 0, 0, 0)
+    mthlim = (ci * 3)(4, 4, 4)
+    bcc = (ci * 4)(0, 0, 0, 0)
+    rpn2 = ctypes.cast(L.clawpack46_rpn2_, ctypes.c_void_p)
+    rpt2 = ctypes.cast(L.clawpack46_rpt2_, ctypes.c_void_p)
+    flux2 = ctypes.cast(L.clawpack46_flux2_, ctypes.c_void_p)
+    aux = np.zeros(1)
+    cfl, ierr = cd(0.0), ci(0)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    cells = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget:
+        for k in range(tiles.shape[0]):
+            q = tiles[k].copy()
+            L.clawpack46_set_block_(ctypes.byref(ci(0)))
+            L.clawpack46_step2_wrap_(
+                ctypes.byref(ci(maxm)), ctypes.byref(ci(meqn)), ctypes.byref(ci(maux)),
+                ctypes.byref(ci(mbc)), method, mthlim, ctypes.byref(ci(0)), ctypes.byref(ci(mwaves)),
+                ctypes.byref(ci(mx)), ctypes.byref(ci(my)), vp(q), vp(aux), ctypes.byref(cd(dx)),
+                ctypes.byref(cd(dy)), ctypes.byref(cd(dt)), ctypes.byref(cfl), vp(work),
+                ctypes.byref(ci(mwork)), ctypes.byref(cd(0.0)), ctypes.byref(cd(0.0)),
+                ctypes.byref(ci(0)), ctypes.byref(cd(0.0)), vp(fp), vp(fm), vp(gp), vp(gm), rpn2,
+                rpt2, rpn2, rpt2, flux2, bcc, ctypes.byref(ierr), ctypes.byref(ci(0)))
+            L.clawpack46_unset_block_()
+            cells += mx * my
+    return cells, time.perf_counter() - t0
 
 
 def cpu_baseline(sim64, budget_s):
-    """Time the oracle patch step (OpenMP over patches) on a sample of level-7
-    patches of the same workload."""
+    """CPU rate of the same patch step on the GPU box's host cores, on a sample
+    of level-7 patches of the workload.  With oracle/_ref built: the reference
+    Fortran step in one process per core ("reference"); the oracle port with
+    OpenMP over patches is reported beside it (or alone when the reference
+    library is absent, as "port")."""
     from oracle import oracle as O
     info = sim64.patch_info()
     lev = info["level"]
     q = sim64.get_q()
     idx = np.nonzero(lev == lev.min())[0][:256]
     sample = np.ascontiguousarray(q[idx])
+    del q
     cfg = O.make_cfg(O.SOLVER_SWE_ROE, 32, 32, 2, grav=1.0)
     dx, dy = info["dx"][idx], info["dy"][idx]
     dt = 0.5 * dx.min()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cells = 0
     t0 = time.perf_counter()
     reps = 0
@@ -104,13 +174,29 @@ def cpu_baseline(sim64, budget_s):
         O.step2_batch(cfg, work, None, dx, dy, dt)
         cells += len(idx) * 32 * 32
         reps += 1
-        if time.perf_counter() - t0 > budget_s:
+        if time.perf_counter() - t0 > budget_s / 2:
             break
     el = time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=cells / el, unit="cell-updates/s", cores=threads, kind="port",
+    port = dict(value=cells / el, unit="cell-updates/s", cores=threads, kind="port",
                 sample=f"{len(idx)} level-7 patches (32x32, meqn 3) x {reps} patch-steps each, "
-                       f"oracle step2 FP64, OpenMP over patches, {el:.1f} s")
+                       f"oracle step2 FP64 (C restatement), OpenMP over patches, {el:.1f} s")
+    if not os.path.exists(REF_LIB):
+        return port, None
+    import multiprocessing as mp
+    nproc = threads
+    shares = np.array_split(sample, nproc)
+    args = [(REF_LIB, sh, float(dx[0]), float(dy[0]), dt, budget_s / 2) for sh in shares if len(sh)]
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(len(args)) as pool:
+        res = pool.map(_ref_worker, args)
+    wall = time.perf_counter() - t0
+    rcells = sum(r[0] for r in res)
+    busy = max(r[1] for r in res)
+    ref = dict(value=rcells / busy, unit="cell-updates/s", cores=len(args), kind="reference",
+               sample=f"{len(idx)} level-7 patches (32x32, meqn 3) split over {len(args)} processes, "
+                      f"the reference Fortran CLAWPACK46_STEP2_WRAP (oracle/_ref, amdflang -O2) "
+                      f"repeated for {busy:.1f} s per process ({wall:.1f} s wall with start-up)")
+    return ref, port
 
 
 def main():
@@ -125,6 +211,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain device copies of 1 GiB (float32), the reference "
                          "kernel for scaling hardware byte counters")
@@ -161,11 +248,11 @@ def main():
         tuned[name] = {"cols_per_lane": best[0], "waves_per_group": best[1],
                        "sweep_ms": {f"cpl{c}_wpg{w}": round(v, 4) for (c, w), v in sweep.items()}}
 
-    r32 = timed(sim32, args, dt)
+    r32 = timed(sim32, args, dt, not args.no_graph)
     r64 = None
     err = None
     if not args.no_fp64:
-        r64 = timed(sim64, args, dt)
+        r64 = timed(sim64, args, dt, not args.no_graph)
         a = sim32.get_q()[:, :, 2:-2, 2:-2]
         b = sim64.get_q()[:, :, 2:-2, 2:-2]
         err = dict(rel_l1=float(np.abs(a - b).sum() / np.abs(b).sum()),
@@ -173,9 +260,9 @@ def main():
                    all_finite=bool(np.isfinite(a).all()))
         del a, b
 
-    cpu = None
+    cpu = port = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sim64, args.cpu_seconds)
+        cpu, port = cpu_baseline(sim64, args.cpu_seconds)
 
     w = 4
     T, N, meqn = 36 * 36, 32 * 32, 3
@@ -200,6 +287,7 @@ def main():
                                "refined to level 9 around 8 seeded humps, subcycled",
                    "patches_per_level": levels, "dt_coarse": dt,
                    "cell_updates_per_step": r32["cells"] // args.steps,
+                   "hipgraph": {"captures": r32["graph_captures"], "replays": r32["graph_replays"]},
                    "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
@@ -209,6 +297,8 @@ def main():
                      "kernel_launches": r32["launches"]},
         "cpu_baseline": cpu,
     }
+    if port is not None:
+        res["cpu_port_openmp"] = port
     res["roofline"]["achieved"] = achieved / 1e9
     res["roofline"]["peak"] = HBM_PEAK / 1e9
     if r64 is not None:

@@ -47,10 +47,17 @@ Ctx::Ctx(const hipclaw_physics& p, const hipclaw_domain& d)
 
 Ctx::~Ctx()
 {
+    clear_graphs();
     prof_clear();
     release_pools();
     if (d_tiles) (void)hipFree(d_tiles);
     if (d_logits) (void)hipFree(d_logits);
+}
+
+void Ctx::clear_graphs()
+{
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
 }
 
 void Ctx::prof_clear()
@@ -124,13 +131,17 @@ static int dmalloc(void** p, size_t bytes)
         *p = nullptr;
         return set_error(HIPCLAW_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
     }
-    return check_hip(hipMemsetAsync(*p, 0, bytes, nullptr), "hipMemset");
+    // zero on the null stream and wait: the context's stream may be a
+    // non-blocking one, which the null stream does not order against
+    HC_TRY(check_hip(hipMemsetAsync(*p, 0, bytes, nullptr), "hipMemset"));
+    return check_hip(hipStreamSynchronize(nullptr), "hipMemset");
 }
 
 // Build pools, geometry and ghost plans for the current forest.  With a
 // transfer list the previous pools are repopulated into the new ones.
 int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
 {
+    HC_TRY(check_hip(hipStreamSynchronize(stream), "rebuild"));
     const int nn = forest.nleaves();
     const int mx = phys.mx, my = phys.my;
     void* npool[2] = {nullptr, nullptr};
@@ -265,6 +276,8 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
                          "push table"));
     }
     clear_pushed();
+    clear_graphs();
+    mesh_epoch++;
     // registers start at zero on a new mesh (a regrid happens after a full
     // coarse step, when every synchronised register has been reset)
     regstride = 8LL * phys.meqn * (phys.mx + phys.my);
@@ -400,6 +413,7 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(tmp);
     if (cfl) (void)hipFree(cfl);
+    clear_graphs();
     if (rc) return rc;
     if (best) *best = launch;
     return HIPCLAW_OK;
@@ -410,6 +424,7 @@ int Ctx::set_time_sync(bool on)
     if (on && phys.solver != HIPCLAW_SOLVER_SWE_ROE)
         return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs a conservative flux (SWE only)");
     tsync = on;
+    clear_graphs();
     if (on && !d_reg && n > 0) HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)n * regstride));
     return HIPCLAW_OK;
 }
@@ -506,7 +521,86 @@ int Ctx::timeinterp(int level, double alpha)
 }
 
 // Subcycled coarse step (fclaw2d_advance_all_levels).
+static unsigned level_bits(const int* parity)
+{
+    unsigned b = 0;
+    for (int L = 0; L < MAXL; L++) b |= (parity[L] & 1u) << L;
+    return b;
+}
+static unsigned level_bits(const bool* f)
+{
+    unsigned b = 0;
+    for (int L = 0; L < MAXL; L++) b |= (f[L] ? 1u : 0u) << L;
+    return b;
+}
+
+// Coarse step with the Courant-number reduction; eager or as a hipGraph.
 int Ctx::advance(double t, double dt, double* maxcfl)
+{
+    const bool graph_ok = use_graph && stream != nullptr && !prof &&
+                          !(swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV);
+    if (!graph_ok) {
+        HC_TRY(advance_body(t, dt));
+        if (maxcfl) HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
+    } else {
+        // Key: dt, pool parities, mesh.  The ghost-push flags at entry do not
+        // change the recorded work: the first fine step updates every level,
+        // which sets them, before any ghost fill reads them.
+        const unsigned pin = level_bits(parity), qin = level_bits(pushed);
+        GraphEntry* e = nullptr;
+        for (auto& g : graphs)
+            if (g.dt == dt && g.parity_in == pin && g.epoch == mesh_epoch) e = &g;
+        if (!e) {
+            // capture: the body runs its host-side bookkeeping as usual while the
+            // device work is recorded; restore the bookkeeping if capture fails
+            int par0[MAXL];
+            bool push0[MAXL];
+            std::memcpy(par0, parity, sizeof(par0));
+            std::memcpy(push0, pushed, sizeof(push0));
+            const long long cells0 = cell_updates;
+            HC_TRY(check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture"));
+            int rc = advance_body(t, dt);
+            if (!rc) rc = hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(stream, &g);
+            hipGraphExec_t ex = nullptr;
+            if (!rc && ec == hipSuccess && g) rc = check_hip(hipGraphInstantiateWithFlags(&ex, g, 0), "instantiate");
+            else if (!rc) rc = check_hip(ec, "end capture");
+            if (g) (void)hipGraphDestroy(g);
+            if (rc) {
+                std::memcpy(parity, par0, sizeof(par0));
+                std::memcpy(pushed, push0, sizeof(push0));
+                cell_updates = cells0;
+                return rc;
+            }
+            if (graphs.size() >= 4) {
+                (void)hipGraphExecDestroy(graphs.front().exec);
+                graphs.erase(graphs.begin());
+            }
+            graphs.push_back({dt, pin, qin, mesh_epoch, ex, level_bits(parity), level_bits(pushed),
+                              cell_updates - cells0, cflacc_count});
+            e = &graphs.back();
+            graph_captures++;
+        } else {
+            graph_replays++;
+            for (int L = 0; L < MAXL; L++) {
+                parity[L] = (e->parity_out >> L) & 1u;
+                pushed[L] = ((e->pushed_out >> L) & 1u) != 0;
+            }
+            cell_updates += e->cells;
+            cflacc_count = e->cflacc;
+        }
+        HC_TRY(check_hip(hipGraphLaunch(e->exec, stream), "graph launch"));
+    }
+    if (maxcfl) {
+        HC_TRY(check_hip(hipMemcpyAsync(maxcfl, d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, stream),
+                         "cfl D2H"));
+        HC_TRY(check_hip(hipStreamSynchronize(stream), "advance"));
+    }
+    return HIPCLAW_OK;
+}
+
+int Ctx::advance_body(double t, double dt)
 {
     const int lo = forest.global_minlevel(), hi = forest.global_maxlevel();
     struct Cnt { int last_step, step_inc, total_steps; double t, dt_step; } c[MAXL];
@@ -578,13 +672,6 @@ int Ctx::advance(double t, double dt, double* maxcfl)
     }
     HC_TRY(ghost_update(lo, hi, 0));
     HC_TRY(time_sync(lo, hi));
-    // maximum over the level CFLs of this coarse step
-    if (maxcfl) {
-        HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
-        HC_TRY(check_hip(hipMemcpyAsync(maxcfl, d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, stream),
-                         "cfl D2H"));
-        HC_TRY(check_hip(hipStreamSynchronize(stream), "advance"));
-    }
     return HIPCLAW_OK;
 }
 
@@ -631,6 +718,8 @@ int Ctx::tag_and_adapt(int init_flag, int* changed)
         launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
                           tag_ivar, refine_threshold, 0, d_tags, stream);
     std::vector<int> rtag(nn), ctag(nn);
+    // synchronous copies run on the null stream: wait for this context's stream
+    HC_TRY(check_hip(hipStreamSynchronize(stream), "tags"));
     HC_TRY(check_hip(hipMemcpy(rtag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
     if (cnn) {
         // learned refinement tags; the plain tag is the fallback and the judge
@@ -675,6 +764,7 @@ int Ctx::tag_and_adapt(int init_flag, int* changed)
         else
             launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
                               tag_ivar, coarsen_threshold, 1, d_tags, stream);
+        HC_TRY(check_hip(hipStreamSynchronize(stream), "tags"));
         HC_TRY(check_hip(hipMemcpy(ctag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
     }
     std::vector<int> tags(nn, 0);
@@ -707,6 +797,7 @@ int Ctx::mass(double* sums)
     else
         launch_mass<float>(d_iota, d_levels, n, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn, d_mass, stream);
     std::vector<double> h((size_t)n * phys.meqn);
+    HC_TRY(check_hip(hipStreamSynchronize(stream), "mass"));
     HC_TRY(check_hip(hipMemcpy(h.data(), d_mass, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "mass"));
     for (int m = 0; m < phys.meqn; m++) {
         double s = 0.0, c = 0.0;  // compensated sum over patches in leaf order
@@ -809,12 +900,14 @@ int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch)
     CTX_CHECK(ctx);
     if (!launch) {
         ctx->c->launch = hipclaw_launch{0, 0};
+        ctx->c->clear_graphs();
         return HIPCLAW_OK;
     }
     if (launch->cols_per_lane < 1 || launch->cols_per_lane > 2 || launch->waves_per_group < 1 ||
         launch->waves_per_group > 4)
         return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
     ctx->c->launch = *launch;
+    ctx->c->clear_graphs();
     return HIPCLAW_OK;
 }
 
@@ -846,14 +939,24 @@ int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on)
 {
     CTX_CHECK(ctx);
     Ctx& c = *ctx->c;
+    c.clear_graphs();
     switch (feature) {
     case HIPCLAW_FEATURE_MERGE_LEVELS: c.merge_levels = on != 0; break;
     case HIPCLAW_FEATURE_GHOST_PUSH:
         c.push_ghosts = on != 0;
         c.clear_pushed();
         break;
+    case HIPCLAW_FEATURE_GRAPH: c.use_graph = on != 0; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown feature");
     }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_graph_stats(hipclaw_ctx* ctx, long long* captures, long long* replays)
+{
+    CTX_CHECK(ctx);
+    if (captures) *captures = ctx->c->graph_captures;
+    if (replays) *replays = ctx->c->graph_replays;
     return HIPCLAW_OK;
 }
 
@@ -923,6 +1026,7 @@ int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* x
 static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
 {
     if (to_device) c.clear_pushed();
+    HC_TRY(check_hip(hipStreamSynchronize(c.stream), "q copy"));
     const long long per = auxdata ? c.atile : c.tile;
     const size_t tot = (size_t)c.n * per;
     if (tot == 0) return HIPCLAW_OK;
@@ -958,6 +1062,7 @@ static int copy_q(Ctx& c, double* host, bool to_device, bool auxdata)
 static int level_copy(Ctx& c, int level, int n, double* const* q, const double* const* aux, bool up)
 {
     if (up) c.clear_pushed();
+    HC_TRY(check_hip(hipStreamSynchronize(c.stream), "level copy"));
     if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
     const int s0 = c.plan[level].s0, np = c.plan[level].s1 - s0;
     if (n != np) return set_error(HIPCLAW_ERR_ARG, "n differs from the number of patches of the level");

@@ -106,6 +106,23 @@ struct Ctx {
     }
     int timeinterp(int level, double alpha);
     int advance(double t, double dt, double* maxcfl);
+    int advance_body(double t, double dt);
+    // hipGraph of a whole coarse step, replayed when dt, the level pool
+    // parities, the ghost-push state, the launch shape and the mesh match
+    bool use_graph = false;
+    long long mesh_epoch = 0;
+    struct GraphEntry {
+        double dt;
+        unsigned parity_in, pushed_in;
+        long long epoch;
+        hipGraphExec_t exec;
+        unsigned parity_out, pushed_out;
+        long long cells;
+        int cflacc;
+    };
+    std::vector<GraphEntry> graphs;
+    long long graph_captures = 0, graph_replays = 0;
+    void clear_graphs();
     int qinit(int kind, const QinitParams& prm);
     int tag_and_adapt(int init_flag, int* changed);
     int mass(double* sums);

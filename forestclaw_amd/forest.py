@@ -144,6 +144,12 @@ class Hipclaw46:
         """H.FEATURE_MERGE_LEVELS / H.FEATURE_GHOST_PUSH (both on by default)."""
         _check(_lib().hipclaw_set_feature(self.h, feature, int(bool(on))))
 
+    def graph_stats(self):
+        """(coarse steps captured as graphs, graph replays)."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib().hipclaw_graph_stats(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def time_sync(self, minlevel: int, maxlevel: int):
         _check(_lib().hipclaw_time_sync(self.h, minlevel, maxlevel))
 

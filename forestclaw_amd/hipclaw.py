@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG, "libhipclaw.so")
 
 SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
 FP64, FP32 = 0, 1
-FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH = 1, 2
+FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH = 1, 2, 3
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "unsupported",
           4: "out of device memory", 5: "call out of sequence"}
 
@@ -108,6 +108,8 @@ _SIGS.update({
     "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
     "hipclaw_set_time_sync": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_feature": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "hipclaw_graph_stats": ([_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
+                            ctypes.c_int),
     "hipclaw_time_sync": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_get_registers": ([_VP, _DP], ctypes.c_int),
     "hipclaw_autotune": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],

@@ -214,10 +214,17 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
  *   MERGE_LEVELS  one patch-step launch for all levels due at a subcycled time
  *   GHOST_PUSH    the step writes updated edge cells into the ghost cells of
  *                 same-level neighbours, replacing the same-level copies of the
- *                 next ghost fill (not used while time sync is on) */
+ *                 next ghost fill (not used while time sync is on)
+ *   GRAPH         (off by default) each coarse step is captured once as a
+ *                 hipGraph and replayed while dt, the pool parities and the mesh
+ *                 are unchanged; needs a non-default stream, not used while
+ *                 profiling or with time-dependent swirl velocities */
 #define HIPCLAW_FEATURE_MERGE_LEVELS 1
 #define HIPCLAW_FEATURE_GHOST_PUSH 2
+#define HIPCLAW_FEATURE_GRAPH 3
 int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on);
+/* Coarse steps captured as graphs and replays of captured graphs so far. */
+int hipclaw_graph_stats(hipclaw_ctx* ctx, long long* captures, long long* replays);
 /* Patch-step launch shape used by this context (NULL = library default). */
 int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch);
 /* Autotuner: times every launch shape (cols_per_lane 1..2 x waves_per_group
