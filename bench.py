@@ -11,8 +11,9 @@ at its subcycled times, time interpolation and all ghost updates.  Regrid is
 not part of a step.  The headline value is the FP32 path; the FP64 path runs
 the same steps from the same data and serves as the FP64 oracle (it is
 bitwise equal to the CPU oracle in the parity tests), which gives the FP32
-error.  The CPU baseline times the oracle's patch step (the restated
-Fortran step2) on a bounded sample of the same patches.
+error.  The CPU baseline times the reference's own Fortran patch step
+(built into oracle/_ref) on a bounded sample of the same patches, one
+process per host core, with the oracle's C restatement under OpenMP beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -31,6 +32,17 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s on 1 MI355X, FP32-batched vs FP64 plain, at <=1e-5 rel L1"
 HBM_PEAK = 8.0e12
+
+# HBM bytes per step launch (average over the 4 launches of a coarse step,
+# ghost push on) of the default workload, whose mesh is fixed by the seed:
+# 2 x FETCH_SIZE + 1 x WRITE_SIZE from two rocprofv3 --pmc runs, the factors
+# being what the same counters read for a 1 GiB device copy
+# (profiles/k_step2_swe_humps_l7-9_hbm_bytes.md).  VALU_ACTIVE is
+# SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the level-9 launch
+# (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
+MEASURED_MESH = {7: 10481, 8: 2357, 9: 85020}
+TRAFFIC_PER_LAUNCH = 3.517e9
+VALU_ACTIVE = 0.85
 
 
 def humps(seed=7, n=8):
@@ -83,6 +95,8 @@ def timed(sim, args, dt, graph):
     import torch
     from forestclaw_amd import hipclaw as H
     sim.set_feature(H.FEATURE_GRAPH, graph)
+    sim.set_feature(H.FEATURE_GHOST_PUSH, not args.no_push)
+    sim.set_feature(H.FEATURE_MERGE_LEVELS, not args.no_merge)
     run_steps(sim, args.warmup, dt)
     c0 = sim.cell_updates()
     torch.cuda.synchronize()
@@ -199,6 +213,39 @@ def cpu_baseline(sim64, budget_s):
     return ref, port
 
 
+def host_path(sim64, n, dt):
+    """PCIe-inclusive rate of the host-resident drop-in path: the deferred
+    hipclaw_single_step_update protocol over n host tiles of the finest level
+    (FP64 arrays, as ForestClaw stores them; one Python call per patch, a
+    batch of up to 4096 patches per flush), FP32 and FP64 device modes."""
+    import torch
+    from forestclaw_amd import hipclaw as H
+    info = sim64.patch_info()
+    lev = info["level"]
+    idx = np.nonzero(lev == lev.max())[0][:n]
+    q = sim64.get_q()
+    tiles = [np.ascontiguousarray(q[i]) for i in idx]
+    del q
+    dx, dy = float(info["dx"][idx[0]]), float(info["dy"][idx[0]])
+    dtf = dt / (1 << int(lev.max() - lev.min()))
+    out = {}
+    for name, prec in (("fp32", H.FP32), ("fp64", H.FP64)):
+        b = H.Batch(H.physics(H.SOLVER_SWE_ROE, 32, 32, precision=prec, grav=1.0),
+                    min(len(tiles), 4096), _STREAM[0].cuda_stream)
+        rates = []
+        for rep in range(3):  # the first pass warms up
+            t0 = time.perf_counter()
+            for it, tq in enumerate(tiles):
+                b.single_step_update(tq, None, dx, dy, rep * dtf, dtf, it, len(tiles))
+            torch.cuda.synchronize()
+            rates.append(len(tiles) * 32 * 32 / (time.perf_counter() - t0))
+        b.close()
+        out[name] = max(rates[1:])
+    return {"value_fp32": out["fp32"], "value_fp64": out["fp64"], "unit": "cell-updates/s",
+            "sample": f"{len(tiles)} level-{int(lev.max())} host tiles (36x36x3 float64) per pass, "
+                      "hipclaw_single_step_update with batches of up to 4096, best of 2 passes"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +262,13 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain device copies of 1 GiB (float32), the reference "
                          "kernel for scaling hardware byte counters")
+    ap.add_argument("--no-push", action="store_true",
+                    help="same-level ghost copies by the ghost kernel instead of the step kernel")
+    ap.add_argument("--no-merge", action="store_true",
+                    help="one step launch per level instead of one per subcycled time")
+    ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
+                    help="also time the host-resident drop-in path (hipclaw_single_step_update "
+                         "on NPATCH host tiles, PCIe included)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU path is "
@@ -263,6 +317,7 @@ def main():
     cpu = port = None
     if not args.no_cpu:
         cpu, port = cpu_baseline(sim64, args.cpu_seconds)
+    hp = host_path(sim64, args.host_path, dt) if args.host_path > 0 else None
 
     w = 4
     T, N, meqn = 36 * 36, 32 * 32, 3
@@ -290,7 +345,11 @@ def main():
                    "hipgraph": {"captures": r32["graph_captures"], "replays": r32["graph_replays"]},
                    "launch": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": TRAFFIC_PER_LAUNCH if levels == MEASURED_MESH else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+                     "algorithmic_per_launch": b_upd * r32["kcells"] / max(r32["launches"], 1),
+                     "valu_active": VALU_ACTIVE if levels == MEASURED_MESH else None,
                      "kernel": "k_step2<float,Swe>",
                      "bytes_per_cell_update": b_upd,
                      "kernel_only_rate": r32["kcells"] / k_time if k_time > 0 else None,
@@ -299,6 +358,10 @@ def main():
     }
     if port is not None:
         res["cpu_port_openmp"] = port
+    if hp is not None:
+        res["host_path_pcie"] = hp
+    if args.no_push or args.no_merge:
+        res["config"]["features"] = {"ghost_push": not args.no_push, "merge_levels": not args.no_merge}
     res["roofline"]["achieved"] = achieved / 1e9
     res["roofline"]["peak"] = HBM_PEAK / 1e9
     if r64 is not None:

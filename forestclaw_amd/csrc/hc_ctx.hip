@@ -386,6 +386,24 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     // a small step (Courant number ~0.1 for unit wave speeds) keeps the data valid
     const double dx = (dom.bx - dom.ax) / dom.mi / (double)(1 << level) / phys.mx;
     const double dt = 0.1 * dx;
+    // Time the kernel variant the level updates launch: with the ghost push
+    // on, the slot-addressed push kernel, whose output pool is the scratch
+    // tiles (same-level neighbours are slots of the same level, so every
+    // push lands in the scratch too).
+    const bool push = push_ghosts && merge_levels && !tsync && phys.mx >= 4 && phys.my >= 4;
+    MultiLevel ml{};
+    ml.levels = d_levels;
+    ml.slot0 = s0;
+    ml.dtl[level] = dt;
+    ml.qin_l[level] = pool[parity[level]];
+    ml.qout_l[level] = static_cast<char*>(tmp) - (ptrdiff_t)s0 * tile * esz;
+    ml.push = d_push;
+    auto run = [&](const hipclaw_launch& l) {
+        if (push)
+            return step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dt, cfl - s0, nullptr,
+                                &l, stream, &ml);
+        return hipclaw_step2_batch(&phys, np, qin, tmp, a, d_dxdy + 2 * s0, nullptr, np, dt, cfl, &l, stream);
+    };
     float bestms = 1e30f;
     int k = 0;
     for (int cpl = 1; cpl <= 2 && !rc; cpl++)
@@ -393,10 +411,9 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
             if (rc) break;
             hipclaw_launch l{cpl, wpg};
             // one untimed launch, then reps timed ones
-            rc = hipclaw_step2_batch(&phys, np, qin, tmp, a, d_dxdy + 2 * s0, nullptr, np, dt, cfl, &l, stream);
+            rc = run(l);
             if (!rc) rc = check_hip(hipEventRecord(e0, stream), "event");
-            for (int r = 0; r < reps && !rc; r++)
-                rc = hipclaw_step2_batch(&phys, np, qin, tmp, a, d_dxdy + 2 * s0, nullptr, np, dt, cfl, &l, stream);
+            for (int r = 0; r < reps && !rc; r++) rc = run(l);
             if (!rc) rc = check_hip(hipEventRecord(e1, stream), "event");
             if (!rc) rc = check_hip(hipEventSynchronize(e1), "event");
             float t = 0;

@@ -24,6 +24,8 @@
 //
 // mbc must be 2 (ForestClaw's default); the host rejects other values.
 #pragma once
+#include <type_traits>
+
 #include "hc_riemann.hpp"
 
 namespace hc {
@@ -105,8 +107,27 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
 // the ghost cells of the same-level neighbour across that face or corner
 // (what fclaw2d_clawpatch46_fort_copy_face / _copy_corner would copy from it,
 // src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_copy.f:17-133).
-template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH>
-__global__ void __launch_bounds__(256)
+// Row-loop shape.  UNR rows per loop iteration (4: the unrolled, renamed
+// windows; 1: rotate after every row); the time-sync and the FP64
+// two-column kernels keep 1, whose unrolled bodies would not fit their
+// register budget.  WPE is the occupancy floor (waves per SIMD) the kernel is
+// register-allocated for: the FP32 kernels and the FP64 one-column kernels
+// keep two waves (at most 256 VGPRs).  HC_STEP_UNROLL / HC_STEP_MIN_WAVES
+// override the defaults.
+#ifndef HC_STEP_UNROLL
+#define HC_STEP_UNROLL 4
+#endif
+#ifndef HC_STEP_MIN_WAVES
+#define HC_STEP_MIN_WAVES 2
+#endif
+template <class R, int CPL, bool SYNC>
+constexpr int STEP_UNR = (SYNC || (sizeof(R) == 8 && CPL == 2)) ? 1 : HC_STEP_UNROLL;
+template <class R, int CPL, bool SYNC>
+constexpr int STEP_WPE = (!SYNC && (sizeof(R) == 4 || CPL == 1)) ? HC_STEP_MIN_WAVES : 1;
+
+template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH,
+          int UNR = STEP_UNR<R, CPL, SYNC>>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEP_WPE<R, CPL, SYNC>)))
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
 {
@@ -167,43 +188,89 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
 
     // ---- carried state -------------------------------------------------
-    R qm1[CPL][ME], qm2[CPL][ME];
-    Face fyp[CPL] = {};
-    Trans typ[CPL] = {};
-    R dotly[CPL][MW];
-    R hmp_prev[CPL][ME], hpp_prev[CPL][ME], fdp_prev[CPL][ME];
-    R fmx1[CPL][ME], fpx1[CPL][ME], fmx2[CPL][ME], fpx2[CPL][ME];
-    R G1[CPL][ME], G2[CPL][ME], d2p[CPL][ME];
-    R um1[CPL], um2[CPL];  // advection: u(c, J-1), u(c, J-2)
+    // The sliding windows are rings indexed by the row number: a value made
+    // at row J lives in slot J mod 4 (windows three rows deep: q, u, the
+    // x-fluxes and G) or J mod 2 (two rows deep: the y-face, its transverse
+    // splits and limiter dot products, d2).  With UNR = 4 the row loop is
+    // unrolled four times and every slot index is a compile-time constant,
+    // so the windows never move between registers; UNR = 1 keeps one row per
+    // iteration and rotates the rings after it.
+    R qr[4][CPL][ME], ur[4][CPL];
+    R fmr[4][CPL][ME], fpr[4][CPL][ME], Gr[4][CPL][ME];
+    Face fyr[2][CPL] = {};
+    Trans tyr[2][CPL] = {};
+    R dotr2[2][CPL][MW];
+    R hmpr[2][CPL][ME], hppr[2][CPL][ME], fdpr[2][CPL][ME], d2r[2][CPL][ME];
     R cfl[CPL];
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
         cfl[k] = R(0);
-        um1[k] = um2[k] = R(0);
 #pragma unroll
-        for (int m = 0; m < ME; m++) {
-            qm1[k][m] = qm2[k][m] = R(0);
-            hmp_prev[k][m] = hpp_prev[k][m] = fdp_prev[k][m] = R(0);
-            fmx1[k][m] = fpx1[k][m] = fmx2[k][m] = fpx2[k][m] = R(0);
-            G1[k][m] = G2[k][m] = d2p[k][m] = R(0);
+        for (int s = 0; s < 4; s++) {
+            ur[s][k] = R(0);
+#pragma unroll
+            for (int m = 0; m < ME; m++) qr[s][k][m] = fmr[s][k][m] = fpr[s][k][m] = Gr[s][k][m] = R(0);
         }
 #pragma unroll
-        for (int w = 0; w < MW; w++) dotly[k][w] = R(0);
+        for (int s = 0; s < 2; s++) {
+#pragma unroll
+            for (int m = 0; m < ME; m++) hmpr[s][k][m] = hppr[s][k][m] = fdpr[s][k][m] = d2r[s][k][m] = R(0);
+#pragma unroll
+            for (int w = 0; w < MW; w++) dotr2[s][k][w] = R(0);
+        }
     }
 
-    for (int J = 0; J < NY; J++) {
-        // ---- load row J ------------------------------------------------
-        R qc[CPL][ME];
-        R ucur[CPL], vcur[CPL], vnext[CPL];
+    auto row = [&](auto phase, const int J) __attribute__((always_inline)) {
+        constexpr int P = decltype(phase)::value;
+        constexpr int S0 = P & 3, S1 = (P + 3) & 3, S2 = (P + 2) & 3;  // rows J, J-1, J-2
+        constexpr int T0 = P & 1, T1 = (P + 1) & 1;                     // rows J, J-1
+        R(&qc)[CPL][ME] = qr[S0];
+        R(&qm1)[CPL][ME] = qr[S1];
+        R(&qm2)[CPL][ME] = qr[S2];
+        R(&ucur)[CPL] = ur[S0];
+        R(&um1)[CPL] = ur[S1];  // advection: u(c, J-1), u(c, J-2)
+        R(&um2)[CPL] = ur[S2];
+        R(&fmxn_w)[CPL][ME] = fmr[S0];
+        R(&fmx1)[CPL][ME] = fmr[S1];
+        R(&fmx2)[CPL][ME] = fmr[S2];
+        R(&fpxn_w)[CPL][ME] = fpr[S0];
+        R(&fpx1)[CPL][ME] = fpr[S1];
+        R(&fpx2)[CPL][ME] = fpr[S2];
+        R(&Gn_w)[CPL][ME] = Gr[S0];
+        R(&G1)[CPL][ME] = Gr[S1];
+        R(&G2)[CPL][ME] = Gr[S2];
+        R(&d2n_w)[CPL][ME] = d2r[T0];
+        R(&d2p)[CPL][ME] = d2r[T1];
+        R(&hmp_new_w)[CPL][ME] = hmpr[T0];
+        R(&hmp_prev)[CPL][ME] = hmpr[T1];
+        R(&hpp_new_w)[CPL][ME] = hppr[T0];
+        R(&hpp_prev)[CPL][ME] = hppr[T1];
+        R(&fdp_w)[CPL][ME] = fdpr[T0];
+        R(&fdp_prev)[CPL][ME] = fdpr[T1];
+        R(&dotry_w)[CPL][MW] = dotr2[T0];
+        R(&dotly)[CPL][MW] = dotr2[T1];
+        Face(&fyn_w)[CPL] = fyr[T0];
+        Face(&fyp)[CPL] = fyr[T1];
+        Trans(&tyn_w)[CPL] = tyr[T0];
+        Trans(&typ)[CPL] = tyr[T1];
+
+        // ---- load row J (rows past the tile, run by the unrolled loop when
+        // NY is not a multiple of 4, load zeros and store nothing) ----------
+        const bool rowin = J < NY;
+        // row offsets through an opaque copy of J: addresses are formed per
+        // row from the per-lane tile base instead of one pointer induction
+        // variable per unrolled row and field kept live across the loop
+        const int Jo = __builtin_amdgcn_readfirstlane(J);
+        R vcur[CPL], vnext[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
 #pragma unroll
             for (int m = 0; m < ME; m++)
-                qc[k][m] = cv[k] ? qp[((long long)m * NY + J) * NX + col[k]] : R(0);
+                qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
             if constexpr (S::MAUX > 0) {
-                ucur[k] = cv[k] ? ap[(long long)J * NX + col[k]] : R(0);
-                vcur[k] = cv[k] ? ap[((long long)NY + J) * NX + col[k]] : R(0);
-                vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + J + 1) * NX + col[k]] : R(0);
+                ucur[k] = (cv[k] && rowin) ? ap[(long long)Jo * NX + col[k]] : R(0);
+                vcur[k] = (cv[k] && rowin) ? ap[((long long)NY + Jo) * NX + col[k]] : R(0);
+                vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + Jo + 1) * NX + col[k]] : R(0);
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
             }
@@ -554,17 +621,17 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R gmu = G1[k][m] + fdm[k][m];
                         R gpc = G2[k][m] + fdp_prev[k][m];
                         R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
-                        qo[((long long)m * NY + Rw) * NX + col[k]] = qn;
+                        qo[((long long)m * NY + (Jo - 2)) * NX + col[k]] = qn;
                         if constexpr (PUSH) {
                             const int c = col[k];
                             if (c <= 3)
-                                stage[((c - 2) * ME + m) * A.my + (Rw - 2)] = qn;
+                                stage[((c - 2) * ME + m) * A.my + (Jo - 4)] = qn;
                             else if (c >= NX - 4)
-                                stage[((2 + c - (NX - 4)) * ME + m) * A.my + (Rw - 2)] = qn;
+                                stage[((2 + c - (NX - 4)) * ME + m) * A.my + (Jo - 4)] = qn;
                             if (Rw <= 3)
-                                stage[4 * ME * A.my + ((Rw - 2) * ME + m) * A.mx + (c - 2)] = qn;
+                                stage[4 * ME * A.my + ((Jo - 4) * ME + m) * A.mx + (c - 2)] = qn;
                             else if (Rw >= NY - 4)
-                                stage[4 * ME * A.my + ((2 + Rw - (NY - 4)) * ME + m) * A.mx + (c - 2)] = qn;
+                                stage[4 * ME * A.my + ((Jo - NY + 4) * ME + m) * A.mx + (c - 2)] = qn;
                         }
                     }
                     if constexpr (SYNC) {
@@ -617,30 +684,63 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             }
         }
 
-        // ---- rotate the sliding windows -----------------------------------
+
+        // ---- this row's values enter the rings (renames when unrolled) ----
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
 #pragma unroll
             for (int m = 0; m < ME; m++) {
-                qm2[k][m] = qm1[k][m];
-                qm1[k][m] = qc[k][m];
-                fmx2[k][m] = fmx1[k][m];
-                fpx2[k][m] = fpx1[k][m];
-                fmx1[k][m] = fmxn[k][m];
-                fpx1[k][m] = fpxn[k][m];
-                G2[k][m] = G1[k][m];
-                G1[k][m] = Gn[k][m];
-                d2p[k][m] = d2n[k][m];
-                hmp_prev[k][m] = hmp_new[k][m];
-                hpp_prev[k][m] = hpp_new[k][m];
-                fdp_prev[k][m] = fdp[k][m];
+                fmxn_w[k][m] = fmxn[k][m];
+                fpxn_w[k][m] = fpxn[k][m];
+                Gn_w[k][m] = Gn[k][m];
+                d2n_w[k][m] = d2n[k][m];
+                hmp_new_w[k][m] = hmp_new[k][m];
+                hpp_new_w[k][m] = hpp_new[k][m];
+                fdp_w[k][m] = fdp[k][m];
             }
 #pragma unroll
-            for (int w = 0; w < MW; w++) dotly[k][w] = dotry[k][w];
-            fyp[k] = fyn[k];
-            typ[k] = tyn[k];
-            um2[k] = um1[k];
-            um1[k] = ucur[k];
+            for (int w = 0; w < MW; w++) dotry_w[k][w] = dotry[k][w];
+            fyn_w[k] = fyn[k];
+            tyn_w[k] = tyn[k];
+        }
+    };
+
+    if constexpr (UNR == 4) {
+        for (int J = 0; J < NY; J += 4) {
+            row(std::integral_constant<int, 0>{}, J);
+            row(std::integral_constant<int, 1>{}, J + 1);
+            row(std::integral_constant<int, 2>{}, J + 2);
+            row(std::integral_constant<int, 3>{}, J + 3);
+        }
+    } else {
+        for (int J = 0; J < NY; J++) {
+            row(std::integral_constant<int, 0>{}, J);
+            // rotate the rings: slot 0 holds row J, slots 3 / 2 / 1 are read
+            // as rows J-1 / J-2 / J-1 by the next row
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                ur[2][k] = ur[3][k];
+                ur[3][k] = ur[0][k];
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    qr[2][k][m] = qr[3][k][m];
+                    qr[3][k][m] = qr[0][k][m];
+                    fmr[2][k][m] = fmr[3][k][m];
+                    fmr[3][k][m] = fmr[0][k][m];
+                    fpr[2][k][m] = fpr[3][k][m];
+                    fpr[3][k][m] = fpr[0][k][m];
+                    Gr[2][k][m] = Gr[3][k][m];
+                    Gr[3][k][m] = Gr[0][k][m];
+                    d2r[1][k][m] = d2r[0][k][m];
+                    hmpr[1][k][m] = hmpr[0][k][m];
+                    hppr[1][k][m] = hppr[0][k][m];
+                    fdpr[1][k][m] = fdpr[0][k][m];
+                }
+#pragma unroll
+                for (int w = 0; w < MW; w++) dotr2[1][k][w] = dotr2[0][k][w];
+                fyr[1][k] = fyr[0][k];
+                tyr[1][k] = tyr[0][k];
+            }
         }
     }
 
@@ -661,22 +761,39 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 ob[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
             };
             const int nx_items = 4 * ME * A.my;
-            for (int e = lane; e < nx_items; e += HC_WAVE) {
-                // lanes pair up on the two edge columns of one row (8 contiguous bytes)
-                const int cb = e & 1, u = e >> 1;
-                const int r = u % A.my, t = u / A.my, m = t % ME, side = t / ME;
-                const int c = side ? NX - 4 + cb : 2 + cb, Rw = r + 2;
-                const int xo = side ? -A.mx : A.mx;
-                const R v = sg[((side * 2 + cb) * ME + m) * A.my + r];
-                if (tb[side] >= 0) put(tb[side], c + xo, Rw, m, v);
-                if (Rw <= 3 && tb[4 + side] >= 0) put(tb[4 + side], c + xo, Rw + A.my, m, v);
-                if (Rw >= NY - 4 && tb[6 + side] >= 0) put(tb[6 + side], c + xo, Rw - A.my, m, v);
+            // x edges: a lane takes one row of a 32-row block and one of the
+            // two edge columns (lane pairs store 8 contiguous bytes)
+            const int cb = lane & 1, rl = lane >> 1;
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                const int nb = tb[side], nbl = tb[4 + side], nbh = tb[6 + side];
+                const int c = (side ? NX - 4 - A.mx : 2 + A.mx) + cb;
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    const R* sx = sg + ((side * 2 + cb) * ME + m) * A.my;
+                    for (int r = rl; r < A.my; r += HC_WAVE / 2) {
+                        const int Rw = r + 2;
+                        const R v = sx[r];
+                        if (nb >= 0) put(nb, c, Rw, m, v);
+                        if (Rw <= 3 && nbl >= 0) put(nbl, c, Rw + A.my, m, v);
+                        if (Rw >= NY - 4 && nbh >= 0) put(nbh, c, Rw - A.my, m, v);
+                    }
+                }
             }
-            const int ny_items = 4 * ME * A.mx;
-            for (int e = lane; e < ny_items; e += HC_WAVE) {
-                const int i = e % A.mx, t = e / A.mx, m = t % ME, sr = t / ME;  // sr = side*2 + row
-                const int side = sr >> 1, Rw = side ? NY - 4 + (sr & 1) : 2 + (sr & 1);
-                if (tb[2 + side] >= 0) put(tb[2 + side], i + 2, Rw + (side ? -A.my : A.my), m, sg[nx_items + e]);
+            // y edges: a lane takes one column of a ghost row
+#pragma unroll
+            for (int side = 0; side < 2; side++) {
+                const int nb = tb[2 + side];
+                if (nb < 0) continue;  // uniform across the wave
+#pragma unroll
+                for (int rr = 0; rr < 2; rr++) {
+                    const int Rt = side ? NY - 4 + rr - A.my : 2 + rr + A.my;
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        const R* sy = sg + nx_items + ((side * 2 + rr) * ME + m) * A.mx;
+                        for (int i = lane; i < A.mx; i += HC_WAVE) put(nb, i + 2, Rt, m, sy[i]);
+                    }
+                }
             }
         }
     }
