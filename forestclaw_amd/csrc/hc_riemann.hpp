@@ -48,9 +48,12 @@ template <class R> struct Adv {
     // apdq split (cell on the high side): {down, up}
     struct Trans { R m_dn, m_up, p_dn, p_up; };
 
+    static constexpr int NC = 0;  // no per-cell data
+    __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+
     template <int DIR>
-    __device__ static inline void rpn2(const R* qa, const R* qb, R sface, const PhysParams&,
-                                       Face& f, Trans&)
+    __device__ static inline void rpn2(const R* qa, const R* qb, const R*, const R*, R sface,
+                                       const PhysParams&, Face& f, Trans&)
     {
         R delta = qb[0] - qa[0];
         f.w[0][0] = delta;
@@ -95,20 +98,48 @@ template <class R> struct Swe {
         f[mv] = un * hut;
     }
 
+    // Per-cell data: sqrt(h), hu/sqrt(h), hv/sqrt(h), hu/h, hv/h, sqrt(g h),
+    // the operations the reference evaluates per face side for the Roe
+    // averages (clawpack46_rpn2.f:91-99) and the entropy-fix speeds
+    // (:170-171, :207-208).  The FP64 kernels compute them once per cell and
+    // share them between the four faces the cell touches (the same
+    // operations, so no bit changes); the FP32 kernels (NC = 0) evaluate them
+    // per face, which keeps their register footprint within two waves per SIMD.
+    static constexpr int NC = sizeof(R) == 8 ? 6 : 0;
+    __device__ static inline void cell(const R* q, const PhysParams& pp, R* cd)
+    {
+        using MM = M<R>;
+        const R g = R(pp.grav);
+        const R sh = MM::sqrt_(q[0]);
+        cd[0] = sh;
+        cd[1] = MM::divp(q[1], sh);
+        cd[2] = MM::divp(q[2], sh);
+        cd[3] = MM::divp(q[1], q[0]);
+        cd[4] = MM::divp(q[2], q[0]);
+        cd[5] = MM::sqrt_(g * q[0]);
+    }
+
     template <int DIR>
-    __device__ static inline void rpn2(const R* qa, const R* qb, R, const PhysParams& pp,
-                                       Face& f, Trans& roe)
+    __device__ static inline void rpn2(const R* qa, const R* qb, const R* ca, const R* cb, R,
+                                       const PhysParams& pp, Face& f, Trans& roe)
     {
         constexpr int mu = DIR == 0 ? 1 : 2;
         constexpr int mv = DIR == 0 ? 2 : 1;
         using MM = M<R>;
         const R g = R(pp.grav);
         R h = (qa[0] + qb[0]) / R(2);
-        R hsl = MM::sqrt_(qa[0]);
-        R hsr = MM::sqrt_(qb[0]);
+        R hsl, hsr, wua, wub, wva, wvb;
+        if constexpr (NC > 0) {
+            hsl = ca[0], hsr = cb[0];
+            wua = ca[mu], wub = cb[mu], wva = ca[mv], wvb = cb[mv];
+        } else {
+            hsl = MM::sqrt_(qa[0]), hsr = MM::sqrt_(qb[0]);
+            wua = MM::divp(qa[mu], hsl), wub = MM::divp(qb[mu], hsr);
+            wva = MM::divp(qa[mv], hsl), wvb = MM::divp(qb[mv], hsr);
+        }
         R hsq2 = hsl + hsr;
-        R u = MM::divp(MM::divp(qa[mu], hsl) + MM::divp(qb[mu], hsr), hsq2);
-        R v = MM::divp(MM::divp(qa[mv], hsl) + MM::divp(qb[mv], hsr), hsq2);
+        R u = MM::divp(wua + wub, hsq2);
+        R v = MM::divp(wva + wvb, hsq2);
         R a = MM::sqrt_(g * h);
         roe.u = u; roe.v = v; roe.a = a;
 
@@ -140,7 +171,9 @@ template <class R> struct Swe {
 
         // entropy fix: amdq from the left-going parts
         do {
-            R s0 = MM::divp(qa[mu], qa[0]) - MM::sqrt_(g * qa[0]);
+            R s0;
+            if constexpr (NC > 0) s0 = ca[2 + mu] - ca[5];
+            else s0 = MM::divp(qa[mu], qa[0]) - MM::sqrt_(g * qa[0]);
             if (s0 > R(0) && f.s[0] > R(0)) {
                 f.am[0] = R(0); f.am[1] = R(0); f.am[2] = R(0);
                 break;
@@ -158,7 +191,9 @@ template <class R> struct Swe {
             for (int m = 0; m < 3; m++) f.am[m] = sfract * f.w[0][m];
             if (f.s[1] > R(0)) break;
             for (int m = 0; m < 3; m++) f.am[m] = f.am[m] + f.s[1] * f.w[1][m];
-            R s03 = MM::divp(qb[mu], qb[0]) + MM::sqrt_(g * qb[0]);
+            R s03;
+            if constexpr (NC > 0) s03 = cb[2 + mu] + cb[5];
+            else s03 = MM::divp(qb[mu], qb[0]) + MM::sqrt_(g * qb[0]);
             R h3 = qb[0] - f.w[2][0];
             R hu3 = qb[mu] - f.w[2][mu];
             R s3 = MM::divp(hu3, h3) + MM::sqrt_(g * h3);
@@ -212,9 +247,12 @@ template <class R> struct Euler5 {
     using Face = FaceT<R, 5, 5>;
     struct Trans { R u, v, enth, a, g1a2, euv, u2v2; };
 
+    static constexpr int NC = 0;  // no per-cell data
+    __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+
     template <int DIR>
-    __device__ static inline void rpn2(const R* qa, const R* qb, R, const PhysParams& pp,
-                                       Face& f, Trans& t)
+    __device__ static inline void rpn2(const R* qa, const R* qb, const R*, const R*, R,
+                                       const PhysParams& pp, Face& f, Trans& t)
     {
         constexpr int mu = DIR == 0 ? 1 : 2;
         constexpr int mv = DIR == 0 ? 2 : 1;

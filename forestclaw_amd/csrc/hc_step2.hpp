@@ -276,6 +276,15 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             }
         }
 
+        // ---- per-cell solver data of row J (shared by its four faces) -------
+        constexpr int NCD = S::NC > 0 ? S::NC : 1;
+        R cdc[CPL][NCD];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            if constexpr (S::NC > 0) S::cell(qc[k], pp, cdc[k]);
+            else cdc[k][0] = R(0);
+        }
+
         // ---- time-sync edge fluxes of the data before the update ----------
         if constexpr (SYNC) {
             double* rg = A.reg + (long long)tix * A.regstride;
@@ -319,6 +328,15 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                 for (int k = 0; k < CPL; k++) qa[k][m] = from_left<CPL>(t, k);
             }
+            R cda[CPL][NCD];
+#pragma unroll
+            for (int n = 0; n < NCD; n++) {
+                R t[CPL];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) t[k] = cdc[k][n];
+#pragma unroll
+                for (int k = 0; k < CPL; k++) cda[k][n] = S::NC > 0 ? from_left<CPL>(t, k) : R(0);
+            }
             R vl_cur[CPL], vl_next[CPL];
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
@@ -327,7 +345,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             }
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
-                S::template rpn2<0>(qa[k], qc[k], ucur[k], pp, fx[k], tx[k]);
+                S::template rpn2<0>(qa[k], qc[k], cda[k], cdc[k], ucur[k], pp, fx[k], tx[k]);
                 if constexpr (S::MAUX > 0) {
                     tx[k] = Trans{vl_cur[k], vl_next[k], vcur[k], vnext[k]};
                 }
@@ -494,7 +512,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         } else {
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
-                S::template rpn2<1>(qm1[k], qc[k], vcur[k], pp, fyn[k], tyn[k]);
+                R cdm[NCD];  // row J-1, recomputed rather than carried
+                if constexpr (S::NC > 0) S::cell(qm1[k], pp, cdm);
+                else cdm[0] = R(0);
+                S::template rpn2<1>(qm1[k], qc[k], cdm, cdc[k], vcur[k], pp, fyn[k], tyn[k]);
                 if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {

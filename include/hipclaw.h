@@ -230,9 +230,11 @@ int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch);
 /* Autotuner: times every launch shape (cols_per_lane 1..2 x waves_per_group
  * 1, 2, 4; reps timed launches each, after one untimed) on the current data of
  * one level (level < 0: the level with the most patches), writing into a
- * temporary buffer, and keeps the fastest for this context.  ms (optional)
- * receives the 6 average times in that order.  No reference counterpart: the
- * reference has one fixed loop nest per patch. */
+ * temporary buffer, and keeps the fastest for this context.  The kernel
+ * variant timed is the one the level updates launch (with the same-level
+ * ghost push when that is on).  ms (optional) receives the 6 average times in
+ * that order.  No reference counterpart: the reference has one fixed loop
+ * nest per patch. */
 int hipclaw_autotune(hipclaw_ctx* ctx, int level, int reps, hipclaw_launch* best, float* ms);
 /* Per-patch host arrays of length num_patches (any pointer may be NULL). */
 int hipclaw_patch_info(hipclaw_ctx* ctx, int* level, int* gx, int* gy, double* xlower,

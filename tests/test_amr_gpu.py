@@ -191,14 +191,17 @@ def test_autotune_and_launch_shapes_bitwise(dev):
     sim.set_launch(None)
 
 
-@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
-def test_merged_launch_and_ghost_push_change_no_bits(dev, precision):
+@pytest.mark.parametrize("precision,launch", [(H.FP64, None), (H.FP32, None), (H.FP32, (1, 2)),
+                                              (H.FP64, (1, 1))])
+def test_merged_launch_and_ghost_push_change_no_bits(dev, precision, launch):
     """hipclaw_advance with one launch per subcycled time and the same-level
     ghost push gives the same bits as per-level launches with ghost copies."""
     res = []
     for on in (True, False):
         sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, precision=precision, refine=0.5)
         sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        if launch is not None:
+            sim.set_launch(*launch)
         sim.set_feature(H.FEATURE_MERGE_LEVELS, on)
         sim.set_feature(H.FEATURE_GHOST_PUSH, on)
         cfl = [sim.advance(s * 0.01, 0.01) for s in range(3)]

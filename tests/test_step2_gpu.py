@@ -159,8 +159,9 @@ def test_fp64_methods_and_limiters(dev, order, mthlim):
     assert np.array_equal(cfl, rcfl)
 
 
-@pytest.mark.parametrize("kind,mx,n", [("swe", 32, 64), ("euler", 16, 32), ("adv", 32, 16)])
-@pytest.mark.parametrize("launch", [None, (1, 4), (2, 4)])
+@pytest.mark.parametrize("kind,mx,n", [("swe", 32, 64), ("euler", 16, 32), ("adv", 32, 16),
+                                       ("swe", 16, 37)])
+@pytest.mark.parametrize("launch", [None, (1, 4), (2, 4), (2, 1)])
 def test_fp32_error_vs_fp64_oracle(dev, kind, mx, n, launch):
     rng = np.random.default_rng(1000 + n)
     q, aux, dx, dy, dt = _batch(kind, rng, n, mx, mx)
