@@ -251,6 +251,65 @@ def host_path(sim64, n, dt):
                       "hipclaw_single_step_update with batches of up to 4096, best of 2 passes"}
 
 
+# algorithmic FLOPs per 36x36 tile of the refinement network (3x3 convolutions
+# 8->16->32->32->16): forward 2*1296*9*sum(Cin*Cout); training adds the data
+# gradient of layers 1-3 and the weight gradient of all four
+CNN_FWD_FLOP = 2 * 1296 * 9 * (8 * 16 + 16 * 32 + 32 * 32 + 32 * 16)
+CNN_TRAIN_FLOP = CNN_FWD_FLOP + 2 * 1296 * 9 * (16 * 32 + 32 * 32 + 32 * 16) + CNN_FWD_FLOP
+BF16_PEAK = 2.5e15
+
+
+def cnn_c4(nruns, batch=4096):
+    """The learned-indicator configuration at its stated size: tiles of
+    seeded dam-break runs (random centre, radius 0.3-0.8, h ratio 1.5-4) on
+    uniform levels 4-6, labelled with the plain tags; one epoch of Adam at
+    batch 4096 and one inference pass, both timed (BF16 MFMA kernels)."""
+    import torch
+    from forestclaw_amd import hipclaw as H
+    from forestclaw_amd import learned_tagging as LT
+    t0 = time.perf_counter()
+    data = LT.generate(nruns=nruns, levels=(4, 5, 6), samples=8, lookahead=0, neg_keep=1.0, seed=11)
+    torch.cuda.synchronize()
+    tgen = time.perf_counter() - t0
+    n = len(data)
+    net = H.Cnn(seed=5)
+    loss = torch.zeros(1, device="cuda")
+    perm = torch.randperm(n, device="cuda", dtype=torch.int64).to(torch.int32)
+    # one untimed batch (work buffers), then the timed epoch
+    net.train_step(data.tiles, data.labels, perm[:batch], batch, 2e-3, loss)
+    torch.cuda.synchronize()
+    nb = n // batch
+    t0 = time.perf_counter()
+    for b in range(nb):
+        net.train_step(data.tiles, data.labels, perm[b * batch:(b + 1) * batch], batch, 2e-3, loss)
+    torch.cuda.synchronize()
+    ttrain = time.perf_counter() - t0
+    logits = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    ib = 16384
+    ids = torch.arange(0, n, dtype=torch.int32, device="cuda")
+    net.forward(data.tiles, logits, idx=ids[:ib], n=ib)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b0 in range(0, n, ib):
+        m = min(ib, n - b0)
+        net.forward(data.tiles, logits[b0:], idx=ids[b0:b0 + m], n=m)
+    torch.cuda.synchronize()
+    tinf = time.perf_counter() - t0
+    pred = (logits[:, 1] > logits[:, 0]).to(torch.int32)
+    acc = float((pred == data.labels).float().mean())
+    train_rate = nb * batch / ttrain
+    inf_rate = n / tinf
+    net.close()
+    return {"tiles": n, "positive": int(data.labels.sum()), "batch": batch,
+            "generate_s": tgen, "train_tiles_per_s": train_rate, "infer_tiles_per_s": inf_rate,
+            "train_tflops": train_rate * CNN_TRAIN_FLOP / 1e12,
+            "infer_tflops": inf_rate * CNN_FWD_FLOP / 1e12,
+            "mfma_peak_tflops": BF16_PEAK / 1e12,
+            "train_frac": train_rate * CNN_TRAIN_FLOP / BF16_PEAK,
+            "final_batch_loss": float(loss.item()), "train_accuracy_after_1_epoch": acc,
+            "sample": f"{nruns} dam-break runs x levels 4-6 x 8 samples, all tiles kept"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +330,9 @@ def main():
                     help="same-level ghost copies by the ghost kernel instead of the step kernel")
     ap.add_argument("--no-merge", action="store_true",
                     help="one step launch per level instead of one per subcycled time")
+    ap.add_argument("--cnn", type=int, default=0, metavar="NRUNS",
+                    help="also time the refinement network on the learned-indicator "
+                         "configuration built from NRUNS dam-break runs (50: about 2M tiles)")
     ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
                     help="also time the host-resident drop-in path (hipclaw_single_step_update "
                          "on NPATCH host tiles, PCIe included)")
@@ -372,6 +434,10 @@ def main():
         res["cpu_port_openmp"] = port
     if hp is not None:
         res["host_path_pcie"] = hp
+    if args.cnn > 0:
+        sim32.close()
+        sim64.close()
+        res["cnn_c4"] = cnn_c4(args.cnn)
     if args.no_push or args.no_merge:
         res["config"]["features"] = {"ghost_push": not args.no_push, "merge_levels": not args.no_merge}
     res["roofline"]["achieved"] = achieved / 1e9

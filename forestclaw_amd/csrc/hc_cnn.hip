@@ -17,10 +17,14 @@
 // is 8 consecutive channels: a single 16-byte load.
 //   forward      M = pixels, N = Cout, K = 9*Cin   (bias + ReLU fused)
 //   data grad    M = pixels, N = Cin,  K = 9*Cout  (ReLU mask fused, W^T)
-//   weight grad  M = Cout,   N = 9*Cin, K = pixels (per-workgroup partial
-//                sums in registers, one FP32 atomic per weight per workgroup)
+//   weight grad  M = Cout,   N = 9*Cin, K = pixels (per-wave partial sums
+//                in registers, bias fused; the pixel-major fragments come
+//                from 16-byte channel loads transposed through LDS; the
+//                partials are summed in a fixed order, so gradients are
+//                deterministic)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -174,94 +178,123 @@ k_conv_bwd_data(const bf16* __restrict__ dout, const bf16* __restrict__ out,
         }
 }
 
-// ---------------------------------------------------------------------------
-// Weight gradient: dW[co][tap*CIN+ci] += sum_p dZ[p][co] * in[p + tap][ci].
-// Workgroup g reduces batch tiles g, g+G, ...; wave w owns row block
-// mb = w % MB and column blocks nb = w / MB + j * (4 / MB).
-// ---------------------------------------------------------------------------
+// Weight and bias gradient, deterministic: every wave of the grid owns the
+// 32-pixel chunks w, w + W, ... (chunk = tile * WG_NCHUNK + c, pixels c*32 ..
+// c*32+31 of the tile) and accumulates the whole dW[CO][9*CIN] plus the bias
+// sums in registers; it writes them to its row of part[W][CO*K + CO], which
+// k_wgrad_reduce sums in a fixed order.  The MFMA reduction runs over pixels,
+// so an operand fragment is 8 pixels of one channel: each lane loads 8
+// consecutive channels of one pixel (one 16-byte load from the channels-last
+// activations), and a transpose through the wave's own LDS hands every lane
+// its 8 pixels of one channel.
+constexpr int WG_NCHUNK = (NPIX + 31) / 32;  // 41 (the last chunk is half empty)
+constexpr int WG_WAVES = 1024;               // waves (partial rows) per launch
+
 template <int CI, int CO>
 __global__ void __launch_bounds__(256)
-k_conv_bwd_w(const bf16* __restrict__ in, const int* __restrict__ idx, const bf16* __restrict__ dout,
-             const bf16* __restrict__ out, float* __restrict__ dW, int n)
+k_conv_wgrad(const bf16* __restrict__ in, const int* __restrict__ idx, const bf16* __restrict__ dout,
+             const bf16* __restrict__ out, float* __restrict__ part, int n)
 {
-    constexpr int K = 9 * CI, MB = CO / 16, NBT = (K + 15) / 16, STEP = 4 / MB;
-    constexpr int NBW = (NBT + STEP - 1) / STEP;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int K = 9 * CI, MB = CO / 16, NBT = (K + 15) / 16;
+    static_assert(CI % 8 == 0 && CO % 16 == 0, "channel counts");
+    __shared__ __align__(16) bf16 sA[4][MB][32][16];
+    __shared__ __align__(16) bf16 sB[4][32][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gw = blockIdx.x * 4 + wv;
     const int m = lane & 15, kq = lane >> 4;
-    const int mb = wave % MB, nb0 = wave / MB;
-    f32x4 acc[NBW];
+    const int li = m >> 1, lh = m & 1;  // loading role: pixel kq*8 + li, channel half lh
+    f32x4 acc[MB][NBT];
+    float bsum[MB];
 #pragma unroll
-    for (int j = 0; j < NBW; j++) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const int co = mb * 16 + m;
-    for (int t = blockIdx.x; t < n; t += gridDim.x) {
-        const bf16* src = in + (long long)(idx ? idx[t] : t) * NPIX * CI;
-        const bf16* g = dout + (long long)t * NPIX * CO;
-        const bf16* o = out + (long long)t * NPIX * CO;
-        for (int p0 = 0; p0 < NPIX; p0 += 32) {
-            // A fragment: dZ at pixels p0 + 8*kq + i, channel co
-            bf16x8 a;
+    for (int b = 0; b < MB; b++) {
+        bsum[b] = 0.f;
+#pragma unroll
+        for (int j = 0; j < NBT; j++) acc[b][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const long long nchunks = (long long)n * WG_NCHUNK;
+    for (long long ch = gw; ch < nchunks; ch += WG_WAVES) {
+        const int t = (int)(ch / WG_NCHUNK), c = (int)(ch - (long long)t * WG_NCHUNK);
+        const int p = c * 32 + kq * 8 + li;  // the pixel this lane loads
+        const bool pin = p < NPIX;
+        const int y = p / TW, x = p - (p / TW) * TW;
+        // A: masked output gradient, 8 pixels of channel mb*16 + m per lane
+        bf16x8 a[MB];
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
+            bf16x8 v = zero8();
+            if (pin) {
+                const long long e = ((long long)t * NPIX + p) * CO + b * 16 + lh * 8;
+                const bf16x8 g = ld8(dout + e), o = ld8(out + e);
+#pragma unroll
+                for (int i = 0; i < 8; i++) v[i] = (float)o[i] > 0.f ? g[i] : (bf16)0.0f;
+            }
+            *reinterpret_cast<bf16x8*>(&sA[wv][b][kq * 8 + li][lh * 8]) = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int b = 0; b < MB; b++) {
 #pragma unroll
             for (int i = 0; i < 8; i++) {
-                const int p = p0 + kq * 8 + i;
-                bf16 v = (bf16)0.0f;
-                if (p < NPIX && (float)o[(long long)p * CO + co] > 0.f) v = g[(long long)p * CO + co];
-                a[i] = v;
-            }
-#pragma unroll
-            for (int j = 0; j < NBW; j++) {
-                const int nb = nb0 + j * STEP;
-                if (nb >= NBT) break;  // uniform across the wave
-                const int col = nb * 16 + m;
-                bf16x8 b = zero8();
-                if (col < K) {
-                    const int tap = col / CI, ci = col - tap * CI;
-                    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-#pragma unroll
-                    for (int i = 0; i < 8; i++) {
-                        const int p = p0 + kq * 8 + i;
-                        if (p < NPIX) {
-                            const int yy = p / TW + dy, xx = p % TW + dx;
-                            if (yy >= 0 && yy < TW && xx >= 0 && xx < TW) b[i] = src[(yy * TW + xx) * CI + ci];
-                        }
-                    }
-                }
-                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+                a[b][i] = sA[wv][b][kq * 8 + i][m];
+                bsum[b] += (float)a[b][i];
             }
         }
+        const bf16* src = in + (long long)(idx ? idx[t] : t) * NPIX * CI;
+#pragma unroll
+        for (int j = 0; j < NBT; j++) {
+            // B: the input at pixel + tap, 8 pixels of column j*16 + m per lane
+            const int col0 = j * 16 + lh * 8;
+            bf16x8 v = zero8();
+            if (pin && col0 < K) {
+                const int tap = col0 / CI, ci0 = col0 - tap * CI;
+                const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+                if (yy >= 0 && yy < TW && xx >= 0 && xx < TW) v = ld8(src + (yy * TW + xx) * CI + ci0);
+            }
+            __builtin_amdgcn_wave_barrier();
+            *reinterpret_cast<bf16x8*>(&sB[wv][kq * 8 + li][lh * 8]) = v;
+            __builtin_amdgcn_wave_barrier();
+            bf16x8 bb;
+#pragma unroll
+            for (int i = 0; i < 8; i++) bb[i] = sB[wv][kq * 8 + i][m];
+#pragma unroll
+            for (int b = 0; b < MB; b++) acc[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[b], bb, acc[b][j], 0, 0, 0);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
+    float* row = part + (long long)gw * (CO * K + CO);
 #pragma unroll
-    for (int j = 0; j < NBW; j++) {
-        const int nb = nb0 + j * STEP;
-        if (nb >= NBT) break;
-        const int col = nb * 16 + m;
-        if (col >= K) continue;
+    for (int b = 0; b < MB; b++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) atomicAdd(&dW[(long long)(mb * 16 + kq * 4 + i) * K + col], acc[j][i]);
+        for (int j = 0; j < NBT; j++) {
+            const int col = j * 16 + m;
+            if (col < K)
+#pragma unroll
+                for (int i = 0; i < 4; i++) row[(b * 16 + kq * 4 + i) * K + col] = acc[b][j][i];
+        }
+        float s = bsum[b];
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (kq == 0) row[CO * K + b * 16 + m] = s;
     }
 }
 
-// Bias gradient: db[co] += sum over pixels of dZ[p][co] (one workgroup per tile).
-template <int CO>
+// G[e] = sum over the WG_WAVES partial rows of part[.][e], e < len, in a fixed
+// order: a block takes 64 entries; its four waves sum a quarter of the rows
+// each, and the quarters are added in wave order.
 __global__ void __launch_bounds__(256)
-k_bias_grad(const bf16* __restrict__ dout, const bf16* __restrict__ out, float* __restrict__ db, int n)
+k_wgrad_reduce(const float* __restrict__ part, int len, float* __restrict__ G)
 {
-    __shared__ float red[256];
-    const int t = blockIdx.x;
-    if (t >= n) return;
-    const int co = threadIdx.x % CO, lane_grp = threadIdx.x / CO, ngrp = 256 / CO;
+    __shared__ float red[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int p = lane_grp; p < NPIX; p += ngrp) {
-        const long long e = ((long long)t * NPIX + p) * CO + co;
-        if ((float)out[e] > 0.f) s += (float)dout[e];
-    }
-    red[threadIdx.x] = s;
+    if (e < len)
+        for (int w = wv * (WG_WAVES / 4); w < (wv + 1) * (WG_WAVES / 4); w++) s += part[(long long)w * len + e];
+    red[wv][lane] = s;
     __syncthreads();
-    if (threadIdx.x < CO) {
-        float tot = 0.f;
-        for (int gI = 0; gI < ngrp; gI++) tot += red[gI * CO + threadIdx.x];
-        atomicAdd(&db[threadIdx.x], tot);
-    }
+    if (wv == 0 && e < len) G[e] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
+
 
 // ---------------------------------------------------------------------------
 // Head: global average pool of conv4 (16 ch) -> logits = Wfc f + bfc.  With
@@ -425,6 +458,7 @@ struct hipclaw_cnn {
     bf16* act[4] = {};         // conv outputs [cap, NPIX, COUT]
     bf16* dact[4] = {};        // gradients w.r.t. conv outputs
     float* logits = nullptr;   // [cap, 2]
+    float* part = nullptr;     // weight-gradient partials [WG_WAVES][CO*K + CO] (largest layer)
     long long step = 0;
 };
 
@@ -491,29 +525,40 @@ int backward(hipclaw_cnn* c, const void* tiles, const int* labels, const int* id
                        c->logits, c->G, c->dact[3], c->loss);
     const bf16* x = static_cast<const bf16*>(tiles);
     const unsigned nb = blocks_for((long long)n * PBLK);
-    const unsigned gw = (unsigned)(n < 1024 ? n : 1024);
+    // weight and bias gradients of layer l: per-wave partials, then a fixed-order sum
+    auto wgrad = [&](int l, auto launch) {
+        launch();
+        const int len = COUT[l] * 9 * CIN[l] + COUT[l];
+        hipLaunchKernelGGL(k_wgrad_reduce, dim3((unsigned)((len + 63) / 64)), dim3(256), 0, s, c->part, len,
+                           c->G + c->L.w[l]);
+    };
+    const dim3 gwg(WG_WAVES / 4);
     // layer 4 (32 -> 16)
-    hipLaunchKernelGGL((k_conv_bwd_w<32, 16>), dim3(gw), dim3(256), 0, s, c->act[2], nullptr, c->dact[3],
-                       c->act[3], c->G + c->L.w[3], n);
-    hipLaunchKernelGGL(k_bias_grad<16>, dim3(n), dim3(256), 0, s, c->dact[3], c->act[3], c->G + c->L.b[3], n);
+    wgrad(3, [&] {
+        hipLaunchKernelGGL((k_conv_wgrad<32, 16>), gwg, dim3(256), 0, s, c->act[2], nullptr, c->dact[3],
+                           c->act[3], c->part, n);
+    });
     hipLaunchKernelGGL((k_conv_bwd_data<32, 16>), dim3(nb), dim3(256), 0, s, c->dact[3], c->act[3],
                        c->WTb[3], c->dact[2], n);
     // layer 3 (32 -> 32)
-    hipLaunchKernelGGL((k_conv_bwd_w<32, 32>), dim3(gw), dim3(256), 0, s, c->act[1], nullptr, c->dact[2],
-                       c->act[2], c->G + c->L.w[2], n);
-    hipLaunchKernelGGL(k_bias_grad<32>, dim3(n), dim3(256), 0, s, c->dact[2], c->act[2], c->G + c->L.b[2], n);
+    wgrad(2, [&] {
+        hipLaunchKernelGGL((k_conv_wgrad<32, 32>), gwg, dim3(256), 0, s, c->act[1], nullptr, c->dact[2],
+                           c->act[2], c->part, n);
+    });
     hipLaunchKernelGGL((k_conv_bwd_data<32, 32>), dim3(nb), dim3(256), 0, s, c->dact[2], c->act[2],
                        c->WTb[2], c->dact[1], n);
     // layer 2 (16 -> 32)
-    hipLaunchKernelGGL((k_conv_bwd_w<16, 32>), dim3(gw), dim3(256), 0, s, c->act[0], nullptr, c->dact[1],
-                       c->act[1], c->G + c->L.w[1], n);
-    hipLaunchKernelGGL(k_bias_grad<32>, dim3(n), dim3(256), 0, s, c->dact[1], c->act[1], c->G + c->L.b[1], n);
+    wgrad(1, [&] {
+        hipLaunchKernelGGL((k_conv_wgrad<16, 32>), gwg, dim3(256), 0, s, c->act[0], nullptr, c->dact[1],
+                           c->act[1], c->part, n);
+    });
     hipLaunchKernelGGL((k_conv_bwd_data<16, 32>), dim3(nb), dim3(256), 0, s, c->dact[1], c->act[1],
                        c->WTb[1], c->dact[0], n);
     // layer 1 (8 -> 16): weights only
-    hipLaunchKernelGGL((k_conv_bwd_w<8, 16>), dim3(gw), dim3(256), 0, s, x, idx, c->dact[0], c->act[0],
-                       c->G + c->L.w[0], n);
-    hipLaunchKernelGGL(k_bias_grad<16>, dim3(n), dim3(256), 0, s, c->dact[0], c->act[0], c->G + c->L.b[0], n);
+    wgrad(0, [&] {
+        hipLaunchKernelGGL((k_conv_wgrad<8, 16>), gwg, dim3(256), 0, s, x, idx, c->dact[0], c->act[0],
+                           c->part, n);
+    });
     return check_hip(hipGetLastError(), "cnn backward");
 }
 
@@ -540,6 +585,9 @@ int hipclaw_cnn_create(void* stream, hipclaw_cnn** out)
     if (!rc) rc = check_hip(hipMalloc((void**)&c->M1, pb), "cnn adam");
     if (!rc) rc = check_hip(hipMalloc((void**)&c->M2, pb), "cnn adam");
     if (!rc) rc = check_hip(hipMalloc((void**)&c->loss, sizeof(float)), "cnn loss");
+    size_t prow = 0;  // the largest layer's weight + bias count
+    for (int l = 0; l < 4; l++) prow = std::max(prow, (size_t)COUT[l] * 9 * CIN[l] + COUT[l]);
+    if (!rc) rc = check_hip(hipMalloc((void**)&c->part, sizeof(float) * WG_WAVES * prow), "cnn partials");
     for (int l = 0; l < 4 && !rc; l++) {
         const size_t wb = sizeof(bf16) * (size_t)COUT[l] * 9 * CIN[l];
         rc = check_hip(hipMalloc((void**)&c->Wb[l], wb), "cnn bf16 weights");
@@ -562,7 +610,7 @@ int hipclaw_cnn_destroy(hipclaw_cnn* c)
 {
     if (!c) return HIPCLAW_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void* bufs[] = {c->P, c->G, c->M1, c->M2, c->loss, c->logits};
+    void* bufs[] = {c->P, c->G, c->M1, c->M2, c->loss, c->logits, c->part};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (int l = 0; l < 4; l++) {
