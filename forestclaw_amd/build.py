@@ -39,6 +39,7 @@ SOURCES = {
     "hc_ctx.hip": STRICT,
     "hc_forest.cpp": STRICT,
     "hc_plan.cpp": STRICT,
+    "hc_output.cpp": STRICT,
 }
 
 

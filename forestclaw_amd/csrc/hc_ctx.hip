@@ -1157,6 +1157,22 @@ int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q)
     return copy_q(*ctx->c, host_q, false, false);
 }
 
+int hipclaw_output_ascii(hipclaw_ctx* ctx, int iframe, double time, const char* dir)
+{
+    CTX_READY(ctx);
+    Ctx& c = *ctx->c;
+    const int n = c.n;
+    std::vector<int> lev(n), gx(n), gy(n);
+    std::vector<double> xl(n), yl(n), dx(n), dy(n);
+    int rc = hipclaw_patch_info(ctx, lev.data(), gx.data(), gy.data(), xl.data(), yl.data(), dx.data(), dy.data());
+    if (rc) return rc;
+    std::vector<double> q((size_t)n * c.tile);  // tile = elements per slot
+    rc = hipclaw_get_q(ctx, q.data());
+    if (rc) return rc;
+    return write_frame(dir, iframe, time, c.dom.mi, c.phys.mx, c.phys.my, c.phys.meqn, c.phys.maux, n, lev.data(),
+                       gx.data(), gy.data(), xl.data(), yl.data(), dx.data(), dy.data(), q.data());
+}
+
 int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux)
 {
     CTX_READY(ctx);

@@ -40,4 +40,10 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
                  double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,
                  const MultiLevel* ml);
 
+// fort.tXXXX / fort.qXXXX frame of n patches (host arrays, q in the tile
+// layout T*meqn per patch, mbc = 2); hc_output.cpp
+int write_frame(const char* dir, int iframe, double time, int mi, int mx, int my, int meqn, int maux, int n,
+                const int* lev, const int* gx, const int* gy, const double* xl, const double* yl,
+                const double* dx, const double* dy, const double* q);
+
 }  // namespace hc

@@ -175,6 +175,12 @@ class Hipclaw46:
         return dict(zip(("patches", "net_refine", "plain_refine", "misses", "false_refines",
                          "fallbacks"), list(st)))
 
+    def output_ascii(self, iframe: int, directory: str = ".", time: float | None = None):
+        """fort.tXXXX / fort.qXXXX frame of the current solution (the reference's
+        ascii-out output)."""
+        _check(_lib().hipclaw_output_ascii(self.h, iframe, self.t if time is None else time,
+                                           directory.encode()))
+
     def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None):
         """Patch-step launch shape for this context (None: library default)."""
         if cols_per_lane is None:

@@ -131,6 +131,12 @@ _SIGS.update({
     "hipclaw_regrid": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
     "hipclaw_initialize": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_mass": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_output_ascii": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p], ctypes.c_int),
+    "hipclaw_ascii_header": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int], ctypes.c_int),
+    "hipclaw_ascii_patch": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_cell_updates": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "hipclaw_profile": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_profile_read": ([_VP, _DP, ctypes.POINTER(ctypes.c_longlong),
@@ -331,3 +337,20 @@ def bc2_batch(phys: Physics, npatch: int, q, mthbc, patch_ids=None, stream=None,
         ntiles = npatch
     _check(lib().hipclaw_bc2_batch(ctypes.byref(phys), npatch, _ptr(q), _ptr(patch_ids),
                                    int(ntiles), _ptr(mthbc), _stream(stream)))
+
+
+def ascii_header(directory: str, iframe: int, time: float, meqn: int, maux: int, ngrids: int) -> None:
+    """fort.tXXXX of a frame (and an empty fort.qXXXX) in directory."""
+    _check(lib().hipclaw_ascii_header(directory.encode(), iframe, time, meqn, maux, ngrids))
+
+
+def ascii_patch(directory: str, iframe: int, q, xlower: float, ylower: float, dx: float, dy: float,
+                patch_num: int, level: int, blockno: int = 0, mpirank: int = 0, mbc: int = 2) -> None:
+    """Append one patch (q: float64 tile [meqn, my+2mbc, mx+2mbc]) to fort.qXXXX."""
+    import numpy as np
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    meqn, ny, nx = q.shape
+    _check(lib().hipclaw_ascii_patch(directory.encode(), iframe, nx - 2 * mbc, ny - 2 * mbc, meqn, mbc, xlower,
+                                     ylower, dx, dy, q.ctypes.data_as(_DP), patch_num, level, blockno,
+                                     mpirank))
+

@@ -288,6 +288,22 @@ int hipclaw_regrid(hipclaw_ctx* ctx, int init_flag, int* changed);
 int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, int nparams,
                        int nsub);
 
+/* ASCII frames in the Clawpack 4.6 format of fclaw2d_clawpatch_output_ascii
+ * (src/patches/clawpatch/fclaw2d_clawpatch_output_ascii.c:32-118 and the
+ * writers of src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_output_ascii.f):
+ * <dir>/fort.tXXXX (time, meqn, ngrids, num_aux, num_dim) and
+ * <dir>/fort.qXXXX (every patch in the reference's order: blocks, then Morton
+ * order within a block, numbered from 0).  dir NULL or "" = current directory.
+ * The values are the context's solution (FP64; FP32 contexts widen). */
+int hipclaw_output_ascii(hipclaw_ctx* ctx, int iframe, double time, const char* dir);
+/* The two Fortran writers themselves, for host-resident patches:
+ * fclaw2d_clawpatch46_fort_header_ascii (creates fort.tXXXX, empties
+ * fort.qXXXX) and fclaw2d_clawpatch46_fort_output_ascii (appends one patch;
+ * q is the (mx+2mbc) x (my+2mbc) x meqn tile, meqn <= 20). */
+int hipclaw_ascii_header(const char* dir, int iframe, double time, int meqn, int maux, int ngrids);
+int hipclaw_ascii_patch(const char* dir, int iframe, int mx, int my, int meqn, int mbc, double xlower,
+                        double ylower, double dx, double dy, const double* q, int patch_num, int level,
+                        int blockno, int mpirank);
 /* FP64 conservation sums sum(q dx dy) per field (meqn doubles). */
 int hipclaw_mass(hipclaw_ctx* ctx, double* sums);
 /* Cell updates performed so far (count_single_step * mx * my). */

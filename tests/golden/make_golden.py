@@ -97,5 +97,33 @@ def main():
     save("timeinterp", qcurr=qc, qlast=ql, qinterp=qi, alpha=0.375, mx=mx, my=my, mbc=mbc)
 
 
+def ascii_golden():
+    """fort.t0003 / fort.q0003 written by the reference's Fortran writers for a
+    few patches whose values exercise the E26.16 edge cases (zero, tiny values
+    that the writer zeroes, negative values, rounding carries, exponents of
+    three digits), with the inputs, under tests/golden/ascii/."""
+    rng = np.random.default_rng(SEED + 7)
+    out = os.path.join(HERE, "ascii")
+    os.makedirs(out, exist_ok=True)
+    patches = []
+    for k, (mx, my, meqn, level, block) in enumerate([(5, 3, 3, 4, 0), (4, 4, 1, 6, 0), (3, 2, 5, 2, 1)]):
+        q = rng.standard_normal((meqn, my + 4, mx + 4)) * 10.0 ** rng.integers(-8, 9, (meqn, my + 4, mx + 4))
+        q[0, 2, 2] = 0.0
+        q[0, 2, 3] = 3e-120            # |q| < 1e-99: written as zero
+        q[0, 3, 2] = -0.99999999999999999
+        q[-1, 2, 4] = 9.999999999999999e99
+        q[-1, 3, 3] = 1.5e150          # three-digit exponent
+        q[-1, 3, 4] = -2.5e-77
+        h = 1.0 / (mx << level)
+        patches.append(dict(q=q, mbc=2, xlower=-2.5 + 5 * (k + 1) * h, ylower=0.125 * k, dx=5 * h, dy=5 * h,
+                            num=3 * k + 1, level=level, block=block))
+    R.ascii_frame(out, 3, 0.1 + 1e-17, 3, 0, patches)
+    save(os.path.join("ascii", "patches"), **{f"{key}{i}": np.asarray(p[key]) for i, p in enumerate(patches)
+                                              for key in p})
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["ascii"]:
+        ascii_golden()
+    else:
+        main()

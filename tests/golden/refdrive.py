@@ -11,6 +11,8 @@ Entry points driven (reference file:line):
   clawpack46_b4step2     applications/clawpack/advection/2d/swirl/user_4.6/b4step2.f:1
   clawpack46_bc2_default src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_bc2_default.f:1
   fclaw2d_clawpatch46_fort_timeinterp src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:11
+  fclaw2d_clawpatch46_fort_header_ascii / _output_ascii
+                         src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_output_ascii.f:13,51
 Common blocks (/cparam/, /compi/, /comvt/) are written directly as data.
 """
 from __future__ import annotations
@@ -149,3 +151,32 @@ def timeinterp(qcurr, qlast, alpha, *, mx, my, mbc):
         ctypes.byref(_ci(meqn)), ctypes.byref(_ci(psize)), _p(qc), _p(ql), _p(qi),
         ctypes.byref(_cd(alpha)), ctypes.byref(ierr))
     return qi, ierr.value
+
+
+def ascii_frame(directory: str, iframe: int, time: float, meqn: int, maux: int, patches) -> None:
+    """fort.tXXXX / fort.qXXXX written by the reference's Fortran writers in
+    directory.  patches: dicts with q [meqn, my+2mbc, mx+2mbc], mbc, xlower,
+    ylower, dx, dy, num, level, block (the writer zeroes tiny values of its
+    q argument in place, so it gets a copy)."""
+    L = _lib("libref_ascii.so")
+    m1 = ctypes.create_string_buffer(f"fort.q{iframe:04d}".encode(), 11)
+    m2 = ctypes.create_string_buffer(f"fort.t{iframe:04d}".encode(), 11)
+    cwd = os.getcwd()
+    os.chdir(directory)
+    try:
+        L.fclaw2d_clawpatch46_fort_header_ascii_(m1, m2, ctypes.byref(_cd(time)), ctypes.byref(_ci(meqn)),
+                                                 ctypes.byref(_ci(maux)), ctypes.byref(_ci(len(patches))),
+                                                 ctypes.c_size_t(10), ctypes.c_size_t(10))
+        for p in patches:
+            q = np.asfortranarray(np.transpose(np.array(p["q"], dtype=np.float64), (2, 1, 0)))
+            nm, ny, nx = p["q"].shape
+            mbc = p["mbc"]
+            L.fclaw2d_clawpatch46_fort_output_ascii_(
+                m1, ctypes.byref(_ci(nx - 2 * mbc)), ctypes.byref(_ci(ny - 2 * mbc)), ctypes.byref(_ci(nm)),
+                ctypes.byref(_ci(mbc)), ctypes.byref(_cd(p["xlower"])), ctypes.byref(_cd(p["ylower"])),
+                ctypes.byref(_cd(p["dx"])), ctypes.byref(_cd(p["dy"])), q.ctypes.data_as(ctypes.c_void_p),
+                ctypes.byref(_ci(p["num"])), ctypes.byref(_ci(p["level"])), ctypes.byref(_ci(p["block"])),
+                ctypes.byref(_ci(0)), ctypes.c_size_t(10))
+    finally:
+        os.chdir(cwd)
+

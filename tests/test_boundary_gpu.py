@@ -7,7 +7,11 @@
   the returned values equal to the oracle's level CFL; FP32 within 1e-5.
 * hipclaw_level_upload / hipclaw_level_download: per-patch host arrays of a
   level round-trip exactly and feed the same solution as hipclaw_set_q.
+* hipclaw_output_ascii: a fort.t / fort.q frame of an AMR context lists every
+  patch once in the reference's order with its geometry and interior values.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -92,3 +96,59 @@ def test_level_upload_download(dev):
         assert all(np.array_equal(a, b) for a, b in zip(new, back))
     with pytest.raises(H.HipclawError):
         sim.level_upload(int(info["level"].min()), [np.zeros(sim.tile_shape)])
+
+
+def _parse_frame(directory, iframe):
+    """(time, meqn, ngrids, patches) of a fort.t/fort.q frame."""
+    t = open(os.path.join(directory, f"fort.t{iframe:04d}")).read().split("\n")
+    time, meqn, ngrids = float(t[0].split()[0]), int(t[1].split()[0]), int(t[2].split()[0])
+    lines = open(os.path.join(directory, f"fort.q{iframe:04d}")).read().split("\n")
+    pos, patches = 0, []
+    for _ in range(ngrids):
+        hdr = [int(lines[pos + k].split()[0]) for k in range(6)]
+        geo = [float(lines[pos + 6 + k].split()[0]) for k in range(4)]
+        num, level, block, rank, mx, my = hdr
+        assert lines[pos + 10] == ""
+        pos += 11
+        q = np.zeros((meqn, my, mx))
+        for j in range(my):
+            for i in range(mx):
+                q[:, j, i] = [float(v) for v in lines[pos].split()]
+                pos += 1
+            assert lines[pos] == "  "
+            pos += 1
+        patches.append(dict(num=num, level=level, block=block, mx=mx, my=my, geo=geo, q=q))
+    assert all(l == "" for l in lines[pos:])
+    return time, meqn, ngrids, patches
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_output_ascii_frame(dev, tmp_path, precision):
+    """hipclaw_output_ascii writes every patch once, numbered 0.. in the
+    reference's order (Morton order of the leaves), with the patch geometry and
+    the interior values to the 16 digits of E26.16."""
+    opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_SWE_ROE, precision=precision,
+                    refine_threshold=0.5, ax=-1.0, bx=1.5, ay=0.0, by=2.5)
+    sim = F.Hipclaw46(opt)
+    sim.initialize("radial_dam", (0.0, 1.0, 0.3, 2.0, 1.0), nsub=4)
+    sim.advance(0.0, 0.002)
+    sim.output_ascii(7, str(tmp_path), time=0.25)
+    info = sim.patch_info()
+    q = sim.get_q()
+    time, meqn, ngrids, patches = _parse_frame(str(tmp_path), 7)
+    assert time == 0.25 and meqn == 3 and ngrids == len(q)
+    assert [p["num"] for p in patches] == list(range(ngrids))
+    # Morton order of the lower-left corners at the finest level
+    maxl = int(info["level"].max())
+
+    def mort(L, gx, gy):
+        x, y, k = gx << (maxl - L), gy << (maxl - L), 0
+        for b in range(16):
+            k |= ((x >> b) & 1) << (2 * b) | ((y >> b) & 1) << (2 * b + 1)
+        return k
+    want = sorted(range(ngrids), key=lambda k: mort(int(info["level"][k]), int(info["gx"][k]), int(info["gy"][k])))
+    for p, k in zip(patches, want):
+        assert p["level"] == info["level"][k] and p["block"] == 0 and (p["mx"], p["my"]) == (8, 8)
+        np.testing.assert_allclose(p["geo"], [info["xlower"][k], info["ylower"][k], info["dx"][k], info["dy"][k]],
+                                   rtol=1e-15, atol=0)
+        np.testing.assert_allclose(p["q"], q[k][:, 2:-2, 2:-2], rtol=1e-15, atol=1e-99)
