@@ -127,3 +127,29 @@ def precision_report(name, checkpoints=(1, 10, 50)):
     sim64.close()
     sim32.close()
     return out
+
+
+# ---------------------------------------------------------------------------
+# The reference's regression cases (applications/clawpack/*/regressions.sh):
+# the options of each regression.ini, run to tfinal with the run loop of
+# fclaw2d_run outstyle 1, giving the three counters its regression_values CSV
+# files hold.
+# ---------------------------------------------------------------------------
+def swirl_regression(precision=H.FP64):
+    """applications/clawpack/advection/2d/swirl/regression.ini (claw-version 4):
+    8x8 patches, levels 2-6, minmax tagging 0.25 / 0.05, smooth refinement,
+    van Leer limiter, swirl period 4, tfinal 0.1 from initial_dt 0.005."""
+    opt = F.Options(minlevel=2, maxlevel=6, mx=8, my=8, solver=H.SOLVER_ADV, precision=precision,
+                    mthbc=(1, 1, 1, 1), mthlim=(3,), order=(2, 2), refine_threshold=0.25,
+                    coarsen_threshold=0.05, smooth_refine=True, smooth_level=0,
+                    refinement_criteria="minmax", regrid_interval=1, use_fixed_dt=False, initial_dt=0.005,
+                    max_cfl=1.0, desired_cfl=0.9, subcycle=True)
+    sim = F.Hipclaw46(opt)
+    sim.set_swirl(4.0)
+    sim.initialize("swirl", (0.5,), nsub=1)
+    sim.run(0.1, nout=1)
+    out = dict(count_amr_advance=sim.count_amr_advance, global_num_patches=sim.num_patches(),
+               count_amr_new_domain=sim.count_amr_new_domain)
+    sim.close()
+    return out
+

@@ -299,6 +299,67 @@ __global__ void k_tag_gradient(const int* __restrict__ slots, const int* __restr
 }
 
 // ---------------------------------------------------------------------------
+// Per-patch statistics of the tagging variable, from which every criterion of
+// the clawpatch tag pass is decided (src/patches/clawpatch/fort_4.6/
+// fclaw2d_clawpatch46_tag4refinement.f:40-75 / _tag4coarsening.f:30-118 with
+// fclaw2d_clawpatch_{value,minmax,difference,gradient}_exceeds_th.f90):
+//   st[0], st[1]  min and max of q over the whole tile, ghost cells included
+//                 (the running qmin/qmax of the reference loops; "value" tests
+//                 every cell, so its decision is max > threshold)
+//   st[2]         max over interior cells of max(|dq_x|, |dq_y|), central
+//                 differences ("difference")
+//   st[3]         max over interior cells of |grad q| with dq/(2 dx) ("gradient")
+// A criterion exceeds its threshold somewhere in the patch iff the statistic
+// exceeds it, so the host decides refinement (per patch) and coarsening (per
+// family: the minmax range over the four siblings together) exactly as the
+// reference's early-exit loops do.
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void __launch_bounds__(256)
+k_tag_stats(const int* __restrict__ slots, const int* __restrict__ levels, int n, Pools<R> P,
+            const double* __restrict__ dxdy, int mx, int my, int meqn, int ivar, double* __restrict__ st)
+{
+    __shared__ double red[4][256];
+    const int p = blockIdx.x;
+    if (p >= n) return;
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const int slot = slots[p];
+    const R* q = P.q[levels[p]] + ((long long)slot * meqn + ivar) * NX * NY;
+    const double dx2 = 2 * dxdy[2 * slot], dy2 = 2 * dxdy[2 * slot + 1];
+    double qmin = (double)q[MBC * NX + MBC], qmax = qmin, dmax = 0.0, gmax = 0.0;
+    for (int c = threadIdx.x; c < NX * NY; c += blockDim.x) {
+        const double v = (double)q[c];
+        qmin = v < qmin ? v : qmin;
+        qmax = v > qmax ? v : qmax;
+    }
+    for (int c = threadIdx.x; c < mx * my; c += blockDim.x) {
+        const int i = c % mx + MBC, j = c / mx + MBC;
+        const double ex = (double)q[j * NX + i + 1] - (double)q[j * NX + i - 1];
+        const double ey = (double)q[(j + 1) * NX + i] - (double)q[(j - 1) * NX + i];
+        const double d = fmax(fabs(ex), fabs(ey));
+        const double dqx = ex / dx2, dqy = ey / dy2;
+        const double g = sqrt(dqx * dqx + dqy * dqy + 0.0 * 0.0);
+        dmax = d > dmax ? d : dmax;
+        gmax = g > gmax ? g : gmax;
+    }
+    red[0][threadIdx.x] = qmin;
+    red[1][threadIdx.x] = qmax;
+    red[2][threadIdx.x] = dmax;
+    red[3][threadIdx.x] = gmax;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + h]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + h]);
+            red[2][threadIdx.x] = fmax(red[2][threadIdx.x], red[2][threadIdx.x + h]);
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) st[4 * (long long)p + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------------------------
 // Initial conditions (the applications' qinit hooks) on cell centres.
 // ---------------------------------------------------------------------------
 template <class R>
@@ -482,6 +543,15 @@ void launch_tag(const int* slots, const int* levels, int n, const Pools<R>& P, c
 }
 
 template <class R>
+void launch_tag_stats(const int* slots, const int* levels, int n, const Pools<R>& P, const double* dxdy,
+                      int mx, int my, int meqn, int ivar, double* st, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_tag_stats<R>, dim3(n), dim3(256), 0, s, slots, levels, n, P, dxdy, mx, my, meqn, ivar,
+                       st);
+}
+
+template <class R>
 void launch_qinit(int kind, const QinitParams& prm, const int* slots, const int* levels, int n,
                   const Pools<R>& P, const double* xylower, const double* dxdy, int mx, int my,
                   int meqn, hipStream_t s)
@@ -516,6 +586,8 @@ void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long 
     template void launch_xfer<R>(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t); \
     template void launch_tag<R>(const int*, const int*, int, const Pools<R>&, const double*, int, int, \
                                 int, int, double, int, int*, hipStream_t);                        \
+    template void launch_tag_stats<R>(const int*, const int*, int, const Pools<R>&, const double*, int, \
+                                      int, int, int, double*, hipStream_t);                      \
     template void launch_qinit<R>(int, const QinitParams&, const int*, const int*, int,             \
                                   const Pools<R>&, const double*, const double*, int, int, int,   \
                                   hipStream_t);                                                   \

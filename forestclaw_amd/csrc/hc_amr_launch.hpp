@@ -28,6 +28,9 @@ template <class R>
 void launch_tag(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, int,
                 double, int, int*, hipStream_t);
 template <class R>
+void launch_tag_stats(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, int, double*,
+                      hipStream_t);
+template <class R>
 void launch_qinit(int, const QinitParams&, const int*, const int*, int, const Pools<R>&,
                   const double*, const double*, int, int, int, hipStream_t);
 template <class R>

@@ -81,7 +81,7 @@ void Ctx::release_pools()
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
-    double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass, &d_reg};
+    double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass, &d_reg, &d_tagst};
     for (auto b : dbufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -97,6 +97,7 @@ void Ctx::release_pools()
         lp.ts.free_(); lp.ts_same.free_(); lp.ts_f2c.free_(); lp.ts_phys.free_();
     }
     n = cap = 0;
+    tagst_cap = 0;
 }
 
 template <class R> Pools<R> Ctx::pools() const
@@ -622,28 +623,29 @@ int Ctx::advance_body(double t, double dt)
     const int lo = forest.global_minlevel(), hi = forest.global_maxlevel();
     struct Cnt { int last_step, step_inc, total_steps; double t, dt_step; } c[MAXL];
     const bool sub = dom.subcycle != 0;
-    for (int L = lo; L <= hi; L++) {
+    // time-step counters of initialize_timestep_counters (src/fclaw2d_advance.c:54-118):
+    // they count from the options' minlevel m0, whose step dt is; a coarsest
+    // present level lo > m0 therefore takes 2^(lo - m0) steps per coarse step.
+    // Without subcycling every level takes 2^(maxlevel - minlevel) steps of
+    // the options' level range.
+    const int m0 = std::min(dom.minlevel, lo);
+    const int rf = 1 << std::max(dom.maxlevel - dom.minlevel, hi - lo);
+    double d = dt;  // the reference's dt_level /= 2 per level from minlevel
+    for (int L = m0; L <= hi; L++) {
         c[L].last_step = 0;
         c[L].t = t;
         if (sub) {
-            c[L].dt_step = dt / (double)(1 << (L - lo));
+            c[L].dt_step = d;
             c[L].step_inc = 1 << (hi - L);
-            c[L].total_steps = 1 << (L - lo);
+            c[L].total_steps = 1 << (L - m0);
+            d /= 2;
         } else {
             c[L].step_inc = 1;
-            c[L].dt_step = dt / (double)(1 << (hi - lo));
-            c[L].total_steps = 1 << (hi - lo);
+            c[L].dt_step = dt / (double)rf;
+            c[L].total_steps = rf;
         }
     }
     cflacc_count = 0;
-    // reference dt_step halving: dt_level /= 2 per level from minlevel
-    if (sub) {
-        double d = dt;
-        for (int L = lo; L <= hi; L++) {
-            c[L].dt_step = d;
-            d /= 2;
-        }
-    }
     // recursive level advance (advance_level).  The recursion is walked on the
     // host first: the levels it updates at one fine step are independent of
     // each other (each reads its own pool and writes its other one), so they
@@ -679,8 +681,15 @@ int Ctx::advance_body(double t, double dt)
             if (sub) {
                 int tl = hi - 1;
                 while (c[hi].last_step % c[tl].step_inc == 0) tl--;
-                HC_TRY(ghost_update(tl + 1, hi, 1));
-                HC_TRY(time_sync(tl + 1, hi));
+                // all present levels synchronised (a time-interpolated level
+                // below lo has no patches): the update of [lo, hi]
+                if (tl < lo) {
+                    HC_TRY(ghost_update(lo, hi, 0));
+                    HC_TRY(time_sync(lo, hi));
+                } else {
+                    HC_TRY(ghost_update(tl + 1, hi, 1));
+                    HC_TRY(time_sync(tl + 1, hi));
+                }
             } else {
                 HC_TRY(ghost_update(lo, hi, 0));
                 HC_TRY(time_sync(lo, hi));
@@ -728,16 +737,36 @@ int Ctx::make_tiles(void* tiles, int* tags)
 int Ctx::tag_and_adapt(int init_flag, int* changed)
 {
     const int nn = n;
+    // statistics of the tagging variable of every patch (k_tag_stats)
+    if (tagst_cap < nn) {
+        if (d_tagst) (void)hipFree(d_tagst);
+        d_tagst = nullptr;
+        tagst_cap = 0;
+        HC_TRY(dmalloc(reinterpret_cast<void**>(&d_tagst), sizeof(double) * 4 * (size_t)nn));
+        tagst_cap = nn;
+    }
     if (esz == 8)
-        launch_tag<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
-                           tag_ivar, refine_threshold, 0, d_tags, stream);
+        launch_tag_stats<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                                 tag_ivar, d_tagst, stream);
     else
-        launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
-                          tag_ivar, refine_threshold, 0, d_tags, stream);
-    std::vector<int> rtag(nn), ctag(nn);
+        launch_tag_stats<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
+                                tag_ivar, d_tagst, stream);
+    std::vector<double> st(4 * (size_t)nn);
     // synchronous copies run on the null stream: wait for this context's stream
     HC_TRY(check_hip(hipStreamSynchronize(stream), "tags"));
-    HC_TRY(check_hip(hipMemcpy(rtag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
+    HC_TRY(check_hip(hipMemcpy(st.data(), d_tagst, sizeof(double) * st.size(), hipMemcpyDeviceToHost), "tags"));
+    // the criterion of clawpatch's refinement-criteria on (min, max, max
+    // difference, max gradient) of a patch or of a family
+    auto exceeds = [&](const double* s4, double thr) {
+        switch (tag_criterion) {
+        case HIPCLAW_TAG_VALUE: return s4[1] > thr;
+        case HIPCLAW_TAG_DIFFERENCE: return s4[2] > thr;
+        case HIPCLAW_TAG_MINMAX: return s4[1] - s4[0] > thr;
+        default: return s4[3] > thr;
+        }
+    };
+    std::vector<int> rtag(nn), ctag(nn, 0);
+    for (int k = 0; k < nn; k++) rtag[k] = exceeds(&st[4 * (size_t)k], refine_threshold) ? 1 : 0;
     if (cnn) {
         // learned refinement tags; the plain tag is the fallback and the judge
         if (tiles_cap < nn) {
@@ -756,40 +785,57 @@ int Ctx::tag_and_adapt(int init_flag, int* changed)
         HC_TRY(check_hip(hipStreamSynchronize(stream), "cnn"));
         HC_TRY(check_hip(hipMemcpy(lg.data(), d_logits, sizeof(float) * lg.size(), hipMemcpyDeviceToHost),
                          "logits"));
-        int st[6] = {nn, 0, 0, 0, 0, 0};
+        int ts6[6] = {nn, 0, 0, 0, 0, 0};
         for (int k = 0; k < nn; k++) {
             const double d = (double)lg[2 * k + 1] - (double)lg[2 * k];
             const double pr = 1.0 / (1.0 + std::exp(-d));
             int t = pr >= 0.5 ? 1 : 0;
-            st[2] += rtag[k] != 0;
+            ts6[2] += rtag[k] != 0;
             if (std::fabs(pr - 0.5) < cnn_margin) {
                 t = rtag[k] != 0;
-                st[5]++;
+                ts6[5]++;
             } else {
-                st[3] += (rtag[k] != 0 && !t);
-                st[4] += (rtag[k] == 0 && t);
+                ts6[3] += (rtag[k] != 0 && !t);
+                ts6[4] += (rtag[k] == 0 && t);
             }
-            st[1] += t;
+            ts6[1] += t;
             rtag[k] = t;
         }
-        std::memcpy(tag_stats, st, sizeof(st));
+        std::memcpy(tag_stats, ts6, sizeof(ts6));
     }
     if (!init_flag) {
-        if (esz == 8)
-            launch_tag<double>(d_iota, d_levels, nn, pools<double>(), d_dxdy, phys.mx, phys.my, phys.meqn,
-                               tag_ivar, coarsen_threshold, 1, d_tags, stream);
-        else
-            launch_tag<float>(d_iota, d_levels, nn, pools<float>(), d_dxdy, phys.mx, phys.my, phys.meqn,
-                              tag_ivar, coarsen_threshold, 1, d_tags, stream);
-        HC_TRY(check_hip(hipStreamSynchronize(stream), "tags"));
-        HC_TRY(check_hip(hipMemcpy(ctag.data(), d_tags, sizeof(int) * nn, hipMemcpyDeviceToHost), "tags"));
+        // families of four sibling leaves above minlevel coarsen when the
+        // criterion does not exceed the coarsen threshold on the four together
+        // (tag4coarsening never coarsens at initialisation)
+        for (int k = 0; k < nn; k++) {
+            const Leaf& l = forest.leaves[k];
+            if (l.level <= dom.minlevel || (l.gx & 1) || (l.gy & 1)) continue;
+            int fam[4];
+            bool ok = true;
+            for (int c = 0; c < 4 && ok; c++) {
+                fam[c] = forest.find(l.level, l.gx + (c & 1), l.gy + (c >> 1));
+                ok = fam[c] >= 0;
+            }
+            if (!ok) continue;
+            double f4[4] = {st[4 * (size_t)fam[0]], st[4 * (size_t)fam[0] + 1], 0.0, 0.0};
+            for (int c = 0; c < 4; c++) {
+                const double* s4 = &st[4 * (size_t)fam[c]];
+                f4[0] = std::min(f4[0], s4[0]);
+                f4[1] = std::max(f4[1], s4[1]);
+                f4[2] = std::max(f4[2], s4[2]);
+                f4[3] = std::max(f4[3], s4[3]);
+            }
+            if (!exceeds(f4, coarsen_threshold))
+                for (int c = 0; c < 4; c++) ctag[fam[c]] = 1;
+        }
     }
     std::vector<int> tags(nn, 0);
     for (int k = 0; k < nn; k++) {
         const Leaf& l = forest.leaves[k];
         if (rtag[k] && l.level < dom.maxlevel) tags[k] = 1;
-        else if (!init_flag && ctag[k] && l.level > dom.minlevel) tags[k] = -1;
+        else if (ctag[k]) tags[k] = -1;
     }
+    if (smooth_refine) tags = forest.smooth_marks(tags, smooth_level);
     std::vector<Leaf> before = forest.leaves;
     std::vector<Transfer> xfer;
     try {
@@ -964,8 +1010,28 @@ int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on)
         c.clear_pushed();
         break;
     case HIPCLAW_FEATURE_GRAPH: c.use_graph = on != 0; break;
+    case HIPCLAW_FEATURE_INIT_GHOSTCELL: c.init_ghostcell = on != 0; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown feature");
     }
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_tag_criterion(hipclaw_ctx* ctx, int criterion)
+{
+    CTX_CHECK(ctx);
+    if (criterion < HIPCLAW_TAG_VALUE || criterion > HIPCLAW_TAG_GRADIENT)
+        return set_error(HIPCLAW_ERR_ARG, "criterion: HIPCLAW_TAG_VALUE, _DIFFERENCE, _MINMAX or _GRADIENT");
+    ctx->c->tag_criterion = criterion;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_set_refinement(hipclaw_ctx* ctx, int smooth_refine, int smooth_level, int coarsen_delay)
+{
+    CTX_CHECK(ctx);
+    if (smooth_level < 0) return set_error(HIPCLAW_ERR_ARG, "smooth_level must be >= 0");
+    if (coarsen_delay != 0) return set_error(HIPCLAW_ERR_UNSUPPORTED, "coarsen_delay > 0 is not supported");
+    ctx->c->smooth_refine = smooth_refine != 0;
+    ctx->c->smooth_level = smooth_level;
     return HIPCLAW_OK;
 }
 
@@ -1275,9 +1341,16 @@ int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, i
     }
     int rc = hipclaw_qinit(ctx, qinit_kind, params, nparams, nsub);
     if (rc) return rc;
-    for (int pass = 0; pass < c.dom.maxlevel - c.dom.minlevel; pass++) {
-        rc = c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
-        if (rc) return rc;
+    // initial refinement (fclaw2d_initialize, src/fclaw2d_initialize.c:160-223):
+    // one tag and adapt pass per level from minlevel to maxlevel, new patches
+    // initialised by qinit, stopping at the first pass that changes nothing.
+    // The tags see the qinit values in the ghost cells unless init_ghostcell
+    // asks for ghost updates (the reference's option of that name).
+    for (int pass = c.dom.minlevel; pass <= c.dom.maxlevel; pass++) {
+        if (c.init_ghostcell) {
+            rc = c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
+            if (rc) return rc;
+        }
         int ch = 0;
         rc = c.tag_and_adapt(1, &ch);
         if (rc) return rc;
@@ -1285,6 +1358,7 @@ int hipclaw_initialize(hipclaw_ctx* ctx, int qinit_kind, const double* params, i
         rc = hipclaw_qinit(ctx, qinit_kind, params, nparams, nsub);
         if (rc) return rc;
     }
+    if (!c.init_ghostcell) return HIPCLAW_OK;
     return c.ghost_update(c.forest.global_minlevel(), c.forest.global_maxlevel(), 0);
 }
 

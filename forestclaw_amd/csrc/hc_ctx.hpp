@@ -74,9 +74,18 @@ struct Ctx {
     bool tsync = false;            // conservative time synchronisation on
     double* d_reg = nullptr;       // time-sync registers, regstride doubles per slot (FP64)
     long long regstride = 0;
-    // tagging parameters
+    // tagging parameters: clawpatch refinement-criteria (HIPCLAW_TAG_*),
+    // threshold variable and thresholds; the domain's refinement strategy
+    // (fclaw2d_domain_set_refinement: smooth-refine, smooth-level)
     double refine_threshold = 0.0, coarsen_threshold = 0.0;
     int tag_ivar = 0;
+    int tag_criterion = HIPCLAW_TAG_GRADIENT;
+    bool smooth_refine = false;
+    int smooth_level = 0;
+    double* d_tagst = nullptr;     // tag statistics, 4 doubles per slot
+    int tagst_cap = 0;
+    // ghost update after the initial refinement passes (init_ghostcell)
+    bool init_ghostcell = false;
     long long cell_updates = 0;    // sum over patch-steps of mx*my
     // HIP-event timing of the patch-step launches (on this context's stream)
     bool prof = false;

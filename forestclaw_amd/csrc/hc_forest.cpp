@@ -194,6 +194,37 @@ int Forest::corner_neighbor(int leaf, int corner, int* nb) const
     return NB_NONE;
 }
 
+std::vector<int> Forest::smooth_marks(const std::vector<int>& tags, int smooth_level) const
+{
+    const int n = nleaves();
+    if ((int)tags.size() != n) throw std::invalid_argument("tag count mismatch");
+    std::vector<int> tgt(n), out(n, 0);
+    for (int k = 0; k < n; k++) tgt[k] = leaves[k].level + (tags[k] > 0 ? 1 : tags[k] < 0 ? -1 : 0);
+    for (int k = 0; k < n; k++) {
+        const int level = leaves[k].level;
+        int mt = tgt[k];
+        auto take = [&](int nb) {
+            if (nb >= 0 && leaves[nb].level >= smooth_level) mt = std::max(mt, tgt[nb]);
+        };
+        for (int f = 0; mt <= level && f < 4; f++) {
+            int nb[2];
+            const int rel = face_neighbors(k, f, nb);
+            if (rel == NB_SAME || rel == NB_COARSE) take(nb[0]);
+            else if (rel == NB_FINE) {
+                take(nb[0]);
+                take(nb[1]);
+            }
+        }
+        for (int c = 0; mt <= level && c < 4; c++) {
+            int nb = -1;
+            corner_neighbor(k, c, &nb);
+            take(nb);
+        }
+        out[k] = mt < level ? -1 : (mt > level ? 1 : 0);
+    }
+    return out;
+}
+
 void Forest::adapt(const std::vector<int>& tags, std::vector<Transfer>& out)
 {
     if (tags.size() != leaves.size()) throw std::invalid_argument("tag count mismatch");

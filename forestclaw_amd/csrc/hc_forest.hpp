@@ -71,6 +71,11 @@ public:
     // restore 2:1 balance across faces and corners.  Fills the transfer list
     // describing how each new leaf is populated from the old forest.
     void adapt(const std::vector<int>& tags, std::vector<Transfer>& out);
+    // Refinement smoothing of fclaw2d_domain_adapt (src/fclaw2d_convenience.c:506-643):
+    // every leaf takes the largest target level (level + tag) over itself and
+    // its face and corner neighbours of level >= smooth_level; the result is
+    // refine (+1) if that exceeds its level, coarsen (-1) if it is below, else 0.
+    std::vector<int> smooth_marks(const std::vector<int>& tags, int smooth_level) const;
 
 private:
     std::unordered_map<uint64_t, int> index_;

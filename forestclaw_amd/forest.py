@@ -39,6 +39,9 @@ class Options:
     regrid_interval: int = 1
     refine_threshold: float = 0.5
     coarsen_threshold: float = 0.1
+    smooth_refine: bool = False
+    smooth_level: int = 0
+    init_ghostcell: bool = False
     subcycle: bool = True
     time_sync: bool = False
     max_cfl: float = 1.0
@@ -55,7 +58,8 @@ class Options:
     mx: int = 32
     my: int = 32
     mbc: int = 2
-    threshold_variable: int = 0
+    threshold_variable: int = 0   # 0-based (the reference's threshold-variable is 1-based)
+    refinement_criteria: str = "gradient"   # value | difference | minmax | gradient
     # [clawpack46]
     solver: int = H.SOLVER_SWE_ROE
     order: tuple = (2, 2)
@@ -89,10 +93,18 @@ class Hipclaw46:
         self.h = h
         _check(L.hipclaw_set_tagging(self.h, opt.threshold_variable, opt.refine_threshold,
                                      opt.coarsen_threshold))
+        _check(L.hipclaw_set_tag_criterion(self.h, H.TAG_CRITERIA[opt.refinement_criteria]))
+        _check(L.hipclaw_set_refinement(self.h, int(opt.smooth_refine), opt.smooth_level, 0))
+        _check(L.hipclaw_set_feature(self.h, H.FEATURE_INIT_GHOSTCELL, int(opt.init_ghostcell)))
         if opt.time_sync:
             _check(L.hipclaw_set_time_sync(self.h, 1))
         self.t = 0.0
         self.n_inner = 0
+        # the reference's regression counters (src/fclaw2d_advance.c:354,
+        # src/fclaw2d_regrid.c:315): advance calls (retaken steps included)
+        # and regrids that changed the mesh
+        self.count_amr_advance = 0
+        self.count_amr_new_domain = 0
 
     def close(self):
         if self.h:
@@ -246,11 +258,13 @@ class Hipclaw46:
     def advance(self, t: float, dt: float, want_cfl: bool = True) -> float:
         c = ctypes.c_double(0.0)
         _check(_lib().hipclaw_advance(self.h, t, dt, ctypes.byref(c) if want_cfl else None))
+        self.count_amr_advance += 1
         return c.value
 
     def regrid(self, init_flag: int = 0) -> bool:
         ch = ctypes.c_int()
         _check(_lib().hipclaw_regrid(self.h, init_flag, ctypes.byref(ch)))
+        self.count_amr_new_domain += int(ch.value != 0)
         return bool(ch.value)
 
     def mass(self) -> np.ndarray:

@@ -15,7 +15,10 @@ LIB_PATH = os.path.join(PKG, "libhipclaw.so")
 
 SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
 FP64, FP32 = 0, 1
-FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH = 1, 2, 3
+FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
+TAG_VALUE, TAG_DIFFERENCE, TAG_MINMAX, TAG_GRADIENT = 0, 1, 2, 3
+TAG_CRITERIA = {"value": TAG_VALUE, "difference": TAG_DIFFERENCE, "minmax": TAG_MINMAX,
+                "gradient": TAG_GRADIENT}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "unsupported",
           4: "out of device memory", 5: "call out of sequence"}
 
@@ -131,6 +134,8 @@ _SIGS.update({
     "hipclaw_regrid": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
     "hipclaw_initialize": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_mass": ([_VP, _DP], ctypes.c_int),
+    "hipclaw_set_tag_criterion": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_set_refinement": ([_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_output_ascii": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p], ctypes.c_int),
     "hipclaw_ascii_header": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                               ctypes.c_int], ctypes.c_int),

@@ -222,6 +222,11 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
 #define HIPCLAW_FEATURE_MERGE_LEVELS 1
 #define HIPCLAW_FEATURE_GHOST_PUSH 2
 #define HIPCLAW_FEATURE_GRAPH 3
+/*   INIT_GHOSTCELL (off by default, as the reference's init_ghostcell option,
+ *                 src/fclaw_options.c:186) ghost updates between the initial
+ *                 refinement passes of hipclaw_initialize and after them; off,
+ *                 the tags and the first step see the qinit ghost values */
+#define HIPCLAW_FEATURE_INIT_GHOSTCELL 4
 int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on);
 /* Coarse steps captured as graphs and replays of captured graphs so far. */
 int hipclaw_graph_stats(hipclaw_ctx* ctx, long long* captures, long long* replays);
@@ -276,9 +281,31 @@ int hipclaw_timeinterp(hipclaw_ctx* ctx, int level, double alpha);
  * the maximum Courant number over all level updates. */
 int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl);
 
-/* Gradient tagging on field ivar (fclaw2d_clawpatch_gradient_exceeds_th.f90). */
+/* Tagging variable (0-based field ivar, the reference's threshold-variable
+ * minus 1) and thresholds of the regrid tag pass. */
 int hipclaw_set_tagging(hipclaw_ctx* ctx, int ivar, double refine_threshold,
                         double coarsen_threshold);
+/* Refinement criterion (clawpatch option refinement-criteria,
+ * src/patches/clawpatch/fclaw2d_clawpatch_options.c:107-110, values of
+ * fclaw2d_clawpatch_options.h:48-54; the library default is GRADIENT):
+ *   VALUE       q > threshold at some cell, ghost cells included
+ *   DIFFERENCE  max(|dq_x|, |dq_y|) > threshold at some interior cell
+ *   MINMAX      max q - min q > threshold over the patch, ghost cells included
+ *               (coarsening: over the four siblings together)
+ *   GRADIENT    |grad q| > threshold at some interior cell
+ * (fort_user/fclaw2d_clawpatch_*_exceeds_th.f90 with the tag loops of
+ * fort_4.6/fclaw2d_clawpatch46_tag4refinement.f / _tag4coarsening.f). */
+#define HIPCLAW_TAG_VALUE 0
+#define HIPCLAW_TAG_DIFFERENCE 1
+#define HIPCLAW_TAG_MINMAX 2
+#define HIPCLAW_TAG_GRADIENT 3
+int hipclaw_set_tag_criterion(hipclaw_ctx* ctx, int criterion);
+/* fclaw2d_domain_set_refinement (src/forestclaw2d.c:1386-1397): refinement
+ * smoothing (the smooth-refine / smooth-level options; every patch takes the
+ * largest target level of itself and its face and corner neighbours of
+ * level >= smooth_level, src/fclaw2d_convenience.c:506-643).  coarsen_delay
+ * must be 0 (delayed coarsening is not built). */
+int hipclaw_set_refinement(hipclaw_ctx* ctx, int smooth_refine, int smooth_level, int coarsen_delay);
 /* fclaw2d_regrid (src/fclaw2d_regrid.c:234-335): tag, adapt with 2:1 balance,
  * repopulate (interpolate2fine / average2coarse), rebuild plans, ghost update.
  * init_flag = 1 disables coarsening as in the initial refinement. */

@@ -62,8 +62,10 @@ def _setup_sigs():
 
 class AmrOracle:
     def __init__(self, cfg: O.OrcCfg, leaves, geom, mthbc, brick=(1, 1), subcycle=True,
-                 swirl_tperiod=0.0, time_sync=False):
-        """leaves: list of (level, gx, gy); geom: array (n, 4) of xlower, ylower, dx, dy."""
+                 swirl_tperiod=0.0, time_sync=False, minlevel=None, maxlevel=None):
+        """leaves: list of (level, gx, gy); geom: array (n, 4) of xlower, ylower, dx, dy.
+        minlevel / maxlevel: the options' level range (default: the leaves' range),
+        from which the time-step counters count."""
         self.L = _setup_sigs()
         self.cfg = cfg
         self.mx, self.my, self.mbc, self.meqn = cfg.mx, cfg.my, cfg.mbc, cfg.meqn
@@ -80,6 +82,8 @@ class AmrOracle:
         self.qlast = [np.zeros(shape) for _ in range(n)]
         self.qti = [np.zeros(shape) for _ in range(n)]
         self.levels = sorted(set(l[0] for l in self.leaves))
+        self.minlevel = self.levels[0] if minlevel is None else minlevel
+        self.maxlevel = self.levels[-1] if maxlevel is None else maxlevel
         self.cell_updates = 0
         self.time_sync = time_sync
         self.reg = [np.zeros(O.reg_size(self.mx, self.my, self.meqn)) for _ in range(n)]
@@ -294,15 +298,22 @@ class AmrOracle:
                                       _p(self.qti[k]), alpha)
 
     def advance(self, t, dt):
+        """fclaw2d_advance_all_levels (src/fclaw2d_advance.c:240-357) with the
+        counters of initialize_timestep_counters (:54-118), which count from
+        the options' minlevel m0: dt is the step of level m0, level L steps
+        2^(L - m0) times (subcycling), or every level 2^(maxlevel - minlevel)
+        times (no subcycling)."""
         lo, hi = min(self.levels), max(self.levels)
+        m0 = min(self.minlevel, lo)
+        rf = 1 << max(self.maxlevel - self.minlevel, hi - lo)
         c = {}
         d = dt
-        for L in range(lo, hi + 1):
+        for L in range(m0, hi + 1):
             if self.subcycle:
-                c[L] = dict(last=0, inc=1 << (hi - L), total=1 << (L - lo), t=t, dt=d)
+                c[L] = dict(last=0, inc=1 << (hi - L), total=1 << (L - m0), t=t, dt=d)
                 d /= 2
             else:
-                c[L] = dict(last=0, inc=1, total=1 << (hi - lo), t=t, dt=dt / (1 << (hi - lo)))
+                c[L] = dict(last=0, inc=1, total=rf, t=t, dt=dt / rf)
         maxcfl = 0.0
 
         def advance_level(level, curr):
@@ -323,9 +334,14 @@ class AmrOracle:
                     tl = hi - 1
                     while c[hi]["last"] % c[tl]["inc"] == 0:
                         tl -= 1
-                    self.ghost_update(tl + 1, hi, 1)
-                    if self.time_sync:
-                        self.sync(tl + 1, hi)
+                    if tl < lo:  # all present levels synchronised
+                        self.ghost_update(lo, hi, 0)
+                        if self.time_sync:
+                            self.sync(lo, hi)
+                    else:
+                        self.ghost_update(tl + 1, hi, 1)
+                        if self.time_sync:
+                            self.sync(tl + 1, hi)
                 else:
                     self.ghost_update(lo, hi, 0)
                     if self.time_sync:

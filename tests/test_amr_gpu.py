@@ -35,7 +35,8 @@ def oracle_from(sim, solver, swirl=0.0):
     cfg = O.make_cfg(solver, sim.opt.mx, sim.opt.my, 2,
                      gamma=sim.opt.gamma, grav=sim.opt.grav)
     amr = A.AmrOracle(cfg, leaves, geom, sim.opt.mthbc, brick=(sim.opt.mi, sim.opt.mj),
-                      subcycle=sim.opt.subcycle, swirl_tperiod=swirl)
+                      subcycle=sim.opt.subcycle, swirl_tperiod=swirl, minlevel=sim.opt.minlevel,
+                      maxlevel=sim.opt.maxlevel)
     q = sim.get_q()
     for k in range(len(leaves)):
         amr.q[k][...] = q[k]
@@ -80,6 +81,33 @@ def test_amr_swe_subcycled_bitwise(dev, subcycle):
         assert c_gpu == c_orc
     assert np.array_equal(interiors(sim.get_q()), interiors(amr.q))
     assert sim.cell_updates() == amr.cell_updates
+
+
+@pytest.mark.parametrize("subcycle", [True, False])
+def test_coarsest_level_above_minlevel_bitwise(dev, subcycle):
+    """Options minlevel 1 / maxlevel 4 on a mesh of levels 2-3 only: dt is the
+    step of level 1, so level 2 takes two steps per coarse step with a fully
+    synchronised ghost update between them (subcycling), or every level takes
+    2^(4-1) steps (no subcycling), as the reference's counters give."""
+    sim = make(H.SOLVER_SWE_ROE, 8, 1, 4, refine=0.5, subcycle=subcycle)
+    lv, gx, gy = [], [], []
+    for y in range(4):
+        for x in range(4):
+            if x < 2 and y < 2:
+                for c in range(4):
+                    lv.append(3), gx.append(2 * x + (c & 1)), gy.append(2 * y + (c >> 1))
+            else:
+                lv.append(2), gx.append(x), gy.append(y)
+    sim.init_leaves(lv, gx, gy)
+    sim.qinit("radial_dam", SMALL_DAM, nsub=4)
+    sim.ghost_update(2, 3)
+    amr = oracle_from(sim, O.SOLVER_SWE_ROE)
+    dt = 0.02
+    for s in range(2):
+        assert sim.advance(s * dt, dt) == amr.advance(s * dt, dt)
+    assert np.array_equal(interiors(sim.get_q()), interiors(amr.q))
+    per_coarse = (8 * 8) * ((12 * 2 + 16 * 4) if subcycle else (28 * 8))
+    assert sim.cell_updates() == amr.cell_updates == 2 * per_coarse
 
 
 def test_euler_brick_bitwise(dev):

@@ -94,7 +94,8 @@ def test_amr_time_sync_bitwise(dev, subcycle):
     leaves = list(zip(info["level"], info["gx"], info["gy"]))
     geom = np.stack([info["xlower"], info["ylower"], info["dx"], info["dy"]], 1)
     cfg = O.make_cfg(O.SOLVER_SWE_ROE, 8, 8, 2, grav=1.0)
-    amr = A.AmrOracle(cfg, leaves, geom, (1, 1, 1, 1), subcycle=subcycle, time_sync=True)
+    amr = A.AmrOracle(cfg, leaves, geom, (1, 1, 1, 1), subcycle=subcycle, time_sync=True,
+                      minlevel=sim.opt.minlevel, maxlevel=sim.opt.maxlevel)
     q = sim.get_q()
     for k in range(len(leaves)):
         amr.q[k][...] = q[k]
