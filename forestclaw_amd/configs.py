@@ -153,3 +153,23 @@ def swirl_regression(precision=H.FP64):
     sim.close()
     return out
 
+
+def radialdam_regression(precision=H.FP64):
+    """applications/clawpack/shallow/2d/radialdam/regression.ini (example 0,
+    claw-version 4): 16x16 patches on [-2.5, 2.5]^2, levels 1-5, minmax tagging
+    0.5 / 0.25, smooth refinement from level 4, MC limiter, time-sync on, tfinal
+    0.4 from initial_dt 0.0625; the dam (r0 0.5, hin 2, hout 1) initialised by
+    the reference's polygon cell average (nsub 0)."""
+    opt = F.Options(minlevel=1, maxlevel=5, mx=16, my=16, solver=H.SOLVER_SWE_ROE, precision=precision,
+                    mthbc=(1, 1, 1, 1), mthlim=(4, 4, 4), order=(2, 2), refine_threshold=0.5,
+                    coarsen_threshold=0.25, smooth_refine=True, smooth_level=4, refinement_criteria="minmax",
+                    regrid_interval=1, use_fixed_dt=False, initial_dt=0.0625, max_cfl=1.0, desired_cfl=0.9,
+                    subcycle=True, time_sync=True, ax=-2.5, bx=2.5, ay=-2.5, by=2.5, grav=1.0)
+    sim = F.Hipclaw46(opt)
+    sim.initialize("radial_dam", (0.0, 0.0, 0.5, 2.0, 1.0), nsub=0)
+    sim.run(0.4, nout=1)
+    out = dict(count_amr_advance=sim.count_amr_advance, global_num_patches=sim.num_patches(),
+               count_amr_new_domain=sim.count_amr_new_domain)
+    sim.close()
+    return out
+

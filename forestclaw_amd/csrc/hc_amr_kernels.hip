@@ -362,6 +362,131 @@ k_tag_stats(const int* __restrict__ slots, const int* __restrict__ levels, int n
 // ---------------------------------------------------------------------------
 // Initial conditions (the applications' qinit hooks) on cell centres.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Cell average of the disc indicator by the reference's polygon rule
+// (cellave2, src/fortran_source2d/cellave2.f:1-95, with the radialdam
+// fdisc, applications/clawpack/shallow/2d/radialdam/fdisc.f): the cell
+// corners inside the disc and the crossings of the cell edges with the
+// circle, located by the zero finder zeroin (Brent's method in the form of
+// Forsythe, Malcolm and Moler, "Computer Methods for Mathematical
+// Computations", 1977, as the reference's copy has it, including its step
+// "b - xm" when xm < 0), bound a polygon whose area fraction is the weight.
+// FP64 with the reference's operation order (this file is built without FMA
+// contraction), so the values are the reference's.
+// ---------------------------------------------------------------------------
+struct Disc {
+    double x0, y0, r0;
+    __device__ double f(double x, double y) const
+    {
+        const double r2 = (x - x0) * (x - x0) + (y - y0) * (y - y0);
+        return r2 - r0 * r0;
+    }
+};
+
+__device__ double zeroin_edge(const Disc& dc, double xa, double ya, double xb, double yb, double tol)
+{
+    auto fs = [&](double s) { return dc.f(xa + s * (xb - xa), ya + s * (yb - ya)); };
+    double eps = 1.0, tol1;
+    do {
+        eps = eps / 2.0;
+        tol1 = 1.0 + eps;
+    } while (tol1 > 1.0);
+    double a = 0.0, b = 1.0, fa = fs(a), fb = fs(b), c, fc, d, e, xm, s, p, q, r;
+    bool restart = true;
+    for (int it = 0; it < 400; it++) {
+        if (restart) {
+            c = a;
+            fc = fa;
+            d = b - a;
+            e = d;
+        }
+        if (fabs(fc) < fabs(fb)) {
+            a = b;
+            b = c;
+            c = a;
+            fa = fb;
+            fb = fc;
+            fc = fa;
+        }
+        tol1 = 2.0 * eps * fabs(b) + 0.5 * tol;
+        xm = 0.5 * (c - b);
+        if (fabs(xm) <= tol1 || fb == 0.0) return b;
+        bool bisect = fabs(e) < tol1 || fabs(fa) <= fabs(fb);
+        if (!bisect) {
+            if (a == c) {
+                s = fb / fa;
+                p = 2.0 * xm * s;
+                q = 1.0 - s;
+            } else {
+                q = fa / fc;
+                r = fb / fc;
+                s = fb / fa;
+                p = s * (2.0 * xm * q * (q - r) - (b - a) * (r - 1.0));
+                q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+            }
+            if (p > 0.0) q = -q;
+            p = fabs(p);
+            if ((2.0 * p) >= (3.0 * xm * q - fabs(tol1 * q)) || p >= fabs(0.5 * e * q)) {
+                bisect = true;
+            } else {
+                e = d;
+                d = p / q;
+            }
+        }
+        if (bisect) {
+            d = xm;
+            e = d;
+        }
+        a = b;
+        fa = fb;
+        if (fabs(d) > tol1) b = b + d;
+        if (fabs(d) <= tol1) {
+            if (xm >= 0) b = b + tol1;
+            else b = b - xm;
+        }
+        fb = fs(b);
+        restart = (fb * (fc / fabs(fc))) > 0.0;
+    }
+    return b;
+}
+
+__device__ double cellave_disc(const Disc& dc, double xlow, double ylow, double dx, double dy)
+{
+    const double xx[5] = {xlow, xlow, xlow + dx, xlow + dx, xlow};
+    const double yy[5] = {ylow, ylow + dy, ylow + dy, ylow, ylow};
+    bool fl[5];
+    bool alll = true, allr = true;
+    for (int i = 0; i < 4; i++) {
+        fl[i] = dc.f(xx[i], yy[i]) < 0.0;
+        alll = alll && fl[i];
+        allr = allr && !fl[i];
+    }
+    fl[4] = fl[0];
+    if (alll) return 1.0;
+    if (allr) return 0.0;
+    double x[10], y[10];
+    int iv = 0;
+    for (int i = 0; i < 4; i++) {
+        if (fl[i]) {
+            x[iv] = xx[i];
+            y[iv] = yy[i];
+            iv++;
+        }
+        if (fl[i] != fl[i + 1]) {
+            const double ss = zeroin_edge(dc, xx[i], yy[i], xx[i + 1], yy[i + 1], 1e-14);
+            x[iv] = xx[i] + ss * (xx[i + 1] - xx[i]);
+            y[iv] = yy[i] + ss * (yy[i + 1] - yy[i]);
+            iv++;
+        }
+    }
+    if (iv == 0) return 0.0;
+    x[iv] = x[0];
+    y[iv] = y[0];
+    double area = 0.0;
+    for (int i = 0; i < iv; i++) area = area + 0.5 * (y[i] + y[i + 1]) * (x[i + 1] - x[i]);
+    return area / (dx * dy);
+}
+
 template <class R>
 __global__ void k_qinit(int kind, QinitParams prm, const int* __restrict__ slots,
                         const int* __restrict__ levels, int n, Pools<R> P,
@@ -387,6 +512,11 @@ __global__ void k_qinit(int kind, QinitParams prm, const int* __restrict__ slots
         v[0] = xc < prm.p[0] ? 1.0 : 0.0;
         break;
     case QINIT_RADIAL_DAM: {  // h = hin inside r0, hout outside; u = v = 0
+        if (prm.nsub == 0) {  // the reference's polygon cell average
+            const double w = cellave_disc(Disc{prm.p[0], prm.p[1], prm.p[2]}, xlo, ylo, dx, dy);
+            v[0] = prm.p[3] * w + prm.p[4] * (1.0 - w);
+            break;
+        }
         int inside = 0;
         for (int b = 0; b < S; b++)
             for (int a = 0; a < S; a++) {

@@ -1261,6 +1261,7 @@ int hipclaw_qinit(hipclaw_ctx* ctx, int kind, const double* params, int nparams,
     CTX_READY(ctx);
     if (kind < 0 || kind > QINIT_SHOCK_BUBBLE) return set_error(HIPCLAW_ERR_ARG, "unknown qinit kind");
     if (nparams < 0 || nparams > 56 || (nparams > 0 && !params)) return set_error(HIPCLAW_ERR_ARG, "bad qinit params");
+    if (nsub < 0 || nsub > 64) return set_error(HIPCLAW_ERR_ARG, "nsub must be 0..64");
     QinitParams prm{};
     for (int k = 0; k < nparams; k++) prm.p[k] = params[k];
     prm.nsub = nsub;

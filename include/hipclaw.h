@@ -260,7 +260,10 @@ int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q);
 int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod);
 int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux);
 
-/* patch_vt->initialize on every leaf. */
+/* patch_vt->initialize on every leaf.  Cell averages use nsub x nsub
+ * sub-cell samples; for HIPCLAW_QINIT_RADIAL_DAM, nsub = 0 selects the
+ * reference's exact polygon average (cellave2 with the radialdam fdisc,
+ * src/fortran_source2d/cellave2.f:1-95, radialdam/user_4.6/qinit.f:25-33). */
 int hipclaw_qinit(hipclaw_ctx* ctx, int kind, const double* params, int nparams, int nsub);
 
 /* fclaw2d_ghost_update (src/fclaw2d_ghost_fill.c:838-856): average, copy,
