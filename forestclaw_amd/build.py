@@ -40,6 +40,7 @@ SOURCES = {
     "hc_forest.cpp": STRICT,
     "hc_plan.cpp": STRICT,
     "hc_output.cpp": STRICT,
+    "hc_source.hip": STRICT,
 }
 
 

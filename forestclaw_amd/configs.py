@@ -131,9 +131,12 @@ def precision_report(name, checkpoints=(1, 10, 50)):
 
 # ---------------------------------------------------------------------------
 # The reference's regression cases (applications/clawpack/*/regressions.sh):
-# the options of each regression.ini, run to tfinal with the run loop of
+# the options in effect for each run, run to tfinal with the run loop of
 # fclaw2d_run outstyle 1, giving the three counters its regression_values CSV
-# files hold.
+# files hold.  The options in effect are the application's fclaw_options.ini
+# overlaid by regression.ini: every option package is registered with
+# fclaw_options.ini, which fclaw_app_options_parse loads before the command
+# line's -F regression.ini (src/fclaw_base.c:759-792).
 # ---------------------------------------------------------------------------
 def swirl_regression(precision=H.FP64):
     """applications/clawpack/advection/2d/swirl/regression.ini (claw-version 4):
@@ -156,13 +159,14 @@ def swirl_regression(precision=H.FP64):
 
 def radialdam_regression(precision=H.FP64):
     """applications/clawpack/shallow/2d/radialdam/regression.ini (example 0,
-    claw-version 4): 16x16 patches on [-2.5, 2.5]^2, levels 1-5, minmax tagging
-    0.5 / 0.25, smooth refinement from level 4, MC limiter, time-sync on, tfinal
-    0.4 from initial_dt 0.0625; the dam (r0 0.5, hin 2, hout 1) initialised by
-    the reference's polygon cell average (nsub 0)."""
+    claw-version 4): 16x16 patches on [-2.5, 2.5]^2, levels 1-5, gradient tagging
+    (refinement-criteria of fclaw_options.ini) 0.5 / 0.25, smooth refinement
+    from level 4, MC limiter, time-sync on, tfinal 0.4 from initial_dt 0.0625;
+    the dam (r0 0.5, hin 2, hout 1) initialised by the reference's polygon cell
+    average (nsub 0)."""
     opt = F.Options(minlevel=1, maxlevel=5, mx=16, my=16, solver=H.SOLVER_SWE_ROE, precision=precision,
                     mthbc=(1, 1, 1, 1), mthlim=(4, 4, 4), order=(2, 2), refine_threshold=0.5,
-                    coarsen_threshold=0.25, smooth_refine=True, smooth_level=4, refinement_criteria="minmax",
+                    coarsen_threshold=0.25, smooth_refine=True, smooth_level=4, refinement_criteria="gradient",
                     regrid_interval=1, use_fixed_dt=False, initial_dt=0.0625, max_cfl=1.0, desired_cfl=0.9,
                     subcycle=True, time_sync=True, ax=-2.5, bx=2.5, ay=-2.5, by=2.5, grav=1.0)
     sim = F.Hipclaw46(opt)
@@ -173,3 +177,26 @@ def radialdam_regression(precision=H.FP64):
     sim.close()
     return out
 
+
+
+def shockbubble_regression(precision=H.FP64):
+    """applications/clawpack/euler/2d/shockbubble/regression.ini (claw-version 4):
+    4x1 brick on [0, 2] x [0, 0.5], 16x16 patches, levels 0-3, difference
+    tagging (refinement-criteria of fclaw_options.ini) 0.5 / 0.2 on density,
+    smooth refinement, Euler + tracer Roe with limiters
+    MC x4 and superbee, walls at y = 0 (the axis), the axisymmetric source,
+    tfinal 0.25 from initial_dt 2.5e-3; the application's qinit (bubble r0 0.2
+    at (0.5, 0), rhoin 0.1, pinf 5, gamma 1.4)."""
+    opt = F.Options(minlevel=0, maxlevel=3, mx=16, my=16, mi=4, mj=1, ax=0.0, bx=2.0, ay=0.0, by=0.5,
+                    solver=H.SOLVER_EULER5_ROE, precision=precision, gamma=1.4, mthbc=(1, 1, 3, 1),
+                    mthlim=(4, 4, 4, 4, 2), order=(2, 2), refine_threshold=0.5, coarsen_threshold=0.2,
+                    smooth_refine=True, smooth_level=0, refinement_criteria="difference", regrid_interval=1,
+                    use_fixed_dt=False, initial_dt=2.5e-3, max_cfl=1.0, desired_cfl=0.9, subcycle=True,
+                    src2=H.SRC2_AXISYMMETRIC)
+    sim = F.Hipclaw46(opt)
+    sim.initialize("shockbubble_app", (1.4, 0.5, 0.0, 0.2, 0.1, 5.0), nsub=0)
+    sim.run(0.25, nout=1)
+    out = dict(count_amr_advance=sim.count_amr_advance, global_num_patches=sim.num_patches(),
+               count_amr_new_domain=sim.count_amr_new_domain)
+    sim.close()
+    return out

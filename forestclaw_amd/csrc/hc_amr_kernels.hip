@@ -567,6 +567,31 @@ __global__ void k_qinit(int kind, QinitParams prm, const int* __restrict__ slots
         for (int m = 0; m < 5; m++) v[m] = acc[m] / (S * S);
         break;
     }
+    case QINIT_SHOCKBUBBLE_APP: {
+        // applications/clawpack/euler/2d/shockbubble: setprob.f:33-58 (states),
+        // user_4.6/qinit.f:23-64 (bubble by cellave2 with the idisc = 2
+        // circle of fdisc.f:36-38, post-shock state left of x = 0.2, tracer
+        // doubled in the interior rows left of x = 0.5)
+        const double gamma = prm.p[0], gamma1 = gamma - 1.0;
+        const double rhoin = prm.p[4], pinf = prm.p[5];
+        const double qin[5] = {rhoin, 0.0, 0.0, 1.0 / gamma1, 1.0};
+        const double qout[5] = {1.0, 0.0, 0.0, 1.0 / gamma1, 0.0};
+        const double win = cellave_disc(Disc{prm.p[1], prm.p[2], prm.p[3]}, xlo, ylo, dx, dy);
+        for (int m = 0; m < 5; m++) v[m] = win * qin[m] + (1.0 - win) * qout[m];
+        if (xlo < 0.2) {
+            const double rinf = (gamma1 + (gamma + 1) * pinf) / ((gamma + 1) + gamma1 * pinf);
+            const double vinf = (1.0 / sqrt(gamma)) * (pinf - 1.0) /
+                                sqrt(0.5 * ((gamma + 1) / gamma) * pinf + 0.5 * gamma1 / gamma);
+            const double einf = 0.5 * rinf * vinf * vinf + pinf / gamma1;
+            v[0] = rinf;
+            v[1] = rinf * vinf;
+            v[2] = 0.0;
+            v[3] = einf;
+            v[4] = 1.0;
+        }
+        if (xlo < 0.5 && j >= MBC && j < MBC + my) v[4] = 2.0 * v[4];
+        break;
+    }
     default:
         v[0] = prm.p[0];
         break;

@@ -11,7 +11,8 @@ enum QinitKind {
     QINIT_SWIRL = 1,        // swirl/user_4.6/qinit.f: q = 1 for x < p[0]
     QINIT_RADIAL_DAM = 2,   // radialdam: h = p[3] inside circle (p[0],p[1]) radius p[2], else p[4]
     QINIT_HUMPS = 3,        // h = p[0] + sum of p[1] Gaussians (cx, cy, A, w) from p[2]
-    QINIT_SHOCK_BUBBLE = 4  // gamma, xshock, rho_inf, u_inf, p_inf, x0, y0, r0, rho_bubble
+    QINIT_SHOCK_BUBBLE = 4, // gamma, xshock, rho_inf, u_inf, p_inf, x0, y0, r0, rho_bubble
+    QINIT_SHOCKBUBBLE_APP = 5  // the shockbubble application: gamma, x0, y0, r0, rhoin, pinf
 };
 
 struct QinitParams {

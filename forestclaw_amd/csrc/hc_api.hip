@@ -43,8 +43,10 @@ int validate_physics(const hipclaw_physics* p)
         if (!(p->grav > 0)) return set_error(HIPCLAW_ERR_ARG, "grav must be > 0");
         break;
     case HIPCLAW_SOLVER_EULER5_ROE:
-        if (p->meqn != 5 || p->mwaves != 5 || p->maux != 0)
-            return set_error(HIPCLAW_ERR_ARG, "euler5 needs meqn=mwaves=5, maux=0");
+        // maux = 1 carries the radius of the axisymmetric source (the
+        // shock-bubble setaux); the step does not read it
+        if (p->meqn != 5 || p->mwaves != 5 || p->maux < 0 || p->maux > 1)
+            return set_error(HIPCLAW_ERR_ARG, "euler5 needs meqn=mwaves=5, maux=0|1");
         if (!(p->gamma > 1)) return set_error(HIPCLAW_ERR_ARG, "gamma must be > 1");
         break;
     default:

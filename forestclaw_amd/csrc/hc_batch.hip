@@ -47,6 +47,7 @@ struct hipclaw_batch {
     std::vector<Staged> staged;
     double t = 0, dt = 0;
     bool have_dt = false;
+    int src2 = HIPCLAW_SRC2_NONE;
 };
 
 namespace {
@@ -68,6 +69,8 @@ int flush(hipclaw_batch* b, double* maxcfl)
     if (!rc)
         rc = hipclaw_step2_batch(&b->phys, n, b->d_q, b->d_q, maux > 0 ? b->d_aux : nullptr, b->d_dxdy,
                                  nullptr, n, b->dt, b->d_cfl, nullptr, b->stream);
+    if (!rc && b->src2)
+        rc = hipclaw_src2_batch(&b->phys, b->src2, n, b->d_q, b->d_aux, nullptr, nullptr, b->t, b->dt, b->stream);
     if (!rc) rc = hipclaw_max_reduce(b->d_cfl, n, b->d_cfl + b->cap, b->stream);
     if (!rc) rc = check_hip(hipMemcpyAsync(b->h_q, b->d_q, qb, hipMemcpyDeviceToHost, b->stream), "batch q D2H");
     if (!rc)
@@ -146,6 +149,18 @@ int hipclaw_batch_destroy(hipclaw_batch* b)
     for (void* p : dp)
         if (p) (void)hipFree(p);
     delete b;
+    return HIPCLAW_OK;
+}
+
+int hipclaw_batch_set_src2(hipclaw_batch* b, int kind)
+{
+    if (!b) return set_error(HIPCLAW_ERR_ARG, "null batch");
+    if (kind != HIPCLAW_SRC2_NONE && kind != HIPCLAW_SRC2_AXISYMMETRIC)
+        return set_error(HIPCLAW_ERR_ARG, "unknown source term");
+    if (kind == HIPCLAW_SRC2_AXISYMMETRIC && (b->phys.maux < 1 || b->phys.meqn < 4))
+        return set_error(HIPCLAW_ERR_ARG, "axisymmetric source needs meqn >= 4 and the radius in aux field 1");
+    if (!b->staged.empty()) return set_error(HIPCLAW_ERR_STATE, "batch holds staged patches");
+    b->src2 = kind;
     return HIPCLAW_OK;
 }
 

@@ -323,6 +323,21 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
     return esz == 8 ? run(double(0)) : run(float(0));
 }
 
+Src2Args Ctx::src2_args(int s0, int np) const
+{
+    Src2Args a{};
+    a.n = np;
+    a.slot0 = s0;
+    a.xylower = d_xylower;
+    a.dxdy = d_dxdy;
+    a.mx = phys.mx;
+    a.my = phys.my;
+    a.meqn = phys.meqn;
+    a.maux = 0;
+    a.gamma1 = phys.gamma - 1.0;
+    return a;
+}
+
 int Ctx::update_level(int level, double t, double dt)
 {
     const int s0 = plan[level].s0, s1 = plan[level].s1;
@@ -347,6 +362,12 @@ int Ctx::update_level(int level, double t, double dt)
     else
         HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
                                    d_cfl + s0, lc, stream));
+    if (src2) {
+        Src2Args sa = src2_args(s0, np);
+        sa.q_l[0] = pool[parity[level] ^ 1];
+        sa.dtl[0] = dt;
+        HC_TRY(src2_launch(sa, src2, phys.precision, nullptr, stream));
+    }
     if (prof) {
         HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
         ev.push_back({e0, e1});
@@ -509,10 +530,19 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
     const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
-    const bool push = push_ghosts && !tsync && phys.mx >= 4 && phys.my >= 4;
+    const bool push = push_ghosts && !tsync && !src2 && phys.mx >= 4 && phys.my >= 4;
     ml.push = push ? d_push : nullptr;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
+    if (src2) {
+        Src2Args sa = src2_args(s0, np);
+        sa.levels = d_levels;
+        for (int L = lo; L <= hi; L++) {
+            sa.q_l[L] = ml.qout_l[L];
+            sa.dtl[L] = dtl[L];
+        }
+        HC_TRY(src2_launch(sa, src2, phys.precision, nullptr, stream));
+    }
     for (int L = lo; L <= hi; L++) pushed[L] = push;
     if (prof) {
         HC_TRY(check_hip(hipEventRecord(e1, stream), "event"));
@@ -980,6 +1010,18 @@ int hipclaw_set_time_sync(hipclaw_ctx* ctx, int on)
     return ctx->c->set_time_sync(on != 0);
 }
 
+int hipclaw_set_src2(hipclaw_ctx* ctx, int kind)
+{
+    CTX_CHECK(ctx);
+    if (kind != HIPCLAW_SRC2_NONE && kind != HIPCLAW_SRC2_AXISYMMETRIC)
+        return set_error(HIPCLAW_ERR_ARG, "unknown source term");
+    if (kind == HIPCLAW_SRC2_AXISYMMETRIC && ctx->c->phys.meqn < 4)
+        return set_error(HIPCLAW_ERR_ARG, "axisymmetric source needs the Euler solver");
+    ctx->c->src2 = kind;
+    ctx->c->clear_graphs();
+    return HIPCLAW_OK;
+}
+
 int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel)
 {
     CTX_READY(ctx);
@@ -1259,7 +1301,7 @@ int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod)
 int hipclaw_qinit(hipclaw_ctx* ctx, int kind, const double* params, int nparams, int nsub)
 {
     CTX_READY(ctx);
-    if (kind < 0 || kind > QINIT_SHOCK_BUBBLE) return set_error(HIPCLAW_ERR_ARG, "unknown qinit kind");
+    if (kind < 0 || kind > QINIT_SHOCKBUBBLE_APP) return set_error(HIPCLAW_ERR_ARG, "unknown qinit kind");
     if (nparams < 0 || nparams > 56 || (nparams > 0 && !params)) return set_error(HIPCLAW_ERR_ARG, "bad qinit params");
     if (nsub < 0 || nsub > 64) return set_error(HIPCLAW_ERR_ARG, "nsub must be 0..64");
     QinitParams prm{};

@@ -40,6 +40,21 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
                  double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,
                  const MultiLevel* ml);
 
+// Source term after the patch step (hc_source.hip) over the slots
+// [slot0, slot0 + n): q_l / dtl indexed by levels[slot] (levels NULL: index 0);
+// the radius from aux field 1 when aux is given, else from xylower / dxdy.
+struct Src2Args {
+    int n, slot0;
+    const int* levels;
+    void* q_l[STEP_MAXL];
+    double dtl[STEP_MAXL];
+    const double* xylower;
+    const double* dxdy;
+    int mx, my, meqn, maux;
+    double gamma1;
+};
+int src2_launch(const Src2Args& a, int kind, int precision, const void* aux, hipStream_t s);
+
 // fort.tXXXX / fort.qXXXX frame of n patches (host arrays, q in the tile
 // layout T*meqn per patch, mbc = 2); hc_output.cpp
 int write_frame(const char* dir, int iframe, double time, int mi, int mx, int my, int meqn, int maux, int n,

@@ -20,7 +20,7 @@ from . import hipclaw as H
 
 
 Domain = H.Domain
-QINIT = {"const": 0, "swirl": 1, "radial_dam": 2, "humps": 3, "shock_bubble": 4}
+QINIT = {"const": 0, "swirl": 1, "radial_dam": 2, "humps": 3, "shock_bubble": 4, "shockbubble_app": 5}
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _VP = ctypes.c_void_p
@@ -65,6 +65,7 @@ class Options:
     order: tuple = (2, 2)
     mthlim: tuple | None = None
     mthbc: tuple = (1, 1, 1, 1)
+    src2: int = H.SRC2_NONE       # the application's fort_src2 when src_term = 1
     grav: float = 1.0
     gamma: float = 1.4
     precision: int = H.FP64
@@ -98,6 +99,8 @@ class Hipclaw46:
         _check(L.hipclaw_set_feature(self.h, H.FEATURE_INIT_GHOSTCELL, int(opt.init_ghostcell)))
         if opt.time_sync:
             _check(L.hipclaw_set_time_sync(self.h, 1))
+        if opt.src2:
+            _check(L.hipclaw_set_src2(self.h, opt.src2))
         self.t = 0.0
         self.n_inner = 0
         # the reference's regression counters (src/fclaw2d_advance.c:354,

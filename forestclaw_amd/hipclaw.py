@@ -17,6 +17,7 @@ SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
 FP64, FP32 = 0, 1
 FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
 TAG_VALUE, TAG_DIFFERENCE, TAG_MINMAX, TAG_GRADIENT = 0, 1, 2, 3
+SRC2_NONE, SRC2_AXISYMMETRIC = 0, 1
 TAG_CRITERIA = {"value": TAG_VALUE, "difference": TAG_DIFFERENCE, "minmax": TAG_MINMAX,
                 "gradient": TAG_GRADIENT}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "unsupported",
@@ -99,6 +100,9 @@ _SIGS = {
                                 ctypes.c_int),
     "hipclaw_bc2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, ctypes.c_int, _VP,
                            _VP], ctypes.c_int),
+    "hipclaw_src2_batch": ([ctypes.POINTER(Physics), ctypes.c_int, ctypes.c_int, _VP, _VP, _VP, _VP,
+                            ctypes.c_double, ctypes.c_double, _VP], ctypes.c_int),
+    "hipclaw_batch_set_src2": ([_VP, ctypes.c_int], ctypes.c_int),
 }
 
 
@@ -110,6 +114,7 @@ _SIGS.update({
     "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
     "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
     "hipclaw_set_time_sync": ([_VP, ctypes.c_int], ctypes.c_int),
+    "hipclaw_set_src2": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_feature": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "hipclaw_graph_stats": ([_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
                             ctypes.c_int),
@@ -200,9 +205,11 @@ def _stream(stream):
 
 def physics(solver: int, mx: int, my: int, *, precision: int = FP64, order=(2, 2),
             mthlim=None, mthbc=(1, 1, 1, 1), grav: float = 1.0, gamma: float = 1.4,
-            efix: int = 1) -> Physics:
+            efix: int = 1, maux: int | None = None) -> Physics:
     p = Physics()
     _check(lib().hipclaw_physics_default(solver, mx, my, ctypes.byref(p)))
+    if maux is not None:
+        p.maux = maux
     p.precision = precision
     p.order[0], p.order[1] = order
     if mthlim is not None:
@@ -314,6 +321,9 @@ class Batch:
             dx, dy, t, dt, it, total, ctypes.byref(c)))
         return c.value
 
+    def set_src2(self, kind: int) -> None:
+        _check(lib().hipclaw_batch_set_src2(self.h, kind))
+
     def close(self):
         if self.h:
             lib().hipclaw_batch_destroy(self.h)
@@ -334,6 +344,12 @@ def swirl_aux_batch(phys: Physics, npatch: int, xylower, dxdy, t: float, dt: flo
                     tperiod: float, aux, stream=None) -> None:
     _check(lib().hipclaw_swirl_aux_batch(ctypes.byref(phys), npatch, _ptr(xylower), _ptr(dxdy),
                                          t, dt, tperiod, _ptr(aux), _stream(stream)))
+
+
+def src2_batch(phys: Physics, kind: int, npatch: int, q, aux=None, xylower=None, dxdy=None, t: float = 0.0,
+               dt: float = 0.0, stream=None) -> None:
+    _check(lib().hipclaw_src2_batch(ctypes.byref(phys), kind, npatch, _ptr(q), _ptr(aux), _ptr(xylower),
+                                    _ptr(dxdy), t, dt, _stream(stream)))
 
 
 def bc2_batch(phys: Physics, npatch: int, q, mthbc, patch_ids=None, stream=None,

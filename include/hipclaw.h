@@ -35,7 +35,9 @@ extern "C" {
  * fort_rpt2 function pointers, which cannot run on the device). */
 #define HIPCLAW_SOLVER_ADV 0        /* clawpack46_rpn2adv / rpt2adv (meqn 1, maux 2) */
 #define HIPCLAW_SOLVER_SWE_ROE 1    /* shallow-water Roe + entropy fix (meqn 3)      */
-#define HIPCLAW_SOLVER_EULER5_ROE 2 /* Euler + tracer Roe + entropy fix (meqn 5)     */
+#define HIPCLAW_SOLVER_EULER5_ROE 2 /* Euler + tracer Roe + entropy fix (meqn 5; maux
+                                       0, or 1 = the radius for the axisymmetric
+                                       source, not read by the step)             */
 
 #define HIPCLAW_FP64 0
 #define HIPCLAW_FP32 1
@@ -138,6 +140,10 @@ int hipclaw_batch_destroy(hipclaw_batch* b);
 int hipclaw_single_step_update(hipclaw_batch* b, double* q, const double* aux, double dx,
                                double dy, double t, double dt, int iter, int total_count,
                                double* maxcfl);
+/* The source term clawpack46_update applies after step2 (HIPCLAW_SRC2_*);
+ * the axisymmetric source reads the radius from aux field 1, so it needs
+ * maux >= 1 with the application's setaux values. */
+int hipclaw_batch_set_src2(hipclaw_batch* b, int kind);
 
 /* Maximum of n device doubles into *out_dev (device), the level CFL of
  * fclaw2d_update_single_step (src/fclaw2d_update_single_step.c:68). */
@@ -149,6 +155,19 @@ int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream
 int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const double* xylower,
                             const double* dxdy, double t, double dt, double tperiod,
                             void* aux, void* stream);
+
+/* Source terms (claw46_vt->fort_src2, called by clawpack46_update after
+ * step2 with the same t and dt when src_term > 0; reference
+ * src/solvers/fc2d_clawpack4.6/fc2d_clawpack46.cpp:186-215, 434-441).
+ * HIPCLAW_SRC2_AXISYMMETRIC: the shockbubble application's src2
+ * (applications/clawpack/euler/2d/shockbubble/user_4.6/src2.f): the
+ * axisymmetric Euler source about y = 0, radius = aux field 1 (its setaux:
+ * ylower + (j-1/2) dy) when aux is given, else computed from xylower / dxdy.
+ * Updates the interior cells of npatch tiles of q in place. */
+#define HIPCLAW_SRC2_NONE 0
+#define HIPCLAW_SRC2_AXISYMMETRIC 1
+int hipclaw_src2_batch(const hipclaw_physics* phys, int kind, int npatch, void* q, const void* aux,
+                       const double* xylower, const double* dxdy, double t, double dt, void* stream);
 
 /* Physical boundary conditions, clawpack46_bc2_default
  * (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_bc2_default.f:1-213),
@@ -184,6 +203,10 @@ typedef struct hipclaw_ctx hipclaw_ctx;
 #define HIPCLAW_QINIT_RADIAL_DAM 2   /* p = x0, y0, r0, h_in, h_out (cell averages)      */
 #define HIPCLAW_QINIT_HUMPS 3        /* p = h0, n, then n x (cx, cy, A, w)               */
 #define HIPCLAW_QINIT_SHOCK_BUBBLE 4 /* p = gamma, xs, rho, u, p post-shock, x0, y0, r, rho_b */
+/* The shockbubble application's qinit (euler/2d/shockbubble/user_4.6/qinit.f
+ * with setprob.f and the idisc = 2 fdisc): p = gamma, x0, y0, r0, rhoin, pinf;
+ * meqn = 5; the bubble is averaged by the polygon rule (nsub ignored). */
+#define HIPCLAW_QINIT_SHOCKBUBBLE_APP 5
 
 int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, void* stream,
                        hipclaw_ctx** out);
@@ -204,6 +227,11 @@ int hipclaw_num_patches(hipclaw_ctx* ctx, int* n);
  * and hipclaw_advance applies fclaw2d_time_sync after each synchronising ghost
  * update (reference src/fclaw2d_advance.c:280-341). */
 int hipclaw_set_time_sync(hipclaw_ctx* ctx, int on);
+/* Source term applied after every patch step ([clawpack46] src_term with the
+ * application's fort_src2; HIPCLAW_SRC2_*, see hipclaw_src2_batch).  The
+ * radius of the axisymmetric source comes from the patch geometry.  Turns
+ * the same-level ghost push off (pushed ghosts would predate the source). */
+int hipclaw_set_src2(hipclaw_ctx* ctx, int kind);
 /* fclaw2d_time_sync(glob, minlevel, maxlevel) (src/fclaw2d_time_sync.c:157-202):
  * same-size and fine-to-coarse corrections, then the register resets. */
 int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel);

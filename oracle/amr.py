@@ -24,6 +24,7 @@ import ctypes
 import numpy as np
 
 from . import oracle as O
+from . import source as SRC
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -62,10 +63,12 @@ def _setup_sigs():
 
 class AmrOracle:
     def __init__(self, cfg: O.OrcCfg, leaves, geom, mthbc, brick=(1, 1), subcycle=True,
-                 swirl_tperiod=0.0, time_sync=False, minlevel=None, maxlevel=None):
+                 swirl_tperiod=0.0, time_sync=False, minlevel=None, maxlevel=None, src2_gamma=None):
         """leaves: list of (level, gx, gy); geom: array (n, 4) of xlower, ylower, dx, dy.
         minlevel / maxlevel: the options' level range (default: the leaves' range),
-        from which the time-step counters count."""
+        from which the time-step counters count.  src2_gamma: apply the
+        shock-bubble axisymmetric source after every patch step
+        (clawpack46_update, fc2d_clawpack46.cpp:434-441) with this gamma."""
         self.L = _setup_sigs()
         self.cfg = cfg
         self.mx, self.my, self.mbc, self.meqn = cfg.mx, cfg.my, cfg.mbc, cfg.meqn
@@ -86,6 +89,7 @@ class AmrOracle:
         self.maxlevel = self.levels[-1] if maxlevel is None else maxlevel
         self.cell_updates = 0
         self.time_sync = time_sync
+        self.src2_gamma = src2_gamma
         self.reg = [np.zeros(O.reg_size(self.mx, self.my, self.meqn)) for _ in range(n)]
 
     # ---------------- topology ------------------------------------------
@@ -242,6 +246,7 @@ class AmrOracle:
             for k in ks:
                 cfl = max(cfl, O.step2_ts(self.cfg, self.q[k], self.geom[k, 2], self.geom[k, 3],
                                           dt, self.reg[k]))
+            self._src2(ks, dt)
             self.cell_updates += len(ks) * self.mx * self.my
             return cfl
         qb = np.stack([self.q[k] for k in ks])
@@ -252,8 +257,16 @@ class AmrOracle:
         cfl = O.step2_batch(self.cfg, qb, aux, self.geom[ks, 2], self.geom[ks, 3], dt)
         for i, k in enumerate(ks):
             self.q[k][...] = qb[i]
+        self._src2(ks, dt)
         self.cell_updates += len(ks) * self.mx * self.my
         return float(cfl.max())
+
+    def _src2(self, ks, dt):
+        if self.src2_gamma is None:
+            return
+        for k in ks:
+            rad = SRC.setaux_rad(self.mx, self.my, self.mbc, self.geom[k, 1], self.geom[k, 3])[0]
+            self.q[k][...] = SRC.src2_axisym(self.q[k], rad, mbc=self.mbc, dt=dt, gamma=self.src2_gamma)
 
     # ---------------- conservative time synchronisation ----------------
     def _reset(self, k, kinds):

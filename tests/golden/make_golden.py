@@ -122,8 +122,31 @@ def ascii_golden():
                                               for key in p})
 
 
+def src2_golden():
+    """The shock-bubble application's src2 and setaux (reference Fortran) on
+    random Euler states near the axis and away from it."""
+    rng = np.random.default_rng(SEED + 7)
+    out = {}
+    for c, (mx, my, ylower, dt) in enumerate([(8, 8, 0.0, 1e-3), (16, 12, 0.0, 4e-3), (6, 10, 0.3, 2e-2)]):
+        mbc = 2
+        dy = 0.5 / (4 * my)
+        aux = R.setaux_axisym(mx=mx, my=my, mbc=mbc, xlower=0.1, ylower=ylower, dx=dy, dy=dy)
+        rho = rng.uniform(0.1, 3.0, (my + 4, mx + 4))
+        u, v = rng.normal(0, 1.0, (2, my + 4, mx + 4))
+        p = rng.uniform(0.5, 5.0, (my + 4, mx + 4))
+        q = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v),
+                      rng.uniform(0, 2, (my + 4, mx + 4))])
+        qo = R.src2_axisym(q, aux, mx=mx, my=my, mbc=mbc, xlower=0.1, ylower=ylower, dx=dy, dy=dy, t=0.0,
+                           dt=dt, gamma=1.4)
+        out.update({f"q{c}": q, f"aux{c}": aux, f"out{c}": qo, f"geom{c}": np.array([ylower, dy, dt, 1.4])})
+    os.makedirs(os.path.join(HERE, "src2"), exist_ok=True)
+    save(os.path.join("src2", "src2"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["ascii"]:
         ascii_golden()
+    elif sys.argv[1:] == ["src2"]:
+        src2_golden()
     else:
         main()

@@ -13,6 +13,8 @@ Entry points driven (reference file:line):
   fclaw2d_clawpatch46_fort_timeinterp src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_timeinterp.f:11
   fclaw2d_clawpatch46_fort_header_ascii / _output_ascii
                          src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_output_ascii.f:13,51
+  clawpack46_src2 / clawpack46_setaux of the shock-bubble application
+                         applications/clawpack/euler/2d/shockbubble/user_4.6/src2.f:1, setaux.f:1
 Common blocks (/cparam/, /compi/, /comvt/) are written directly as data.
 """
 from __future__ import annotations
@@ -180,3 +182,32 @@ def ascii_frame(directory: str, iframe: int, time: float, meqn: int, maux: int, 
     finally:
         os.chdir(cwd)
 
+
+
+def src2_axisym(q: np.ndarray, aux: np.ndarray, *, mx, my, mbc, xlower, ylower, dx, dy, t, dt, gamma):
+    """The shock-bubble application's clawpack46_src2
+    (applications/clawpack/euler/2d/shockbubble/user_4.6/src2.f:1), on a copy."""
+    L = _lib("libref_src2.so")
+    arr = (_cd * 2).in_dll(L, "cparam_")
+    arr[0] = gamma
+    arr[1] = gamma - 1.0
+    meqn, maux = q.shape[0], aux.shape[0]
+    qf = np.ascontiguousarray(q, dtype=np.float64).copy()
+    af = np.ascontiguousarray(aux, dtype=np.float64)
+    L.clawpack46_src2_(
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(meqn)), ctypes.byref(_ci(mbc)),
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)),
+        ctypes.byref(_cd(dx)), ctypes.byref(_cd(dy)), _p(qf), ctypes.byref(_ci(maux)), _p(af),
+        ctypes.byref(_cd(t)), ctypes.byref(_cd(dt)))
+    return qf
+
+
+def setaux_axisym(*, mx, my, mbc, xlower, ylower, dx, dy):
+    """The shock-bubble application's clawpack46_setaux (user_4.6/setaux.f:1): rad."""
+    L = _lib("libref_src2.so")
+    aux = np.zeros((1, my + 2 * mbc, mx + 2 * mbc))
+    L.clawpack46_setaux_(
+        ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(mbc)), ctypes.byref(_ci(mx)),
+        ctypes.byref(_ci(my)), ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_cd(dx)),
+        ctypes.byref(_cd(dy)), ctypes.byref(_ci(1)), _p(aux))
+    return aux

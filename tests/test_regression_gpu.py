@@ -12,8 +12,9 @@ The radial-dam case starts from the reference's polygon cell average of the
 dam (cellave2, qinit nsub = 0), checked here bitwise against the oracle's
 restatement first.
 
-The shock-bubble case needs the axisymmetric source term and the inflow
-boundary routine of its application (src2, bc2 hooks), which are not built.
+The shock-bubble case adds the application's initial condition and its
+axisymmetric source term after every patch step (src2); with its mthbc
+(1 1 3 1) the application's bc2 reduces to the default routine.
 """
 import os
 
@@ -78,3 +79,9 @@ def test_radialdam_regression_counts(dev):
     got = CF.radialdam_regression()
     print("\nradialdam", got)
     assert got == expected("radialdam")
+
+
+def test_shockbubble_regression_counts(dev):
+    got = CF.shockbubble_regression()
+    print("\nshockbubble", got)
+    assert got == expected("shockbubble")
