@@ -136,9 +136,10 @@ def precision_report(name, checkpoints=(1, 10, 50)):
 # files hold.  The options in effect are the application's fclaw_options.ini
 # overlaid by regression.ini: every option package is registered with
 # fclaw_options.ini, which fclaw_app_options_parse loads before the command
-# line's -F regression.ini (src/fclaw_base.c:759-792).
+# line's -F regression.ini (src/fclaw_base.c:759-792).  claw_version = 5 runs
+# the same cases through the Clawpack 5 solver (regression_values_1.csv).
 # ---------------------------------------------------------------------------
-def swirl_regression(precision=H.FP64):
+def swirl_regression(precision=H.FP64, claw_version=4):
     """applications/clawpack/advection/2d/swirl/regression.ini (claw-version 4):
     8x8 patches, levels 2-6, minmax tagging 0.25 / 0.05, smooth refinement,
     van Leer limiter, swirl period 4, tfinal 0.1 from initial_dt 0.005."""
@@ -146,7 +147,7 @@ def swirl_regression(precision=H.FP64):
                     mthbc=(1, 1, 1, 1), mthlim=(3,), order=(2, 2), refine_threshold=0.25,
                     coarsen_threshold=0.05, smooth_refine=True, smooth_level=0,
                     refinement_criteria="minmax", regrid_interval=1, use_fixed_dt=False, initial_dt=0.005,
-                    max_cfl=1.0, desired_cfl=0.9, subcycle=True)
+                    max_cfl=1.0, desired_cfl=0.9, subcycle=True, claw_version=claw_version)
     sim = F.Hipclaw46(opt)
     sim.set_swirl(4.0)
     sim.initialize("swirl", (0.5,), nsub=1)
@@ -157,7 +158,7 @@ def swirl_regression(precision=H.FP64):
     return out
 
 
-def radialdam_regression(precision=H.FP64):
+def radialdam_regression(precision=H.FP64, claw_version=4):
     """applications/clawpack/shallow/2d/radialdam/regression.ini (example 0,
     claw-version 4): 16x16 patches on [-2.5, 2.5]^2, levels 1-5, gradient tagging
     (refinement-criteria of fclaw_options.ini) 0.5 / 0.25, smooth refinement
@@ -168,7 +169,8 @@ def radialdam_regression(precision=H.FP64):
                     mthbc=(1, 1, 1, 1), mthlim=(4, 4, 4), order=(2, 2), refine_threshold=0.5,
                     coarsen_threshold=0.25, smooth_refine=True, smooth_level=4, refinement_criteria="gradient",
                     regrid_interval=1, use_fixed_dt=False, initial_dt=0.0625, max_cfl=1.0, desired_cfl=0.9,
-                    subcycle=True, time_sync=True, ax=-2.5, bx=2.5, ay=-2.5, by=2.5, grav=1.0)
+                    subcycle=True, time_sync=True, ax=-2.5, bx=2.5, ay=-2.5, by=2.5, grav=1.0,
+                    claw_version=claw_version)
     sim = F.Hipclaw46(opt)
     sim.initialize("radial_dam", (0.0, 0.0, 0.5, 2.0, 1.0), nsub=0)
     sim.run(0.4, nout=1)
@@ -179,7 +181,7 @@ def radialdam_regression(precision=H.FP64):
 
 
 
-def shockbubble_regression(precision=H.FP64):
+def shockbubble_regression(precision=H.FP64, claw_version=4):
     """applications/clawpack/euler/2d/shockbubble/regression.ini (claw-version 4):
     4x1 brick on [0, 2] x [0, 0.5], 16x16 patches, levels 0-3, difference
     tagging (refinement-criteria of fclaw_options.ini) 0.5 / 0.2 on density,
@@ -192,7 +194,7 @@ def shockbubble_regression(precision=H.FP64):
                     mthlim=(4, 4, 4, 4, 2), order=(2, 2), refine_threshold=0.5, coarsen_threshold=0.2,
                     smooth_refine=True, smooth_level=0, refinement_criteria="difference", regrid_interval=1,
                     use_fixed_dt=False, initial_dt=2.5e-3, max_cfl=1.0, desired_cfl=0.9, subcycle=True,
-                    src2=H.SRC2_AXISYMMETRIC)
+                    src2=H.SRC2_AXISYMMETRIC, claw_version=claw_version)
     sim = F.Hipclaw46(opt)
     sim.initialize("shockbubble_app", (1.4, 0.5, 0.0, 0.2, 0.1, 5.0), nsub=0)
     sim.run(0.25, nout=1)

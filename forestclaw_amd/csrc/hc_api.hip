@@ -32,7 +32,8 @@ int validate_physics(const hipclaw_physics* p)
         return set_error(HIPCLAW_ERR_ARG, "precision must be HIPCLAW_FP64 or HIPCLAW_FP32");
     if (p->order[0] < 1 || p->order[0] > 2 || p->order[1] < 0 || p->order[1] > 2)
         return set_error(HIPCLAW_ERR_ARG, "order must be (1|2, 0|1|2)");
-    switch (p->solver) {
+    if ((p->solver & ~(HIPCLAW_CLAW5 | 0xff)) != 0) return set_error(HIPCLAW_ERR_ARG, "unknown solver flags");
+    switch (solver_family(p->solver)) {
     case HIPCLAW_SOLVER_ADV:
         if (p->meqn != 1 || p->mwaves != 1 || p->maux != 2)
             return set_error(HIPCLAW_ERR_ARG, "advection needs meqn=mwaves=1, maux=2");
@@ -169,7 +170,7 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
     out->grav = 1.0;
     out->gamma = 1.4;
     for (int f = 0; f < 4; f++) out->mthbc[f] = 1;
-    switch (solver) {
+    switch (solver_family(solver)) {
     case HIPCLAW_SOLVER_ADV: out->meqn = out->mwaves = 1; out->maux = 2; break;
     case HIPCLAW_SOLVER_SWE_ROE: out->meqn = out->mwaves = 3; break;
     case HIPCLAW_SOLVER_EULER5_ROE: out->meqn = out->mwaves = 5; break;
@@ -198,7 +199,7 @@ int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void
                              double* registers, const hipclaw_launch* launch, void* stream)
 {
     if (!registers) return set_error(HIPCLAW_ERR_ARG, "registers is NULL");
-    if (phys && phys->solver != HIPCLAW_SOLVER_SWE_ROE)
+    if (phys && solver_family(phys->solver) != HIPCLAW_SOLVER_SWE_ROE)
         return set_error(HIPCLAW_ERR_UNSUPPORTED, "time-sync registers need the SWE solver");
     return step2_batch_impl(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt,
                             cfl_patch, registers, launch, stream);
@@ -247,6 +248,11 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     a.order2 = phys->order[0];
     a.transverse = phys->order[1];
     for (int w = 0; w < 5; w++) a.mthlim[w] = w < phys->mwaves ? phys->mthlim[w] : 0;
+    // the Clawpack 5 inline limiter selects with a computed GO TO over five
+    // labels (clawpack5_inlinelimiter.f:59); 6 falls through to minmod
+    if (is_claw5(phys->solver))
+        for (int w = 0; w < 5; w++)
+            if (a.mthlim[w] == 6) a.mthlim[w] = 1;
     a.pp.grav = phys->grav;
     a.pp.gamma = phys->gamma;
     a.pp.efix = phys->efix;
@@ -295,7 +301,8 @@ int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const doubl
 {
     int rc = validate_physics(phys);
     if (rc) return rc;
-    if (phys->solver != HIPCLAW_SOLVER_ADV) return set_error(HIPCLAW_ERR_ARG, "swirl aux needs the advection solver");
+    if (solver_family(phys->solver) != HIPCLAW_SOLVER_ADV)
+        return set_error(HIPCLAW_ERR_ARG, "swirl aux needs the advection solver");
     if (npatch <= 0) return HIPCLAW_OK;
     if (tperiod == 0.0) return HIPCLAW_OK;  // velocities fixed by setaux
     const double pi2 = 8.0 * atan(1.0);

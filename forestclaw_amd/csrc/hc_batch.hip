@@ -20,6 +20,8 @@
 
 using hc::check_hip;
 using hc::set_error;
+using hc::is_claw5;
+using hc::patch_from_host;
 using hc::validate_physics;
 
 namespace {
@@ -84,6 +86,23 @@ int flush(hipclaw_batch* b, double* maxcfl)
     // interior cells back to the callers' arrays; their ghost cells stay as given
     const long long per = b->T * meqn;
     const int NX = b->phys.mx + 4, NY = b->phys.my + 4;
+    if (is_claw5(b->phys.solver)) {
+        for (int k = 0; k < n; k++) {
+            double* dst = b->staged[k].q;
+            for (int j = 2; j < NY - 2; j++)
+                for (int i = 2; i < NX - 2; i++)
+                    for (int m = 0; m < meqn; m++) {
+                        const long long o = ((long long)m * NY + j) * NX + i;
+                        dst[((long long)j * NX + i) * meqn + m] =
+                            b->esz == 8 ? static_cast<double*>(b->h_q)[k * per + o]
+                                        : (double)static_cast<float*>(b->h_q)[k * per + o];
+                    }
+        }
+        *maxcfl = *b->h_cfl;
+        b->staged.clear();
+        b->have_dt = false;
+        return HIPCLAW_OK;
+    }
     for (int k = 0; k < n; k++) {
         double* dst = b->staged[k].q;
         for (int m = 0; m < meqn; m++)
@@ -181,14 +200,16 @@ int hipclaw_single_step_update(hipclaw_batch* b, double* q, const double* aux, d
     b->have_dt = true;
     const int k = (int)b->staged.size();
     const long long per = b->T * b->phys.meqn, aper = b->T * b->phys.maux;
-    if (b->esz == 8) {
+    const bool c5 = is_claw5(b->phys.solver);
+    if (b->esz == 8 && !c5) {
         std::memcpy(static_cast<double*>(b->h_q) + k * per, q, sizeof(double) * per);
         if (aper) std::memcpy(static_cast<double*>(b->h_aux) + k * aper, aux, sizeof(double) * aper);
+    } else if (b->esz == 8) {
+        patch_from_host(q, static_cast<double*>(b->h_q) + k * per, b->phys.meqn, b->T, true);
+        if (aper) patch_from_host(aux, static_cast<double*>(b->h_aux) + k * aper, b->phys.maux, b->T, true);
     } else {
-        float* dq = static_cast<float*>(b->h_q) + k * per;
-        for (long long i = 0; i < per; i++) dq[i] = (float)q[i];
-        float* da = static_cast<float*>(b->h_aux) + k * aper;
-        for (long long i = 0; i < aper; i++) da[i] = (float)aux[i];
+        patch_from_host(q, static_cast<float*>(b->h_q) + k * per, b->phys.meqn, b->T, c5);
+        if (aper) patch_from_host(aux, static_cast<float*>(b->h_aux) + k * aper, b->phys.maux, b->T, c5);
     }
     b->h_dxdy[2 * k] = dx;
     b->h_dxdy[2 * k + 1] = dy;

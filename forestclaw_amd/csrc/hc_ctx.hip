@@ -346,7 +346,7 @@ int Ctx::update_level(int level, double t, double dt)
     char* qin = static_cast<char*>(pool[parity[level]]) + (size_t)s0 * tile * esz;
     char* qout = static_cast<char*>(pool[parity[level] ^ 1]) + (size_t)s0 * tile * esz;
     char* a = aux ? static_cast<char*>(aux) + (size_t)s0 * atile * esz : nullptr;
-    if (swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV)
+    if (swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV)
         HC_TRY(hipclaw_swirl_aux_batch(&phys, np, d_xylower + 2 * s0, d_dxdy + 2 * s0, t, dt,
                                        swirl_tperiod, a, stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -460,7 +460,7 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
 
 int Ctx::set_time_sync(bool on)
 {
-    if (on && phys.solver != HIPCLAW_SOLVER_SWE_ROE)
+    if (on && solver_family(phys.solver) != HIPCLAW_SOLVER_SWE_ROE)
         return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs a conservative flux (SWE only)");
     tsync = on;
     clear_graphs();
@@ -516,7 +516,7 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         ml.dtl[L] = dtl[L];
         ml.qin_l[L] = pool[parity[L]];
         ml.qout_l[L] = pool[parity[L] ^ 1];
-        if (swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV) {
+        if (swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV) {
             const int a0 = plan[L].s0, na = plan[L].s1 - a0;
             char* a = static_cast<char*>(aux) + (size_t)a0 * atile * esz;
             HC_TRY(hipclaw_swirl_aux_batch(&phys, na, d_xylower + 2 * a0, d_dxdy + 2 * a0, tl[L], dtl[L],
@@ -586,7 +586,7 @@ static unsigned level_bits(const bool* f)
 int Ctx::advance(double t, double dt, double* maxcfl)
 {
     const bool graph_ok = use_graph && stream != nullptr && !prof &&
-                          !(swirl_tperiod > 0 && phys.solver == HIPCLAW_SOLVER_ADV);
+                          !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV);
     if (!graph_ok) {
         HC_TRY(advance_body(t, dt));
         if (maxcfl) HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
@@ -1236,33 +1236,88 @@ static int level_copy(Ctx& c, int level, int n, double* const* q, const double* 
     return rc;
 }
 
+// Host patch arrays in the Clawpack 5 layout (HIPCLAW_CLAW5 solvers) go
+// through tile-layout copies.
+static std::vector<double> to_tiles(const double* host, int npatch, int nf, long long cells)
+{
+    std::vector<double> t((size_t)npatch * nf * cells);
+    for (int k = 0; k < npatch; k++)
+        patch_from_host(host + (size_t)k * nf * cells, t.data() + (size_t)k * nf * cells, nf, cells, true);
+    return t;
+}
+
 int hipclaw_level_upload(hipclaw_ctx* ctx, int level, int n, const double* const* q,
                          const double* const* aux)
 {
     CTX_READY(ctx);
-    return level_copy(*ctx->c, level, n, const_cast<double* const*>(q), aux, true);
+    Ctx& c = *ctx->c;
+    if (!is_claw5(c.phys.solver) || n <= 0 || !q)
+        return level_copy(c, level, n, const_cast<double* const*>(q), aux, true);
+    const long long cells = (long long)(c.phys.mx + 4) * (c.phys.my + 4);
+    std::vector<std::vector<double>> tq(n), ta;
+    std::vector<double*> pq(n), pa;
+    for (int k = 0; k < n; k++) {
+        if (!q[k]) return set_error(HIPCLAW_ERR_ARG, "a patch pointer is NULL");
+        tq[k] = to_tiles(q[k], 1, c.phys.meqn, cells);
+        pq[k] = tq[k].data();
+    }
+    if (aux && c.phys.maux > 0) {
+        ta.resize(n);
+        pa.resize(n);
+        for (int k = 0; k < n; k++) {
+            if (!aux[k]) return set_error(HIPCLAW_ERR_ARG, "an aux pointer is NULL");
+            ta[k] = to_tiles(aux[k], 1, c.phys.maux, cells);
+            pa[k] = ta[k].data();
+        }
+    }
+    return level_copy(c, level, n, pq.data(), pa.empty() ? nullptr : pa.data(), true);
 }
 
 int hipclaw_level_download(hipclaw_ctx* ctx, int level, int n, double* const* q)
 {
     CTX_READY(ctx);
-    return level_copy(*ctx->c, level, n, q, nullptr, false);
+    Ctx& c = *ctx->c;
+    if (!is_claw5(c.phys.solver) || n <= 0 || !q) return level_copy(c, level, n, q, nullptr, false);
+    const long long cells = (long long)(c.phys.mx + 4) * (c.phys.my + 4);
+    std::vector<std::vector<double>> tq(n);
+    std::vector<double*> pq(n);
+    for (int k = 0; k < n; k++) {
+        if (!q[k]) return set_error(HIPCLAW_ERR_ARG, "a patch pointer is NULL");
+        tq[k].resize((size_t)c.phys.meqn * cells);
+        pq[k] = tq[k].data();
+    }
+    int rc = level_copy(c, level, n, pq.data(), nullptr, false);
+    if (rc) return rc;
+    for (int k = 0; k < n; k++) patch_to_host(pq[k], q[k], c.phys.meqn, cells, true);
+    return HIPCLAW_OK;
 }
 
 int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q)
 {
     CTX_READY(ctx);
     if (!host_q) return set_error(HIPCLAW_ERR_ARG, "host_q is NULL");
-    return copy_q(*ctx->c, const_cast<double*>(host_q), true, false);
+    Ctx& c = *ctx->c;
+    if (!is_claw5(c.phys.solver)) return copy_q(c, const_cast<double*>(host_q), true, false);
+    std::vector<double> t = to_tiles(host_q, c.n, c.phys.meqn, (long long)(c.phys.mx + 4) * (c.phys.my + 4));
+    return copy_q(c, t.data(), true, false);
 }
 
 int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q)
 {
     CTX_READY(ctx);
     if (!host_q) return set_error(HIPCLAW_ERR_ARG, "host_q is NULL");
-    int rc = check_hip(hipStreamSynchronize(ctx->c->stream), "sync");
+    Ctx& c = *ctx->c;
+    int rc = check_hip(hipStreamSynchronize(c.stream), "sync");
     if (rc) return rc;
-    return copy_q(*ctx->c, host_q, false, false);
+    if (!is_claw5(c.phys.solver)) return copy_q(c, host_q, false, false);
+    const long long cells = (long long)(c.phys.mx + 4) * (c.phys.my + 4);
+    std::vector<double> t((size_t)c.n * c.phys.meqn * cells);
+    rc = copy_q(c, t.data(), false, false);
+    if (rc) return rc;
+    for (int k = 0; k < c.n; k++)
+        patch_to_host(t.data() + (size_t)k * c.phys.meqn * cells, host_q + (size_t)k * c.phys.meqn * cells,
+                      c.phys.meqn, cells, true);
+    return HIPCLAW_OK;
 }
 
 int hipclaw_output_ascii(hipclaw_ctx* ctx, int iframe, double time, const char* dir)
@@ -1274,8 +1329,9 @@ int hipclaw_output_ascii(hipclaw_ctx* ctx, int iframe, double time, const char* 
     std::vector<double> xl(n), yl(n), dx(n), dy(n);
     int rc = hipclaw_patch_info(ctx, lev.data(), gx.data(), gy.data(), xl.data(), yl.data(), dx.data(), dy.data());
     if (rc) return rc;
-    std::vector<double> q((size_t)n * c.tile);  // tile = elements per slot
-    rc = hipclaw_get_q(ctx, q.data());
+    std::vector<double> q((size_t)n * c.tile);  // tile = elements per slot, tile layout
+    rc = check_hip(hipStreamSynchronize(c.stream), "sync");
+    if (!rc) rc = copy_q(c, q.data(), false, false);
     if (rc) return rc;
     return write_frame(dir, iframe, time, c.dom.mi, c.phys.mx, c.phys.my, c.phys.meqn, c.phys.maux, n, lev.data(),
                        gx.data(), gy.data(), xl.data(), yl.data(), dx.data(), dy.data(), q.data());
@@ -1285,14 +1341,17 @@ int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux)
 {
     CTX_READY(ctx);
     if (!host_aux || ctx->c->phys.maux == 0) return set_error(HIPCLAW_ERR_ARG, "no aux for this solver");
-    ctx->c->swirl_tperiod = 0;
-    return copy_q(*ctx->c, const_cast<double*>(host_aux), true, true);
+    Ctx& c = *ctx->c;
+    c.swirl_tperiod = 0;
+    if (!is_claw5(c.phys.solver)) return copy_q(c, const_cast<double*>(host_aux), true, true);
+    std::vector<double> t = to_tiles(host_aux, c.n, c.phys.maux, (long long)(c.phys.mx + 4) * (c.phys.my + 4));
+    return copy_q(c, t.data(), true, true);
 }
 
 int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod)
 {
     CTX_CHECK(ctx);
-    if (ctx->c->phys.solver != HIPCLAW_SOLVER_ADV) return set_error(HIPCLAW_ERR_ARG, "swirl needs the advection solver");
+    if (solver_family(ctx->c->phys.solver) != HIPCLAW_SOLVER_ADV) return set_error(HIPCLAW_ERR_ARG, "swirl needs the advection solver");
     if (!(tperiod >= 0)) return set_error(HIPCLAW_ERR_ARG, "tperiod must be >= 0");
     ctx->c->swirl_tperiod = tperiod;
     return HIPCLAW_OK;

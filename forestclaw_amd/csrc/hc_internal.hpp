@@ -10,6 +10,34 @@
 
 namespace hc {
 
+// Solver id = family (HIPCLAW_SOLVER_*) | optional HIPCLAW_CLAW5 flag.
+inline int solver_family(int solver) { return solver & ~HIPCLAW_CLAW5; }
+inline bool is_claw5(int solver) { return (solver & HIPCLAW_CLAW5) != 0; }
+
+// Host patch arrays.  The Clawpack 4.6 layout q(i, j, m) is the library's
+// tile layout; the Clawpack 5 layout q(m, i, j) (fclaw2d_clawpatch5) has the
+// fields fastest.  nf fields over NX*NY cells.
+template <class T>
+inline void patch_from_host(const double* src, T* dst, int nf, long long cells, bool claw5)
+{
+    if (!claw5) {
+        for (long long i = 0; i < cells * nf; i++) dst[i] = (T)src[i];
+        return;
+    }
+    for (long long c = 0; c < cells; c++)
+        for (int m = 0; m < nf; m++) dst[m * cells + c] = (T)src[c * nf + m];
+}
+template <class T>
+inline void patch_to_host(const T* src, double* dst, int nf, long long cells, bool claw5)
+{
+    if (!claw5) {
+        for (long long i = 0; i < cells * nf; i++) dst[i] = (double)src[i];
+        return;
+    }
+    for (long long c = 0; c < cells; c++)
+        for (int m = 0; m < nf; m++) dst[c * nf + m] = (double)src[m * cells + c];
+}
+
 // Records a message for hipclaw_last_error() and returns code.
 int set_error(int code, const std::string& msg);
 int check_hip(hipError_t e, const char* what);

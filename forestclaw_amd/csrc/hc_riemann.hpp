@@ -12,6 +12,14 @@
 //           clawpack46_rpt2_euler5.f:48-118 (rpt2 recomputes the same Roe
 //           averages from the same two cells; they are reused here)
 //
+// The Clawpack 5 solvers of the same applications (C5 = true) differ only in
+// operation order, which the C5 instantiations follow:
+//   SWE     clawpack5_rpn2.f90:76,78 and clawpack5_rpt2.f90:59,61 scale by
+//           (0.5/a) where the 4.6 solvers divide by 2a;
+//   Euler5  clawpack5_rpt2_euler5.f90:67 takes gamma1/c**2 (c = sqrt of the
+//           4.6 a2) for the transverse split; its rpn2 and the Adv pair
+//           compute exactly what the 4.6 ones do.
+//
 // Naming: qa is the cell on the low side of the face ("qr(i-1)" in the
 // reference), qb the cell on the high side ("ql(i)").  DIR = 0 for an x-face,
 // 1 for a y-face (reference ixy = DIR+1).
@@ -76,7 +84,7 @@ template <class R> struct Adv {
 // ---------------------------------------------------------------------------
 // Shallow water, Roe solver with Harten-Hyman entropy fix.
 // ---------------------------------------------------------------------------
-template <class R> struct Swe {
+template <class R, bool C5> struct SweT {
     static constexpr int ME = 3, MW = 3, MAUX = 0;
     static constexpr bool HAS_CONS = true;
     using Face = FaceT<R, 3, 3>;
@@ -146,9 +154,16 @@ template <class R> struct Swe {
         R d0 = qb[0] - qa[0];
         R d1 = qb[mu] - qa[mu];
         R d2 = qb[mv] - qa[mv];
-        R a1 = MM::divp((u + a) * d0 - d1, R(2) * a);
+        R a1, a3;
         R a2 = -v * d0 + d2;
-        R a3 = MM::divp(-(u - a) * d0 + d1, R(2) * a);
+        if constexpr (C5) {
+            const R ha = MM::divp(R(0.5), a);
+            a1 = ((u + a) * d0 - d1) * ha;
+            a3 = (-(u - a) * d0 + d1) * ha;
+        } else {
+            a1 = MM::divp((u + a) * d0 - d1, R(2) * a);
+            a3 = MM::divp(-(u - a) * d0 + d1, R(2) * a);
+        }
 
         f.w[0][0] = a1; f.w[0][mu] = a1 * (u - a); f.w[0][mv] = a1 * v;
         f.s[0] = u - a;
@@ -220,9 +235,16 @@ template <class R> struct Swe {
         constexpr int mv = DIR == 0 ? 2 : 1;
         using MM = M<R>;
         const R u = roe.u, v = roe.v, a = roe.a;
-        R a1 = MM::divp((v + a) * asdq[0] - asdq[mv], R(2) * a);
+        R a1, a3;
         R a2 = asdq[mu] - u * asdq[0];
-        R a3 = MM::divp(-(v - a) * asdq[0] + asdq[mv], R(2) * a);
+        if constexpr (C5) {
+            const R ha = MM::divp(R(0.5), a);
+            a1 = ha * ((v + a) * asdq[0] - asdq[mv]);
+            a3 = ha * (-(v - a) * asdq[0] + asdq[mv]);
+        } else {
+            a1 = MM::divp((v + a) * asdq[0] - asdq[mv], R(2) * a);
+            a3 = MM::divp(-(v - a) * asdq[0] + asdq[mv], R(2) * a);
+        }
         R wb[3][3], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = R(0); wb[1][mu] = a2; wb[1][mv] = R(0); sb[1] = v;
@@ -241,7 +263,7 @@ template <class R> struct Swe {
 // ---------------------------------------------------------------------------
 // Euler equations + advected tracer, Roe solver with entropy fix.
 // ---------------------------------------------------------------------------
-template <class R> struct Euler5 {
+template <class R, bool C5> struct Euler5T {
     static constexpr int ME = 5, MW = 5, MAUX = 0;
     static constexpr bool HAS_CONS = false;
     using Face = FaceT<R, 5, 5>;
@@ -272,7 +294,9 @@ template <class R> struct Euler5 {
         R a = MM::sqrt_(a2c);
         R g1a2 = MM::divp(g1, a2c);
         R euv = enth - u2v2;
-        t.u = u; t.v = v; t.enth = enth; t.a = a; t.g1a2 = g1a2; t.euv = euv; t.u2v2 = u2v2;
+        t.u = u; t.v = v; t.enth = enth; t.a = a; t.euv = euv; t.u2v2 = u2v2;
+        if constexpr (C5) t.g1a2 = MM::divp(g1, a * a);  // transverse split only
+        else t.g1a2 = g1a2;
 
         R d0 = qb[0] - qa[0], d1 = qb[mu] - qa[mu], d2 = qb[mv] - qa[mv], d3 = qb[3] - qa[3];
         R a3 = g1a2 * (euv * d0 + u * d1 + v * d2 - d3);
@@ -395,6 +419,11 @@ template <class R> struct Euler5 {
         }
     }
 };
+
+template <class R> using Swe = SweT<R, false>;
+template <class R> using Swe5 = SweT<R, true>;
+template <class R> using Euler5 = Euler5T<R, false>;
+template <class R> using Euler5C5 = Euler5T<R, true>;
 
 // Limiter phi(r) of the reference's inline limiter
 // (src/solvers/fc2d_clawpack4.6/fortran_source/clawpack46_inlinelimiter.f:66-108).

@@ -63,9 +63,30 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
 }
 
 template <class R>
-static int launch_step2_t(const StepArgs& a, int solver, int cpl, int wpg, const void* qin,
+static int launch_step2_t(const StepArgs& a, int solver_id, int cpl, int wpg, const void* qin,
                           void* qout, const void* aux, double* cfl, hipStream_t s)
 {
+    const int solver = solver_family(solver_id);
+    // the Clawpack 5 solvers differ from the 4.6 ones only in FP64 rounding
+    // (operation order); the FP32 kernels serve both
+    if constexpr (sizeof(R) == 8) {
+        if (is_claw5(solver_id)) {
+            if (cpl == 1) {
+                switch (solver) {
+                case SOLVER_ADV: return launch_one<R, Adv, 1>(a, wpg, qin, qout, aux, cfl, s);
+                case SOLVER_SWE: return launch_one<R, Swe5, 1>(a, wpg, qin, qout, aux, cfl, s);
+                case SOLVER_EULER5: return launch_one<R, Euler5C5, 1>(a, wpg, qin, qout, aux, cfl, s);
+                }
+            } else if (cpl == 2) {
+                switch (solver) {
+                case SOLVER_ADV: return launch_one<R, Adv, 2>(a, wpg, qin, qout, aux, cfl, s);
+                case SOLVER_SWE: return launch_one<R, Swe5, 2>(a, wpg, qin, qout, aux, cfl, s);
+                case SOLVER_EULER5: return launch_one<R, Euler5C5, 2>(a, wpg, qin, qout, aux, cfl, s);
+                }
+            }
+            return set_error(HIPCLAW_ERR_ARG, "unsupported solver / cols_per_lane");
+        }
+    }
     if (cpl == 1) {
         switch (solver) {
         case SOLVER_ADV: return launch_one<R, Adv, 1>(a, wpg, qin, qout, aux, cfl, s);

@@ -66,6 +66,7 @@ class Options:
     mthlim: tuple | None = None
     mthbc: tuple = (1, 1, 1, 1)
     src2: int = H.SRC2_NONE       # the application's fort_src2 when src_term = 1
+    claw_version: int = 4         # [user] claw-version: 4 (fc2d_clawpack46) or 5 (fc2d_clawpack5)
     grav: float = 1.0
     gamma: float = 1.4
     precision: int = H.FP64
@@ -81,7 +82,10 @@ class Hipclaw46:
     def __init__(self, opt: Options, stream: int | None = None):
         self.opt = opt
         L = _lib()
-        self.phys = H.physics(opt.solver, opt.mx, opt.my, precision=opt.precision, order=opt.order,
+        if opt.claw_version not in (4, 5):
+            raise ValueError("claw_version must be 4 or 5")
+        solver = opt.solver | (H.CLAW5 if opt.claw_version == 5 else 0)
+        self.phys = H.physics(solver, opt.mx, opt.my, precision=opt.precision, order=opt.order,
                               mthlim=opt.mthlim, mthbc=opt.mthbc, grav=opt.grav, gamma=opt.gamma)
         self.dom = Domain(opt.mi, opt.mj, opt.ax, opt.bx, opt.ay, opt.by, opt.minlevel,
                           opt.maxlevel, 1 if opt.subcycle else 0)
@@ -123,7 +127,12 @@ class Hipclaw46:
     # ---- forest / data ----------------------------------------------------
     @property
     def tile_shape(self):
-        return (self.phys.meqn, self.opt.my + 2 * self.opt.mbc, self.opt.mx + 2 * self.opt.mbc)
+        """Shape of one host patch array: (meqn, NY, NX) for Clawpack 4.6,
+        q(i, j, m) in Fortran order; (NY, NX, meqn) for Clawpack 5, q(m, i, j)."""
+        ny, nx = self.opt.my + 2 * self.opt.mbc, self.opt.mx + 2 * self.opt.mbc
+        if self.opt.claw_version == 5:
+            return (ny, nx, self.phys.meqn)
+        return (self.phys.meqn, ny, nx)
 
     def num_patches(self) -> int:
         n = ctypes.c_int()

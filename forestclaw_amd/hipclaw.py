@@ -18,6 +18,7 @@ FP64, FP32 = 0, 1
 FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
 TAG_VALUE, TAG_DIFFERENCE, TAG_MINMAX, TAG_GRADIENT = 0, 1, 2, 3
 SRC2_NONE, SRC2_AXISYMMETRIC = 0, 1
+CLAW5 = 0x100   # solver flag: the Clawpack 5 solver and host layout q(meqn, i, j)
 TAG_CRITERIA = {"value": TAG_VALUE, "difference": TAG_DIFFERENCE, "minmax": TAG_MINMAX,
                 "gradient": TAG_GRADIENT}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "unsupported",

@@ -39,6 +39,16 @@ extern "C" {
                                        0, or 1 = the radius for the axisymmetric
                                        source, not read by the step)             */
 
+/* Flag OR-ed into the solver id: the Clawpack 5 variant of the solver family
+ * (the fc2d_clawpack5 solver, src/solvers/fc2d_clawpack5/, with the
+ * applications' clawpack5_rpn2 / rpt2).  Its FP64 arithmetic follows the
+ * Clawpack 5 solvers' operation order, and host patch arrays exchanged with
+ * the library (hipclaw_single_step_update, level upload / download,
+ * set_q / get_q, set_aux) use the Clawpack 5 layout q(meqn, i, j)
+ * (fclaw2d_clawpatch5 storage) instead of q(i, j, meqn).  Device buffers keep
+ * the library's tile layout either way. */
+#define HIPCLAW_CLAW5 0x100
+
 #define HIPCLAW_FP64 0
 #define HIPCLAW_FP32 1
 

@@ -112,9 +112,15 @@ static void rpn2_swe(const orc_cfg* c, int ixy, slice_t* sl)
         double d0 = Q1(0, k) - Q1(0, k - 1);
         double d1 = Q1(mu, k) - Q1(mu, k - 1);
         double d2 = Q1(mv, k) - Q1(mv, k - 1);
-        double a1 = ((u + a) * d0 - d1) / (2.0 * a);
+        double a1, a3;
         double a2 = -v * d0 + d2;
-        double a3 = (-(u - a) * d0 + d1) / (2.0 * a);
+        if (c->claw5) {
+            a1 = ((u + a) * d0 - d1) * (0.5 / a);
+            a3 = (-(u - a) * d0 + d1) * (0.5 / a);
+        } else {
+            a1 = ((u + a) * d0 - d1) / (2.0 * a);
+            a3 = (-(u - a) * d0 + d1) / (2.0 * a);
+        }
 
         WV(0, 0, k) = a1; WV(0, mu, k) = a1 * (u - a); WV(0, mv, k) = a1 * v;
         SP(0, k) = u - a;
@@ -351,16 +357,22 @@ static void rpt2_adv(const orc_cfg* c, int ixy, int imp, const double* asdq,
 static void rpt2_swe(const orc_cfg* c, int ixy, int imp, const double* asdq,
                      slice_t* sl)
 {
-    (void)c; (void)imp;
+    (void)imp;
     const int mu = (ixy == 1) ? 1 : 2;
     const int mv = (ixy == 1) ? 2 : 1;
     const int N = sl->N;
     for (int k = 1; k < N; k++) {
         double u = ROE(0, k), v = ROE(1, k), a = ROE(2, k);
         double q0 = asdq[0 * N + k], qu = asdq[mu * N + k], qv = asdq[mv * N + k];
-        double a1 = ((v + a) * q0 - qv) / (2 * a);
+        double a1, a3;
         double a2 = qu - u * q0;
-        double a3 = (-(v - a) * q0 + qv) / (2 * a);
+        if (c->claw5) {
+            a1 = (0.5 / a) * ((v + a) * q0 - qv);
+            a3 = (0.5 / a) * (-(v - a) * q0 + qv);
+        } else {
+            a1 = ((v + a) * q0 - qv) / (2 * a);
+            a3 = (-(v - a) * q0 + qv) / (2 * a);
+        }
         double wb[3][3], sb[3];
         wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a); sb[0] = v - a;
         wb[1][0] = 0.0; wb[1][mu] = a2; wb[1][mv] = 0.0; sb[1] = v;
@@ -380,7 +392,7 @@ static void rpt2_swe(const orc_cfg* c, int ixy, int imp, const double* asdq,
 static void rpt2_euler5(const orc_cfg* c, int ixy, int imp, const double* asdq,
                         slice_t* sl)
 {
-    (void)c; (void)imp;
+    (void)imp;
     const int mu = (ixy == 1) ? 1 : 2;
     const int mv = (ixy == 1) ? 2 : 1;
     const int N = sl->N;
@@ -389,6 +401,7 @@ static void rpt2_euler5(const orc_cfg* c, int ixy, int imp, const double* asdq,
          * the same formulas as rpn2_euler5; the cached values are identical. */
         double u = ROE(0, k), v = ROE(1, k), enth = ROE(2, k), a = ROE(3, k),
                g1a2 = ROE(4, k), euv = ROE(5, k), u2v2 = ROE(6, k);
+        if (c->claw5) g1a2 = (c->gamma - 1.0) / (a * a);
         double s0 = asdq[0 * N + k], su = asdq[mu * N + k], sv = asdq[mv * N + k],
                s3 = asdq[3 * N + k], s4 = asdq[4 * N + k];
         double a3 = g1a2 * (euv * s0 + u * su + v * sv - s3);
@@ -478,7 +491,8 @@ static void limiter(const orc_cfg* c, slice_t* sl)
             if (k == sl->mbc - 1) continue;
             if (wnorm2 == 0.0) continue;
             double r = (SP(w, k) > 0.0) ? dotl / wnorm2 : dotr / wnorm2;
-            double phi = orc_limiter_phi(c->mthlim[w], r);
+            /* Clawpack 5: computed GO TO over 1..5, 6 falls through to minmod */
+            double phi = orc_limiter_phi(c->claw5 && c->mthlim[w] == 6 ? 1 : c->mthlim[w], r);
             for (int m = 0; m < sl->meqn; m++) WV(w, m, k) = phi * WV(w, m, k);
         }
     }

@@ -33,6 +33,10 @@ typedef struct {
     int efix;          /* entropy fix (reference: always on, data efix /.true./)*/
     double grav;       /* /cparam/ grav  (SWE)                                  */
     double gamma;      /* /cparam/ gamma (Euler), gamma1 = gamma - 1            */
+    int claw5;         /* 1: the Clawpack 5 solvers' operation order (SWE:
+                          clawpack5_rpn2.f90:76,78, clawpack5_rpt2.f90:59,61
+                          scale by 0.5/a; Euler5: clawpack5_rpt2_euler5.f90:67
+                          takes gamma1/c**2 for the transverse split)           */
 } orc_cfg;
 
 /* One patch step in place (clawpack46_step2_wrap semantics, mcapa = 0).

@@ -30,6 +30,7 @@ class OrcCfg(ctypes.Structure):
         ("mthlim", ctypes.c_int * 8),
         ("efix", ctypes.c_int),
         ("grav", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("claw5", ctypes.c_int),
     ]
 
 
@@ -70,7 +71,7 @@ def _dp(a: np.ndarray):
 
 def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = None,
              order=(2, 2), mthlim=None, grav: float = 1.0, gamma: float = 1.4,
-             efix: int = 1) -> OrcCfg:
+             efix: int = 1, claw5: bool = False) -> OrcCfg:
     meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5}[solver]
     mwaves = meqn
     if maux is None:
@@ -84,6 +85,7 @@ def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = 
         c.mthlim[k] = lim[k] if k < len(lim) else 0
     c.efix = efix
     c.grav, c.gamma = grav, gamma
+    c.claw5 = int(claw5)
     return c
 
 

@@ -143,10 +143,53 @@ def src2_golden():
     save(os.path.join("src2", "src2"), **out)
 
 
+def claw5_golden():
+    """The Clawpack 5 patch step (clawpack5_step2_wrap with the applications'
+    clawpack5_rpn2 / rpt2), inputs and outputs in the Clawpack 5 layout
+    q(meqn, i, j) (numpy (NY, NX, meqn)), including limiter 6, which that
+    step's limiter treats as minmod."""
+    if not R.available5():
+        raise SystemExit("oracle/_ref Clawpack 5 libraries not built: run `make -C oracle ref` first")
+    rng = np.random.default_rng(SEED + 5)
+    t5 = lambda a: np.ascontiguousarray(np.transpose(a, (1, 2, 0)))  # noqa: E731
+    out = {}
+    k = 0
+    for phys, (mx, my), lim in [("swe", (16, 12), (4, 4, 4)), ("swe", (16, 16), (5, 6, 3)),
+                                ("euler", (16, 16), (4, 4, 4, 4, 2)), ("euler", (12, 16), (1, 2, 6, 4, 3)),
+                                ("adv", (16, 16), (4,)), ("adv", (12, 12), (6,))]:
+        mbc = 2
+        aux = None
+        if phys == "swe":
+            q, dx, dy = C.swe_state(rng, mx, my, mbc)
+            dt = C.swe_dt(q, dx, dy)
+            R.set_params5("swe", grav=1.0)
+        elif phys == "euler":
+            q, dx, dy = C.euler_state(rng, mx, my, mbc)
+            dt = C.euler_dt(q, dx, dy)
+            R.set_params5("euler", gamma=1.4)
+        else:
+            q, dx, dy = C.adv_state(rng, mx, my, mbc)
+            aux = t5(R.swirl_aux(mx=mx, my=my, mbc=mbc, xlower=0.0, ylower=0.0, dx=dx, dy=dy, t=0.4,
+                                 dt=0.01, tperiod=8.0))
+            dt = C.adv_dt(np.transpose(aux, (2, 0, 1)), dx, dy)
+        q5 = t5(q)
+        qo, cfl = R.step2_claw5(phys, q5, aux, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt, mthlim=lim)
+        out.update({f"q{k}": q5, f"qout{k}": qo, f"cfl{k}": cfl, f"dims{k}": np.array([mx, my]),
+                    f"geom{k}": np.array([dx, dy, dt]), f"mthlim{k}": np.array(lim),
+                    f"phys{k}": np.array(phys)})
+        if aux is not None:
+            out[f"aux{k}"] = aux
+        k += 1
+    os.makedirs(os.path.join(HERE, "claw5"), exist_ok=True)
+    save(os.path.join("claw5", "step2_claw5"), **out)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["ascii"]:
         ascii_golden()
     elif sys.argv[1:] == ["src2"]:
         src2_golden()
+    elif sys.argv[1:] == ["claw5"]:
+        claw5_golden()
     else:
         main()

@@ -211,3 +211,85 @@ def setaux_axisym(*, mx, my, mbc, xlower, ylower, dx, dy):
         ctypes.byref(_ci(my)), ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_cd(dx)),
         ctypes.byref(_cd(dy)), ctypes.byref(_ci(1)), _p(aux))
     return aux
+
+
+RP5 = {  # physics -> (library, rpn2 symbol, rpt2 symbol) of the Clawpack 5 step
+    "adv": ("libref5_adv.so", "clawpack5_rpn2adv_", "clawpack5_rpt2adv_"),
+    "swe": ("libref5_swe.so", "clawpack5_rpn2_", "clawpack5_rpt2_"),
+    "euler": ("libref5_euler.so", "clawpack5_rpn2_euler5_", "clawpack5_rpt2_euler5_"),
+}
+
+
+def available5() -> bool:
+    return all(os.path.exists(os.path.join(REF_DIR, RP5[k][0])) for k in RP5)
+
+
+def set_params5(physics: str, *, grav=1.0, gamma=1.4, tperiod=4.0):
+    L = _lib(RP5[physics][0])
+    if physics == "swe":
+        _cd.in_dll(L, "cparam_").value = grav
+    elif physics == "euler":
+        _cd.in_dll(L, "cparam_").value = gamma
+    else:
+        pi = 4.0 * math.atan(1.0)
+        arr = (_cd * 2).in_dll(L, "compi_")
+        arr[0] = pi
+        arr[1] = 2.0 * pi
+        _cd.in_dll(L, "comvt_").value = tperiod
+
+
+def step2_claw5(physics: str, q5: np.ndarray, aux5: np.ndarray | None, *, mx, my, mbc, dx, dy, dt,
+                order=(2, 2), mthlim=None, xlower=0.0, ylower=0.0, t=0.0):
+    """Run clawpack5_step2_wrap (src/solvers/fc2d_clawpack5/fortran_source/
+    clawpack5_step2_wrap.f:1) on a copy of q5, in the Clawpack 5 layout
+    q(meqn, i, j): numpy shape (NY, NX, meqn).  The module parameters are set
+    through clawpack5_set_amr_module as fc2d_clawpack5.cpp does.  Returns
+    (q_after, cfl)."""
+    libname, rpn2_sym, rpt2_sym = RP5[physics]
+    L = _lib(libname)
+    meqn = q5.shape[2]
+    mwaves = meqn
+    maux = 0 if aux5 is None else aux5.shape[2]
+    qf = np.ascontiguousarray(q5, dtype=np.float64).copy()
+    auxf = (np.zeros(1) if aux5 is None else np.ascontiguousarray(aux5, dtype=np.float64).copy())
+    maxm = max(mx, my)
+    mwork = (maxm + 2 * mbc) * (12 * meqn + (meqn + 1) * mwaves + 3 * maux + 2)
+    work = np.zeros(mwork)
+    size = meqn * (mx + 2 * mbc) * (my + 2 * mbc)
+    fp, fm, gp, gm = (np.zeros(size) for _ in range(4))
+    method = (_ci * 7)(0, order[0], order[1], 0, 0, 0, 0)
+    lim = list(mthlim) if mthlim is not None else [4] * mwaves
+    mthlim_c = (_ci * mwaves)(*lim)
+    L.clawpack5_set_amr_module_(ctypes.byref(_ci(mwaves)), ctypes.byref(_ci(0)), mthlim_c, method,
+                                ctypes.byref(_ci(0)))
+    bcc = (_ci * 4)(0, 0, 0, 0)
+    cfl = _cd(0.0)
+    ierr = _ci(0)
+    rpn2 = ctypes.cast(getattr(L, rpn2_sym), ctypes.c_void_p)
+    rpt2 = ctypes.cast(getattr(L, rpt2_sym), ctypes.c_void_p)
+    flux2 = ctypes.cast(L.clawpack5_flux2_, ctypes.c_void_p)
+    L.clawpack5_step2_wrap_(
+        ctypes.byref(_ci(maxm)), ctypes.byref(_ci(meqn)), ctypes.byref(_ci(maux)),
+        ctypes.byref(_ci(mbc)), method, mthlim_c, ctypes.byref(_ci(0)),
+        ctypes.byref(_ci(mwaves)), ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)),
+        _p(qf), _p(auxf), ctypes.byref(_cd(dx)), ctypes.byref(_cd(dy)),
+        ctypes.byref(_cd(dt)), ctypes.byref(cfl), _p(work), ctypes.byref(_ci(mwork)),
+        ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_ci(0)),
+        ctypes.byref(_cd(t)), _p(fp), _p(fm), _p(gp), _p(gm), rpn2, rpt2, flux2, bcc, ctypes.byref(ierr))
+    if ierr.value != 0:
+        raise RuntimeError(f"clawpack5_step2_wrap ierror={ierr.value}")
+    return qf, cfl.value
+
+
+def swirl_aux5(*, mx, my, mbc, xlower, ylower, dx, dy, t, dt, tperiod):
+    """The swirl application's Clawpack 5 b4step2 (user_5.0/b4step2.f90:1):
+    aux in the (NY, NX, maux) layout."""
+    L = _lib("libref5_adv.so")
+    set_params5("adv", tperiod=tperiod)
+    aux = np.zeros((my + 2 * mbc, mx + 2 * mbc, 2))
+    q = np.zeros((my + 2 * mbc, mx + 2 * mbc, 1))
+    L.clawpack5_b4step2_(
+        ctypes.byref(_ci(mbc)), ctypes.byref(_ci(mx)), ctypes.byref(_ci(my)), ctypes.byref(_ci(1)), _p(q),
+        ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_cd(dx)), ctypes.byref(_cd(dy)),
+        ctypes.byref(_cd(t)), ctypes.byref(_cd(dt)), ctypes.byref(_ci(2)), _p(aux))
+    return aux
