@@ -624,6 +624,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         // ---- interior cell row R = J-2 ------------------------------------
         const int Rw = J - 2;
         if (Rw >= 2 && Rw <= NY - 3) {
+            // neighbour terms by DPP from every lane first, then one branch
+            // per column for the update and its stores
+            R e1rv[CPL][ME], e2lv[CPL][ME];
 #pragma unroll
             for (int m = 0; m < ME; m++) {
                 R t1[CPL], t2[CPL];
@@ -634,11 +637,17 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
 #pragma unroll
                 for (int k = 0; k < CPL; k++) {
-                    R e1r = from_right<CPL>(t1, k);
-                    R e2l = from_left<CPL>(t2, k);
-                    if (cv[k] && col[k] >= 2 && col[k] <= NX - 3) {
-                        R fmr = (fmx2[k][m] + e2[k][m]) + e1r;
-                        R fpc = (fpx2[k][m] + e2l) + e1[k][m];
+                    e1rv[k][m] = from_right<CPL>(t1, k);
+                    e2lv[k][m] = from_left<CPL>(t2, k);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                if (cv[k] && col[k] >= 2 && col[k] <= NX - 3) {
+#pragma unroll
+                    for (int m = 0; m < ME; m++) {
+                        R fmr = (fmx2[k][m] + e2[k][m]) + e1rv[k][m];
+                        R fpc = (fpx2[k][m] + e2lv[k][m]) + e1[k][m];
                         R gmu = G1[k][m] + fdm[k][m];
                         R gpc = G2[k][m] + fdp_prev[k][m];
                         R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
@@ -655,6 +664,15 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                                 stage[4 * ME * A.my + ((Jo - NY + 4) * ME + m) * A.mx + (c - 2)] = qn;
                         }
                     }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < ME; m++) {
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const R e1r = e1rv[k][m], e2l = e2lv[k][m];
+                    (void)e1r;
+                    (void)e2l;
                     if constexpr (SYNC) {
                         // fluctuations on the patch edges (accumulate_waves)
                         double* rg = A.reg + (long long)tix * A.regstride;
