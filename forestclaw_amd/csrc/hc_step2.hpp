@@ -644,6 +644,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
                 if (cv[k] && col[k] >= 2 && col[k] <= NX - 3) {
+                    // push staging slots of this cell: x-edge column (two
+                    // deep on each side) and y-edge row, -1 where none
+                    const int c = col[k];
+                    const int sxc = c <= 3 ? c - 2 : (c >= NX - 4 ? 2 + c - (NX - 4) : -1);
+                    const int syr = Rw <= 3 ? Jo - 4 : (Rw >= NY - 4 ? Jo - NY + 4 : -1);
 #pragma unroll
                     for (int m = 0; m < ME; m++) {
                         R fmr = (fmx2[k][m] + e2[k][m]) + e1rv[k][m];
@@ -653,15 +658,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R qn = (qm2[k][m] - dtdx * (fmr - fpc)) - dtdy * (gmu - gpc);
                         qo[((long long)m * NY + (Jo - 2)) * NX + col[k]] = qn;
                         if constexpr (PUSH) {
-                            const int c = col[k];
-                            if (c <= 3)
-                                stage[((c - 2) * ME + m) * A.my + (Jo - 4)] = qn;
-                            else if (c >= NX - 4)
-                                stage[((2 + c - (NX - 4)) * ME + m) * A.my + (Jo - 4)] = qn;
-                            if (Rw <= 3)
-                                stage[4 * ME * A.my + ((Jo - 4) * ME + m) * A.mx + (c - 2)] = qn;
-                            else if (Rw >= NY - 4)
-                                stage[4 * ME * A.my + ((Jo - NY + 4) * ME + m) * A.mx + (c - 2)] = qn;
+                            if (sxc >= 0) stage[(sxc * ME + m) * A.my + (Jo - 4)] = qn;
+                            if (syr >= 0) stage[4 * ME * A.my + (syr * ME + m) * A.mx + (c - 2)] = qn;
                         }
                     }
                 }
