@@ -43,9 +43,9 @@ HBM_PEAK = 8.0e12
 # (profiles/k_step2_swe_humps_l7-9_hbm_bytes.md).  VALU_ACTIVE is
 # SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the level-9 launch
 # (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
-MEASURED_MESH = {7: 10481, 8: 2357, 9: 85020}
-TRAFFIC_PER_LAUNCH = 3.639e9
-VALU_ACTIVE = 0.78
+MEASURED_MESH = {7: 10481, 8: 2349, 9: 85052}
+TRAFFIC_PER_LAUNCH = 3.612e9
+VALU_ACTIVE = 0.83
 
 
 def humps(seed=7, n=8):
