@@ -131,11 +131,29 @@ __global__ void k_bc2(int npatch, int mx, int my, int meqn, R* q, const int* ids
 }
 
 // Maximum of n non-negative doubles (one workgroup; n is a patch count).
-__global__ void k_max_reduce(const double* x, int n, double* out)
+__global__ void __launch_bounds__(1024) k_max_reduce(const double* x, int n, double* out)
 {
     __shared__ double s[1024];
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v = x[i] > v ? x[i] : v;
+    // eight independent loads in flight per thread: a level has up to ~10^5
+    // patches, and one dependent load per iteration left the single
+    // workgroup waiting on memory (max is exact, so the grouping is free)
+    constexpr int U = 8;
+    const int B = blockDim.x;
+    double vv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) vv[u] = 0.0;
+    int i = threadIdx.x;
+    for (; i + (U - 1) * B < n; i += U * B) {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) t[u] = x[i + u * B];
+#pragma unroll
+        for (int u = 0; u < U; u++) vv[u] = t[u] > vv[u] ? t[u] : vv[u];
+    }
+    for (; i < n; i += B) vv[0] = x[i] > vv[0] ? x[i] : vv[0];
+    double v = vv[0];
+#pragma unroll
+    for (int u = 1; u < U; u++) v = vv[u] > v ? vv[u] : v;
     s[threadIdx.x] = v;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
