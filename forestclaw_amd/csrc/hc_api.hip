@@ -64,29 +64,55 @@ int validate_physics(const hipclaw_physics* p)
 // (applications/clawpack/advection/2d/swirl/user_4.6/b4step2.f:25-39).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void k_swirl_aux(int npatch, int mx, int my, const double* xylower,
-                            const double* dxdy, double vt, R* aux)
+// psi(x, y) = sin^2(pi x) sin^2(pi y) / pi is a product of one factor per
+// coordinate, and the three corners b4step2 evaluates per cell, (xll, yll),
+// (xll, yll+dy), (xll+dx, yll), use the coordinates xll_i, xll_i + dx, yll_j,
+// yll_j + dy as the reference forms them.  One workgroup per patch evaluates
+// the 2 (NX + NY) sines of those coordinates once into LDS; every cell then
+// combines them with the same operations as psi, so the values are those of
+// evaluating psi per cell (6 sines per cell before).
+__global__ void __launch_bounds__(256) k_swirl_aux(int npatch, int mx, int my, const double* xylower,
+                                                   const double* dxdy, double vt, R* aux)
 {
+    extern __shared__ double sn[];  // sin(pi*xll), sin(pi*(xll+dx)), sin(pi*yll), sin(pi*(yll+dy))
     const int NX = mx + 4, NY = my + 4;
     const long long T = (long long)NX * NY;
-    long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (gid >= T * npatch) return;
-    const int p = (int)(gid / T);
-    const int cell = (int)(gid - (long long)p * T);
-    const int j = cell / NX, i = cell - j * NX;
+    const int p = blockIdx.x;
+    if (p >= npatch) return;
     const double pi = 4.0 * atan(1.0);
     const double dx = dxdy[2 * p], dy = dxdy[2 * p + 1];
-    const double xll = xylower[2 * p] + (double)(i - 2) * dx;
-    const double yll = xylower[2 * p + 1] + (double)(j - 2) * dy;
-    auto psi = [pi](double x, double y) {
-        double sx = sin(pi * x), sy = sin(pi * y);
-        return ((sx * sx) * (sy * sy)) / pi;
-    };
-    const double pij = psi(xll, yll);
-    const double u = (vt / dy) * (psi(xll, yll + dy) - pij);
-    const double v = -(vt / dx) * (psi(xll + dx, yll) - pij);
-    aux[p * 2 * T + cell] = R(u);
-    aux[p * 2 * T + T + cell] = R(v);
+    const double x0 = xylower[2 * p], y0 = xylower[2 * p + 1];
+    double* sx0 = sn;
+    double* sx1 = sn + NX;
+    double* sy0 = sn + 2 * NX;
+    double* sy1 = sn + 2 * NX + NY;
+    for (int t = threadIdx.x; t < 2 * (NX + NY); t += blockDim.x) {
+        double arg;
+        if (t < 2 * NX) {
+            const int i = t < NX ? t : t - NX;
+            const double xll = x0 + (double)(i - 2) * dx;
+            arg = t < NX ? xll : xll + dx;
+        } else {
+            const int u = t - 2 * NX;
+            const int j = u < NY ? u : u - NY;
+            const double yll = y0 + (double)(j - 2) * dy;
+            arg = u < NY ? yll : yll + dy;
+        }
+        sn[t] = sin(pi * arg);
+    }
+    __syncthreads();
+    const double vty = vt / dy, vtx = vt / dx;
+    R* au = aux + (long long)p * 2 * T;
+    for (int cell = threadIdx.x; cell < (int)T; cell += blockDim.x) {
+        const int j = cell / NX, i = cell - j * NX;
+        const double a0 = sx0[i] * sx0[i], a1 = sx1[i] * sx1[i];
+        const double b0 = sy0[j] * sy0[j], b1 = sy1[j] * sy1[j];
+        const double pij = (a0 * b0) / pi;
+        const double u = vty * ((a0 * b1) / pi - pij);
+        const double v = -vtx * ((a1 * b0) / pi - pij);
+        au[cell] = R(u);
+        au[T + cell] = R(v);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -325,15 +351,13 @@ int hipclaw_swirl_aux_batch(const hipclaw_physics* phys, int npatch, const doubl
     if (tperiod == 0.0) return HIPCLAW_OK;  // velocities fixed by setaux
     const double pi2 = 8.0 * atan(1.0);
     const double vt = cos(pi2 * (t + dt / 2.0) / tperiod);
-    const long long n = (long long)(phys->mx + 4) * (phys->my + 4) * npatch;
-    const int bs = 256;
-    const unsigned nb = (unsigned)((n + bs - 1) / bs);
+    const size_t lds = sizeof(double) * 2 * (size_t)(phys->mx + 4 + phys->my + 4);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (phys->precision == HIPCLAW_FP64)
-        hipLaunchKernelGGL(k_swirl_aux<double>, dim3(nb), dim3(bs), 0, s, npatch, phys->mx, phys->my,
+        hipLaunchKernelGGL(k_swirl_aux<double>, dim3(npatch), dim3(256), lds, s, npatch, phys->mx, phys->my,
                            xylower, dxdy, vt, static_cast<double*>(aux));
     else
-        hipLaunchKernelGGL(k_swirl_aux<float>, dim3(nb), dim3(bs), 0, s, npatch, phys->mx, phys->my,
+        hipLaunchKernelGGL(k_swirl_aux<float>, dim3(npatch), dim3(256), lds, s, npatch, phys->mx, phys->my,
                            xylower, dxdy, vt, static_cast<float*>(aux));
     return check_hip(hipGetLastError(), "k_swirl_aux launch");
 }

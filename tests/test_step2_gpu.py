@@ -201,12 +201,13 @@ def test_bc2_bitwise(dev):
         assert np.array_equal(got, exp)
 
 
-def test_swirl_aux_device(dev):
+@pytest.mark.parametrize("mx,my", [(32, 32), (16, 24)])
+def test_swirl_aux_device(dev, mx, my):
     import torch
-    mx = my = 32
     n = 4
     xy = np.array([[0.0, 0.0], [0.25, 0.5], [0.5, 0.125], [0.875, 0.875]])
     dxdy = np.full((n, 2), 1.0 / 256)
+    dxdy[1] = (1.0 / 128, 1.0 / 512)
     p = H.physics(H.SOLVER_ADV, mx, my)
     aux = torch.zeros((n, 2, my + 4, mx + 4), dtype=torch.float64, device=dev)
     H.swirl_aux_batch(p, n, _t(xy, dev), _t(dxdy, dev), 1.3, 0.002, 8.0, aux)
@@ -217,7 +218,7 @@ def test_swirl_aux_device(dev):
         # device sin() and glibc sin() may differ by an ulp; an edge velocity is a
         # difference of two stream-function values (|psi| <= 1/pi) divided by dx,
         # so a few ulps of psi become eps/pi/dx ~ 2e-14 here: allow 16 of them
-        tol = 16 * np.finfo(np.float64).eps * (1 / np.pi) / dxdy[k, 0]
+        tol = 16 * np.finfo(np.float64).eps * (1 / np.pi) / dxdy[k].min()
         assert np.abs(got[k] - ref).max() <= tol
 
 
