@@ -259,6 +259,51 @@ CNN_TRAIN_FLOP = CNN_FWD_FLOP + 2 * 1296 * 9 * (16 * 32 + 32 * 32 + 32 * 16) + C
 BF16_PEAK = 2.5e15
 
 
+def advection(level, steps=10, warmup=3):
+    """Scalar advection with the swirl velocity field (period 8) on a uniform
+    level of 32x32 patches, FP32 and FP64: per coarse step the velocity field
+    (b4step2) and the step, with and without the time-dependent velocities.
+    The step kernel's rate is HIP-event time of its launches; algorithmic
+    bytes per patch: read q and the two aux fields of the 36x36 tile, write
+    the 32x32 interior."""
+    import torch
+    from forestclaw_amd import forest as F
+    from forestclaw_amd import hipclaw as H
+    out = {}
+    for prec, name, w in [(H.FP32, "fp32", 4), (H.FP64, "fp64", 8)]:
+        for tperiod in (8.0, 0.0):
+            opt = F.Options(minlevel=level, maxlevel=level, mx=32, my=32, solver=H.SOLVER_ADV, precision=prec)
+            sim = F.Hipclaw46(opt)
+            sim.set_swirl(8.0)
+            sim.init_uniform(level)
+            sim.qinit("swirl", (0.5,), nsub=1)
+            sim.set_swirl(tperiod)
+            sim.ghost_update(level, level)
+            shape, _ = sim.autotune(level, 3)
+            dt = 0.5 / (32 << level)
+            for s in range(warmup):
+                sim.advance(s * dt, dt)
+            torch.cuda.synchronize()
+            sim.profile(True)
+            t0 = time.perf_counter()
+            for s in range(steps):
+                sim.advance((warmup + s) * dt, dt)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            ms, nl, _ = sim.profile_read()
+            sim.profile(False)
+            npatch = sim.num_patches()
+            algo = w * (36 * 36 * 3 + 32 * 32) * npatch
+            key = f"{name}_{'swirl' if tperiod else 'fixed_velocity'}"
+            out[key] = {"patches": npatch, "ms_per_step": 1e3 * (t1 - t0) / steps,
+                        "cell_updates_per_s": npatch * 1024 * steps / (t1 - t0),
+                        "step_kernel_ms": ms / nl, "step_kernel_GBs": algo / (ms / nl * 1e-3) / 1e9,
+                        "step_kernel_hbm_frac": algo / (ms / nl * 1e-3) / HBM_PEAK,
+                        "launch": {"cols_per_lane": shape[0], "waves_per_group": shape[1]}}
+            sim.close()
+    return out
+
+
 def cnn_c4(nruns, batch=4096):
     """The learned-indicator configuration at its stated size: tiles of
     seeded dam-break runs (random centre, radius 0.3-0.8, h ratio 1.5-4) on
@@ -333,6 +378,9 @@ def main():
     ap.add_argument("--cnn", type=int, default=0, metavar="NRUNS",
                     help="also time the refinement network on the learned-indicator "
                          "configuration built from NRUNS dam-break runs (50: about 2M tiles)")
+    ap.add_argument("--advection", type=int, default=0, metavar="LEVEL",
+                    help="only time scalar advection with the swirl field on a uniform level "
+                         "(one JSON line of its own)")
     ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
                     help="also time the host-resident drop-in path (hipclaw_single_step_update "
                          "on NPATCH host tiles, PCIe included)")
@@ -343,6 +391,10 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.advection > 0:
+        print(json.dumps({"workload": f"scalar advection, swirl field, uniform level {args.advection}, 32x32",
+                          "advection": advection(args.advection, args.steps, args.warmup)}))
+        return
     from forestclaw_amd import hipclaw as H
     assert torch.cuda.is_available(), "no GPU visible"
 
