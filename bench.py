@@ -259,28 +259,35 @@ CNN_TRAIN_FLOP = CNN_FWD_FLOP + 2 * 1296 * 9 * (16 * 32 + 32 * 32 + 32 * 16) + C
 BF16_PEAK = 2.5e15
 
 
-def advection(level, steps=10, warmup=3):
+def advection(level, steps=10, warmup=3, fixed=False):
     """Scalar advection with the swirl velocity field (period 8) on a uniform
-    level of 32x32 patches, FP32 and FP64: per coarse step the velocity field
-    (b4step2) and the step, with and without the time-dependent velocities.
+    level of 32x32 patches, FP32 and FP64: coarse steps with the velocities of
+    each step (b4step2), and with fixed velocities when fixed is set.
     The step kernel's rate is HIP-event time of its launches; algorithmic
-    bytes per patch: read q and the two aux fields of the 36x36 tile, write
-    the 32x32 interior."""
+    bytes per patch: read q of the 36x36 tile (and the two velocity fields
+    when they are read from aux: fixed velocities; with per-step velocities
+    the step kernel evaluates them itself), write the 32x32 interior."""
     import torch
     from forestclaw_amd import forest as F
     from forestclaw_amd import hipclaw as H
     out = {}
     for prec, name, w in [(H.FP32, "fp32", 4), (H.FP64, "fp64", 8)]:
-        for tperiod in (8.0, 0.0):
+        for tperiod in ((8.0, 0.0) if fixed else (8.0,)):
             opt = F.Options(minlevel=level, maxlevel=level, mx=32, my=32, solver=H.SOLVER_ADV, precision=prec)
             sim = F.Hipclaw46(opt)
             sim.set_swirl(8.0)
             sim.init_uniform(level)
             sim.qinit("swirl", (0.5,), nsub=1)
-            sim.set_swirl(tperiod)
             sim.ghost_update(level, level)
-            shape, _ = sim.autotune(level, 3)
             dt = 0.5 / (32 << level)
+            if tperiod == 0.0:
+                # fixed velocities: store the field of t = 0 in aux with one
+                # unfused step, then keep it
+                sim.set_feature(H.FEATURE_FUSE_AUX, False)
+                sim.advance(0.0, dt)
+                sim.set_feature(H.FEATURE_FUSE_AUX, True)
+            sim.set_swirl(tperiod)
+            shape, _ = sim.autotune(level, 3)
             for s in range(warmup):
                 sim.advance(s * dt, dt)
             torch.cuda.synchronize()
@@ -293,7 +300,7 @@ def advection(level, steps=10, warmup=3):
             ms, nl, _ = sim.profile_read()
             sim.profile(False)
             npatch = sim.num_patches()
-            algo = w * (36 * 36 * 3 + 32 * 32) * npatch
+            algo = w * (36 * 36 * (1 if tperiod else 3) + 32 * 32) * npatch
             key = f"{name}_{'swirl' if tperiod else 'fixed_velocity'}"
             out[key] = {"patches": npatch, "ms_per_step": 1e3 * (t1 - t0) / steps,
                         "cell_updates_per_s": npatch * 1024 * steps / (t1 - t0),
@@ -381,6 +388,8 @@ def main():
     ap.add_argument("--advection", type=int, default=0, metavar="LEVEL",
                     help="only time scalar advection with the swirl field on a uniform level "
                          "(one JSON line of its own)")
+    ap.add_argument("--advection-fixed", action="store_true",
+                    help="with --advection: also time fixed velocities read from aux")
     ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
                     help="also time the host-resident drop-in path (hipclaw_single_step_update "
                          "on NPATCH host tiles, PCIe included)")
@@ -393,7 +402,8 @@ def main():
     import torch
     if args.advection > 0:
         print(json.dumps({"workload": f"scalar advection, swirl field, uniform level {args.advection}, 32x32",
-                          "advection": advection(args.advection, args.steps, args.warmup)}))
+                          "advection": advection(args.advection, args.steps, args.warmup,
+                                                 args.advection_fixed)}))
         return
     from forestclaw_amd import hipclaw as H
     assert torch.cuda.is_available(), "no GPU visible"

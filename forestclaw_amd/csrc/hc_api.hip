@@ -315,6 +315,9 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         a.levels = ml->levels;
         a.slot0 = ml->slot0;
         a.push = ml->push;
+        a.swirl = ml->swirl;
+        a.xylower = ml->xylower;
+        for (int L = 0; L < STEP_MAXL; L++) a.vtl[L] = ml->vtl[L];
         for (int L = 0; L < STEP_MAXL; L++) {
             a.dtl[L] = ml->dtl[L];
             a.qin_l[L] = ml->qin_l[L];

@@ -338,6 +338,19 @@ Src2Args Ctx::src2_args(int s0, int np) const
     return a;
 }
 
+// Swirl advection: the velocity field b4step2 stores before each step is
+// evaluated inside the step kernel (same operations) unless switched off.
+bool Ctx::fused_swirl() const
+{
+    return fuse_aux && swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV;
+}
+
+double Ctx::swirl_vt(double t, double dt) const
+{
+    const double pi2 = 8.0 * atan(1.0);
+    return cos(pi2 * (t + dt / 2.0) / swirl_tperiod);
+}
+
 int Ctx::update_level(int level, double t, double dt)
 {
     const int s0 = plan[level].s0, s1 = plan[level].s1;
@@ -346,7 +359,8 @@ int Ctx::update_level(int level, double t, double dt)
     char* qin = static_cast<char*>(pool[parity[level]]) + (size_t)s0 * tile * esz;
     char* qout = static_cast<char*>(pool[parity[level] ^ 1]) + (size_t)s0 * tile * esz;
     char* a = aux ? static_cast<char*>(aux) + (size_t)s0 * atile * esz : nullptr;
-    if (swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV)
+    const bool fz = fused_swirl();
+    if (!fz && swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV)
         HC_TRY(hipclaw_swirl_aux_batch(&phys, np, d_xylower + 2 * s0, d_dxdy + 2 * s0, t, dt,
                                        swirl_tperiod, a, stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -356,10 +370,22 @@ int Ctx::update_level(int level, double t, double dt)
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
     const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
-    if (tsync)
+    if (tsync) {
         HC_TRY(hipclaw_step2_batch_sync(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
                                         d_cfl + s0, d_reg + (size_t)s0 * regstride, lc, stream));
-    else
+    } else if (fz) {
+        MultiLevel ml{};
+        ml.levels = d_levels;
+        ml.slot0 = s0;
+        ml.dtl[level] = dt;
+        ml.qin_l[level] = pool[parity[level]];
+        ml.qout_l[level] = pool[parity[level] ^ 1];
+        ml.swirl = 1;
+        ml.xylower = d_xylower;
+        ml.vtl[level] = swirl_vt(t, dt);
+        HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dt, d_cfl, nullptr, lc,
+                            stream, &ml));
+    } else
         HC_TRY(hipclaw_step2_batch(&phys, np, qin, qout, a, d_dxdy + 2 * s0, nullptr, np, dt,
                                    d_cfl + s0, lc, stream));
     if (src2) {
@@ -420,6 +446,11 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     ml.qin_l[level] = pool[parity[level]];
     ml.qout_l[level] = static_cast<char*>(tmp) - (ptrdiff_t)s0 * tile * esz;
     ml.push = d_push;
+    if (fused_swirl()) {
+        ml.swirl = 1;
+        ml.xylower = d_xylower;
+        ml.vtl[level] = swirl_vt(0.0, dt);
+    }
     auto run = [&](const hipclaw_launch& l) {
         if (push)
             return step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dt, cfl - s0, nullptr,
@@ -516,7 +547,11 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         ml.dtl[L] = dtl[L];
         ml.qin_l[L] = pool[parity[L]];
         ml.qout_l[L] = pool[parity[L] ^ 1];
-        if (swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV) {
+        if (fused_swirl()) {
+            ml.swirl = 1;
+            ml.xylower = d_xylower;
+            ml.vtl[L] = swirl_vt(tl[L], dtl[L]);
+        } else if (swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV) {
             const int a0 = plan[L].s0, na = plan[L].s1 - a0;
             char* a = static_cast<char*>(aux) + (size_t)a0 * atile * esz;
             HC_TRY(hipclaw_swirl_aux_batch(&phys, na, d_xylower + 2 * a0, d_dxdy + 2 * a0, tl[L], dtl[L],
@@ -1053,6 +1088,7 @@ int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on)
         break;
     case HIPCLAW_FEATURE_GRAPH: c.use_graph = on != 0; break;
     case HIPCLAW_FEATURE_INIT_GHOSTCELL: c.init_ghostcell = on != 0; break;
+    case HIPCLAW_FEATURE_FUSE_AUX: c.fuse_aux = on != 0; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown feature");
     }
     return HIPCLAW_OK;

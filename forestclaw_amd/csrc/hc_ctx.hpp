@@ -74,6 +74,7 @@ struct Ctx {
     int tag_stats[6] = {};         // patches, net refine, plain refine, misses, false refines, fallbacks
     bool tsync = false;            // conservative time synchronisation on
     int src2 = HIPCLAW_SRC2_NONE;  // source term after every patch step (src_term)
+    bool fuse_aux = true;          // swirl velocities evaluated in the step kernel
     double* d_reg = nullptr;       // time-sync registers, regstride doubles per slot (FP64)
     long long regstride = 0;
     // tagging parameters: clawpatch refinement-criteria (HIPCLAW_TAG_*),
@@ -103,6 +104,8 @@ struct Ctx {
     int ghost_update(int minlevel, int maxlevel, int time_interp);
     int update_level(int level, double t, double dt);
     Src2Args src2_args(int s0, int np) const;
+    bool fused_swirl() const;
+    double swirl_vt(double t, double dt) const;
     // one merged launch for the contiguous levels [lo, hi] due at the same
     // subcycled time (t[L], dt[L] per level)
     int update_levels(int lo, int hi, const double* t, const double* dt);

@@ -62,6 +62,11 @@ struct MultiLevel {
     const void* qin_l[STEP_MAXL];
     void* qout_l[STEP_MAXL];
     const int* push;  // optional same-level ghost push table (8 ints per slot)
+    // advection: the swirl velocity field evaluated in the step kernel
+    // (xylower by slot, vt = cos(2 pi (t + dt/2) / T) per level) instead of aux
+    int swirl;
+    const double* xylower;
+    double vtl[STEP_MAXL];
 };
 int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                  const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,

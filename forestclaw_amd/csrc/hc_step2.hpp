@@ -59,6 +59,15 @@ struct StepArgs {
     // corners 0..3 (-1: none); updated edge cells are also written into their
     // ghost cells (the same-level copy of the next ghost update).
     const int* push;
+    // Advection with the swirl velocity field evaluated in the kernel
+    // (swirl != 0) instead of read from aux: the velocities b4step2 would
+    // store, from the patch corner (xylower, per slot / tile like dxdy) and
+    // vt = cos(2 pi (t + dt/2) / T) of the step (vtl[level] in multi-level
+    // launches).
+    int swirl;
+    const double* xylower;
+    double vt;
+    double vtl[STEP_MAXL];
 };
 
 // Time-sync register blocks of one face: P (fp/gp), M (fm/gm), E0/E1 (edge
@@ -220,6 +229,44 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         }
     }
 
+    // ---- advection: the swirl velocity field in the kernel ---------------
+    // psi(x, y) = sin^2(pi x) sin^2(pi y) / pi at the corners (xll, yll),
+    // (xll, yll+dy), (xll+dx, yll) of every cell, with the operations of the
+    // velocity-field kernel (k_swirl_aux), so the velocities are the ones it
+    // would store: the column sines are kept per lane, the row sines of each
+    // patch are evaluated once into LDS (after the push staging area).
+    double swx0[CPL], swx1[CPL];
+    const double* swy = nullptr;
+    double swvtx = 0.0, swvty = 0.0;
+    const double swpi = 4.0 * atan(1.0);
+    if constexpr (S::MAUX > 0) {
+        if (A.swirl) {
+            const double x0 = pact ? A.xylower[2 * tix] : 0.0;
+            const double y0 = pact ? A.xylower[2 * tix + 1] : 0.0;
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const double xll = x0 + (double)(col[k] - 2) * dx;
+                swx0[k] = sin(swpi * xll);
+                swx1[k] = sin(swpi * (xll + dx));
+            }
+            const size_t stage_bytes = PUSH ? sizeof(R) * (size_t)(blockDim.x >> 6) * A.ppw * pstage : 0;
+            double* ys = reinterpret_cast<double*>(hc_smem + ((stage_bytes + 7) & ~(size_t)7)) +
+                         ((threadIdx.x >> 6) * A.ppw + (slot < A.ppw ? slot : 0)) * 2 * NY;
+            if (slot < A.ppw) {
+                for (int t = lp; t < 2 * NY; t += A.lpp) {
+                    const int j = t < NY ? t : t - NY;
+                    const double yll = y0 + (double)(j - 2) * dy;
+                    ys[t] = sin(swpi * (t < NY ? yll : yll + dy));
+                }
+            }
+            swy = ys;
+            const double vt = A.levels ? A.vtl[lev] : A.vt;
+            swvty = vt / dy;
+            swvtx = vt / dx;
+            __syncthreads();
+        }
+    }
+
     auto row = [&](auto phase, const int J) __attribute__((always_inline)) {
         constexpr int P = decltype(phase)::value;
         constexpr int S0 = P & 3, S1 = (P + 3) & 3, S2 = (P + 2) & 3;  // rows J, J-1, J-2
@@ -268,9 +315,29 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             for (int m = 0; m < ME; m++)
                 qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
             if constexpr (S::MAUX > 0) {
-                ucur[k] = (cv[k] && rowin) ? ap[(long long)Jo * NX + col[k]] : R(0);
-                vcur[k] = (cv[k] && rowin) ? ap[((long long)NY + Jo) * NX + col[k]] : R(0);
-                vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + Jo + 1) * NX + col[k]] : R(0);
+                if (A.swirl) {
+                    // velocities of (c, J) and the v of (c, J+1), as k_swirl_aux
+                    ucur[k] = vcur[k] = vnext[k] = R(0);
+                    const double a0 = swx0[k] * swx0[k], a1 = swx1[k] * swx1[k];
+                    if (rowin) {
+                        const double b0 = swy[Jo] * swy[Jo], b1 = swy[NY + Jo] * swy[NY + Jo];
+                        const double pij = (a0 * b0) / swpi;
+                        const double u = swvty * ((a0 * b1) / swpi - pij);
+                        const double v = -swvtx * ((a1 * b0) / swpi - pij);
+                        ucur[k] = cv[k] ? R(u) : R(0);
+                        vcur[k] = cv[k] ? R(v) : R(0);
+                    }
+                    if (J + 1 < NY) {
+                        const double b0 = swy[Jo + 1] * swy[Jo + 1];
+                        const double pij = (a0 * b0) / swpi;
+                        const double v = -swvtx * ((a1 * b0) / swpi - pij);
+                        vnext[k] = cv[k] ? R(v) : R(0);
+                    }
+                } else {
+                    ucur[k] = (cv[k] && rowin) ? ap[(long long)Jo * NX + col[k]] : R(0);
+                    vcur[k] = (cv[k] && rowin) ? ap[((long long)NY + Jo) * NX + col[k]] : R(0);
+                    vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + Jo + 1) * NX + col[k]] : R(0);
+                }
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
             }

@@ -10,7 +10,8 @@ template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC
 static void launch_k(const StepArgs& a, long long blocks, int wpg, const void* qin, void* qout,
                      const void* aux, double* cfl, hipStream_t s)
 {
-    const size_t lds = PUSH ? sizeof(R) * (size_t)wpg * a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) : 0;
+    size_t lds = PUSH ? sizeof(R) * (size_t)wpg * a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) : 0;
+    if (a.swirl) lds = ((lds + 7) & ~(size_t)7) + sizeof(double) * (size_t)wpg * a.ppw * 2 * (a.my + 4);
     hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH>), dim3((unsigned)blocks),
                        dim3(HC_WAVE * wpg), lds, s, a, static_cast<const R*>(qin),
                        static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
@@ -28,10 +29,13 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     a.ppw = HC_WAVE / a.lpp;
     if (a.push) {
         // the push stage must fit a workgroup's LDS: fewer waves per group if needed
-        const size_t per_wave = sizeof(R) * (size_t)a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my);
+        const size_t per_wave = sizeof(R) * (size_t)a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) +
+                                (a.swirl ? sizeof(double) * (size_t)a.ppw * 2 * (a.my + 4) + 8 : 0);
         while (wpg > 1 && per_wave * wpg > 65536) wpg--;
         if (per_wave > 65536) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patches too large for the ghost push");
     }
+    if (a.swirl && !(SolverT<R>::MAUX > 0)) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the advection solver");
+    if (a.swirl && !a.xylower) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the patch corners");
     const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
     const long long blocks = (waves + wpg - 1) / wpg;
     if (blocks == 0) return HIPCLAW_OK;

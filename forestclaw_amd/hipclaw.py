@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG, "libhipclaw.so")
 SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
 FP64, FP32 = 0, 1
 FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
+FEATURE_FUSE_AUX = 5
 TAG_VALUE, TAG_DIFFERENCE, TAG_MINMAX, TAG_GRADIENT = 0, 1, 2, 3
 SRC2_NONE, SRC2_AXISYMMETRIC = 0, 1
 CLAW5 = 0x100   # solver flag: the Clawpack 5 solver and host layout q(meqn, i, j)

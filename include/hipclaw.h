@@ -265,6 +265,9 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
  *                 refinement passes of hipclaw_initialize and after them; off,
  *                 the tags and the first step see the qinit ghost values */
 #define HIPCLAW_FEATURE_INIT_GHOSTCELL 4
+/* Swirl advection: evaluate the velocity field inside the step kernel instead
+ * of storing it in aux before each step (default on; the same values). */
+#define HIPCLAW_FEATURE_FUSE_AUX 5
 int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on);
 /* Coarse steps captured as graphs and replays of captured graphs so far. */
 int hipclaw_graph_stats(hipclaw_ctx* ctx, long long* captures, long long* replays);
@@ -294,7 +297,10 @@ int hipclaw_level_download(hipclaw_ctx* ctx, int level, int n, double* const* q)
 int hipclaw_set_q(hipclaw_ctx* ctx, const double* host_q);
 int hipclaw_get_q(hipclaw_ctx* ctx, double* host_q);
 /* Swirl velocity field recomputed before every patch step (b4step2), period
- * tperiod; tperiod = 0 keeps the aux arrays as set. */
+ * tperiod; tperiod = 0 keeps the aux arrays as set.  With
+ * HIPCLAW_FEATURE_FUSE_AUX on (the default) the step kernel evaluates the
+ * field itself and the aux arrays are not written, so after tperiod = 0 they
+ * hold what hipclaw_set_aux / hipclaw_level_upload or an unfused step left. */
 int hipclaw_set_swirl(hipclaw_ctx* ctx, double tperiod);
 int hipclaw_set_aux(hipclaw_ctx* ctx, const double* host_aux);
 

@@ -124,6 +124,28 @@ def test_euler_brick_bitwise(dev):
     assert np.array_equal(interiors(sim.get_q()), interiors(amr.q))
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+@pytest.mark.parametrize("merge", [True, False])
+def test_swirl_fused_velocities_bitwise(dev, precision, merge):
+    """The step kernel's own swirl velocities (HIPCLAW_FEATURE_FUSE_AUX) give
+    the bits of the velocity-field kernel followed by the step, on a
+    subcycled two-level mesh with non-square cells."""
+    runs = []
+    for fuse in (False, True):
+        sim = make(H.SOLVER_ADV, 16, 2, 3, precision=precision, box=(0.0, 1.0, 0.0, 0.75))
+        sim.set_feature(H.FEATURE_MERGE_LEVELS, merge)
+        sim.set_feature(H.FEATURE_FUSE_AUX, fuse)
+        sim.set_swirl(8.0)
+        sim.initialize("swirl", (0.5,))
+        dt = 0.004
+        cfl = [sim.advance(s * dt, dt) for s in range(4)]
+        runs.append((sim.get_q(), cfl, sim.num_patches()))
+        sim.close()
+    assert runs[0][2] == runs[1][2] and runs[0][2] > 16
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(interiors(runs[0][0]), interiors(runs[1][0]))
+
+
 def test_swirl_advection_device_velocities(dev):
     sim = make(H.SOLVER_ADV, 16, 2, 2)
     sim.set_swirl(8.0)
