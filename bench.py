@@ -259,6 +259,45 @@ CNN_TRAIN_FLOP = CNN_FWD_FLOP + 2 * 1296 * 9 * (16 * 32 + 32 * 32 + 32 * 16) + C
 BF16_PEAK = 2.5e15
 
 
+def small_configs(args):
+    """The other configurations (forestclaw_amd.configs C1 swirl, C2 radial
+    dam, C3 shock-bubble) on their meshes after the initial refinement, no
+    regrid: FP32 batched (merged launches, ghost push, graph replays where
+    allowed) against FP64 plain (one launch per level step, eager), each
+    timed over its stated run length after two warm-up steps."""
+    import torch
+    from forestclaw_amd import configs as CF
+    from forestclaw_amd import hipclaw as H
+    lengths = {"C1": 200, "C2": 100, "C3": 30}
+    out = {}
+    for name, nsteps in lengths.items():
+        res = {}
+        for label, prec, plain in (("fp32_batched", H.FP32, False), ("fp64_plain", H.FP64, True)):
+            sim, dt = CF.build(name, prec)
+            sim.set_feature(H.FEATURE_GRAPH, not plain)
+            sim.set_feature(H.FEATURE_GHOST_PUSH, not plain)
+            sim.set_feature(H.FEATURE_MERGE_LEVELS, not plain)
+            t = 0.0
+            for s in range(2):
+                sim.advance(t, dt)
+                t += dt
+            c0 = sim.cell_updates()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(nsteps):
+                sim.advance(t, dt)
+                t += dt
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            res[label] = {"cell_updates_per_s": (sim.cell_updates() - c0) / (t1 - t0),
+                          "ms_per_step": 1e3 * (t1 - t0) / nsteps, "patches": sim.num_patches()}
+            sim.close()
+        res["speedup"] = res["fp32_batched"]["cell_updates_per_s"] / res["fp64_plain"]["cell_updates_per_s"]
+        res["steps"] = nsteps
+        out[name] = res
+    return out
+
+
 def advection(level, steps=10, warmup=3, fixed=False):
     """Scalar advection with the swirl velocity field (period 8) on a uniform
     level of 32x32 patches, FP32 and FP64: coarse steps with the velocities of
@@ -388,6 +427,9 @@ def main():
     ap.add_argument("--advection", type=int, default=0, metavar="LEVEL",
                     help="only time scalar advection with the swirl field on a uniform level "
                          "(one JSON line of its own)")
+    ap.add_argument("--configs", action="store_true",
+                    help="only time configurations C1-C3, FP32 batched against FP64 plain "
+                         "(one JSON line of its own)")
     ap.add_argument("--advection-fixed", action="store_true",
                     help="with --advection: also time fixed velocities read from aux")
     ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
@@ -400,6 +442,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.configs:
+        print(json.dumps({"workload": "configurations C1-C3", "configs": small_configs(args)}))
+        return
     if args.advection > 0:
         print(json.dumps({"workload": f"scalar advection, swirl field, uniform level {args.advection}, 32x32",
                           "advection": advection(args.advection, args.steps, args.warmup,

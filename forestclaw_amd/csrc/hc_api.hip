@@ -280,9 +280,13 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     if (launch) {
         cpl = launch->cols_per_lane;
         wpg = launch->waves_per_group;
-    } else if (phys->precision == HIPCLAW_FP32 && phys->mx + 4 > 32) {
+    } else if (phys->precision == HIPCLAW_FP32 && phys->mx + 4 > 32 && npatch >= 2048) {
+        // two columns per lane (three patches per wave) pays on large
+        // launches; a small one needs every patch on a wave of its own
         cpl = 2;
     }
+    // a launch of fewer waves than four per CU spreads them one per workgroup
+    if (!launch && npatch < 1024) wpg = 1;
     if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4)
         return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
     StepArgs a{};
