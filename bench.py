@@ -269,11 +269,13 @@ def small_configs(args):
     from forestclaw_amd import configs as CF
     from forestclaw_amd import hipclaw as H
     lengths = {"C1": 200, "C2": 100, "C3": 30}
+    if not _STREAM:
+        _STREAM.append(torch.cuda.Stream())
     out = {}
     for name, nsteps in lengths.items():
         res = {}
         for label, prec, plain in (("fp32_batched", H.FP32, False), ("fp64_plain", H.FP64, True)):
-            sim, dt = CF.build(name, prec)
+            sim, dt = CF.build(name, prec, stream=_STREAM[0].cuda_stream)
             sim.set_feature(H.FEATURE_GRAPH, not plain)
             sim.set_feature(H.FEATURE_GHOST_PUSH, not plain)
             sim.set_feature(H.FEATURE_MERGE_LEVELS, not plain)
@@ -290,7 +292,8 @@ def small_configs(args):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             res[label] = {"cell_updates_per_s": (sim.cell_updates() - c0) / (t1 - t0),
-                          "ms_per_step": 1e3 * (t1 - t0) / nsteps, "patches": sim.num_patches()}
+                          "ms_per_step": 1e3 * (t1 - t0) / nsteps, "patches": sim.num_patches(),
+                          "graph_replays": sim.graph_stats()[1]}
             sim.close()
         res["speedup"] = res["fp32_batched"]["cell_updates_per_s"] / res["fp64_plain"]["cell_updates_per_s"]
         res["steps"] = nsteps

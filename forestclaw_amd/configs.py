@@ -76,19 +76,22 @@ def c3_dt(coarsest=3):
     return 0.8 * dx / 3.6  # bubble sound speed sqrt(1.4/0.138) = 3.2 plus the flow
 
 
-def build(name, precision=H.FP64):
-    """(simulation, coarse dt) with initial data and the initial refinement."""
+def build(name, precision=H.FP64, stream=None):
+    """(simulation, coarse dt) with initial data and the initial refinement;
+    stream: the HIP stream of the simulation (default: torch's current one;
+    coarse steps are replayed as graphs only on a stream other than the null
+    stream)."""
     if name == "C1":
-        sim = F.Hipclaw46(c1_options(precision))
+        sim = F.Hipclaw46(c1_options(precision), stream=stream)
         sim.set_swirl(8.0)
         sim.initialize("swirl", (0.5,))
         return sim, c1_dt()
     if name == "C2":
-        sim = F.Hipclaw46(c2_options(precision))
+        sim = F.Hipclaw46(c2_options(precision), stream=stream)
         sim.initialize("radial_dam", c2_case(), nsub=4)
         return sim, c2_dt(int(sim.patch_info()["level"].min()))
     if name == "C3":
-        sim = F.Hipclaw46(c3_options(precision))
+        sim = F.Hipclaw46(c3_options(precision), stream=stream)
         sim.initialize("shock_bubble", c3_params(), nsub=4)
         return sim, c3_dt(int(sim.patch_info()["level"].min()))
     raise ValueError(name)
