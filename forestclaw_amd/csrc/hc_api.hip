@@ -287,8 +287,12 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     }
     // a launch of fewer waves than four per CU spreads them one per workgroup
     if (!launch && npatch < 1024) wpg = 1;
-    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4)
-        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4");
+    // a launch of fewer patches than the chip has wave slots splits the rows
+    // of every patch into bands, one wave each: the sweep, which is latency
+    // bound there, gets 2 or 4 times shorter for 4 extra rows per band
+    int nband = launch ? launch->row_bands : (npatch <= 512 ? 4 : (npatch <= 1024 ? 2 : 1));
+    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4 || nband < 0 || nband > 4)
+        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..4");
     StepArgs a{};
     a.npatch = npatch;
     a.mx = phys->mx;
@@ -327,6 +331,19 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
             a.qin_l[L] = ml->qin_l[L];
             a.qout_l[L] = ml->qout_l[L];
         }
+    }
+    // bands of at least two rows each, none empty; an update in place must
+    // not overwrite rows that another band of the patch still reads, so it
+    // sweeps unbanded
+    if (nband < 2 || q_in == q_out) {
+        a.nband = 1;
+        a.band_rows = a.my;
+    } else {
+        a.band_rows = (a.my + nband - 1) / nband;
+        if (a.band_rows < 2) a.band_rows = 2;
+        a.nband = (a.my + a.band_rows - 1) / a.band_rows;
+        if (a.nband < 2) a.band_rows = a.my;
+        else wpg = a.nband;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (phys->precision == HIPCLAW_FP64)

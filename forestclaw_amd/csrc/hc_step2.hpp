@@ -68,6 +68,15 @@ struct StepArgs {
     const double* xylower;
     double vt;
     double vtl[STEP_MAXL];
+    // Row bands (nband > 1, q_out != q_in): a workgroup of
+    // nband waves takes the same ppw patches, wave b the interior rows
+    // [2 + b*band_rows, 2 + (b+1)*band_rows) (clipped to my), sweeping from
+    // at least two rows below to two rows above them; the dependence of an
+    // interior row on the data reaches two rows each way, as the ghost width
+    // does, so every band computes its rows exactly as a whole-patch sweep
+    // would.  The bands of a patch stage its edge cells for the ghost push
+    // in one shared area and push them together.
+    int nband, band_rows;
 };
 
 // Time-sync register blocks of one face: P (fp/gp), M (fm/gm), E0/E1 (edge
@@ -149,13 +158,25 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int gwave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
     const int slot = lane / A.lpp;
     const int lp = lane - slot * A.lpp;
-    const int pidx = gwave * A.ppw + slot;
+    const int band = A.nband > 1 ? (int)(threadIdx.x >> 6) : 0;
+    const int pidx = (A.nband > 1 ? (int)blockIdx.x : gwave) * A.ppw + slot;
     const bool pin = (slot < A.ppw) && (pidx < A.npatch);
     const int praw = pin ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
     // a list entry outside the tile range is skipped, never dereferenced
     const bool pact = pin && praw >= 0 && praw < A.ntiles;
     const int patch = pact ? praw : 0;
     const int NX = A.mx + 4, NY = A.my + 4;
+    // rows of this wave: interior rows [ra, rb) are written, the sweep runs
+    // over rows J0 .. Jlast (the whole tile without bands)
+    const int ra = A.nband > 1 ? 2 + band * A.band_rows : 2;
+    const int rb = A.nband > 1 ? min(2 + (band + 1) * A.band_rows, NY - 2) : NY - 2;
+    const bool first_band = band == 0;
+    const bool last_band = A.nband <= 1 || band == A.nband - 1;
+    // a band's sweep starts on a multiple of the unroll factor, so every row
+    // runs through the unrolled copy (and its FP32 contractions) that a
+    // whole-patch sweep runs it through
+    const int J0 = first_band ? 0 : ((ra - 2) & ~(UNR - 1));
+    const int Jlast = last_band ? NY - 1 : rb + 1;
 
     // tile index into the tile-indexed arrays; multi-level launches address
     // slots and take q pools and dt from the slot's level
@@ -170,10 +191,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
     // PUSH: this patch's edge cells are staged in LDS during the sweep, per
     // patch 4*meqn*(mx+my) values: x edges [side][col][m][row], then y edges
-    // [side][row][m][col] (sides: low, high; two cells deep)
+    // [side][row][m][col] (sides: low, high; two cells deep); the row bands
+    // of a patch share its area
     extern __shared__ char hc_smem[];
     const int pstage = 4 * ME * (A.mx + A.my);
-    R* stage = reinterpret_cast<R*>(hc_smem) + ((threadIdx.x >> 6) * A.ppw + (slot < A.ppw ? slot : 0)) * pstage;
+    const int swave = A.nband > 1 ? 0 : (int)(threadIdx.x >> 6);  // staging wave
+    const int nstage = A.nband > 1 ? 1 : (int)(blockDim.x >> 6);   // staging waves per group
+    R* stage = reinterpret_cast<R*>(hc_smem) + (swave * A.ppw + (slot < A.ppw ? slot : 0)) * pstage;
     const double dx = pact ? A.dxdy[2 * tix] : 1.0;
     const double dy = pact ? A.dxdy[2 * tix + 1] : 1.0;
     const R dtdx = R(dtp / dx);
@@ -197,9 +221,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
 
     // ---- carried state -------------------------------------------------
-    // The sliding windows are rings indexed by the row number: a value made
-    // at row J lives in slot J mod 4 (windows three rows deep: q, u, the
-    // x-fluxes and G) or J mod 2 (two rows deep: the y-face, its transverse
+    // The sliding windows are rings indexed by the row number counted from
+    // the first row J0 of the sweep: a value made at row J lives in slot
+    // (J - J0) mod 4 (windows three rows deep: q, u, the x-fluxes and G) or
+    // (J - J0) mod 2 (two rows deep: the y-face, its transverse
     // splits and limiter dot products, d2).  With UNR = 4 the row loop is
     // unrolled four times and every slot index is a compile-time constant,
     // so the windows never move between registers; UNR = 1 keeps one row per
@@ -249,7 +274,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 swx0[k] = sin(swpi * xll);
                 swx1[k] = sin(swpi * (xll + dx));
             }
-            const size_t stage_bytes = PUSH ? sizeof(R) * (size_t)(blockDim.x >> 6) * A.ppw * pstage : 0;
+            const size_t stage_bytes = PUSH ? sizeof(R) * (size_t)nstage * A.ppw * pstage : 0;
             double* ys = reinterpret_cast<double*>(hc_smem + ((stage_bytes + 7) & ~(size_t)7)) +
                          ((threadIdx.x >> 6) * A.ppw + (slot < A.ppw ? slot : 0)) * 2 * NY;
             if (slot < A.ppw) {
@@ -303,7 +328,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
         // ---- load row J (rows past the tile, run by the unrolled loop when
         // NY is not a multiple of 4, load zeros and store nothing) ----------
-        const bool rowin = J < NY;
+        const bool rowin = J <= Jlast;
+        // row within the sweep: a band starts as a whole-patch sweep starts
+        // (no face below its first row, its first row not x-swept), so the
+        // rows it writes see the operations of a whole-patch sweep
+        const int Jr = J - J0;
         // row offsets through an opaque copy of J: addresses are formed per
         // row from the per-lane tile base instead of one pointer induction
         // variable per unrolled row and field kept live across the loop
@@ -362,7 +391,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 double qd[ME], fl[ME];
 #pragma unroll
                 for (int m = 0; m < ME; m++) qd[m] = (double)qc[k][m];
-                if (cv[k] && J >= 2 && J <= NY - 3 && (c == 1 || c == 2 || c == NX - 3 || c == NX - 2)) {
+                if (cv[k] && J >= ra && J < rb && (c == 1 || c == 2 || c == NX - 3 || c == NX - 2)) {
                     const int face = c <= 2 ? 0 : 1;
                     const int blk = (c == 2 || c == NX - 3) ? 2 : 3;
                     S::template cons_flux<0>(qd, pp.grav, fl);
@@ -370,7 +399,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                     for (int m = 0; m < ME; m++) reg_add(r + m * A.my, dtel_y, fl[m]);
                 }
-                if (cv[k] && c >= 2 && c <= NX - 3 && (J == 1 || J == 2 || J == NY - 3 || J == NY - 2)) {
+                if (cv[k] && c >= 2 && c <= NX - 3 &&
+                    (((J == 1 || J == 2) && first_band) || ((J == NY - 3 || J == NY - 2) && last_band))) {
                     const int face = J <= 2 ? 2 : 3;
                     const int blk = (J == 2 || J == NY - 3) ? 2 : 3;
                     S::template cons_flux<1>(qd, pp.grav, fl);
@@ -383,7 +413,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
         // ---- x-sweep on row J (reference rows 2-mbc .. my+mbc-1) -----------
         R fmxn[CPL][ME], fpxn[CPL][ME], Gn[CPL][ME], d2n[CPL][ME];
-        if (J >= 1 && J <= NY - 2) {
+        if (Jr >= 1 && J <= NY - 2) {
             Face fx[CPL];
             Trans tx[CPL];
             R qa[CPL][ME];
@@ -417,7 +447,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     tx[k] = Trans{vl_cur[k], vl_next[k], vcur[k], vnext[k]};
                 }
                 // Courant number on faces i = 1 .. mx+1 of every swept row
-                if (cv[k] && col[k] >= 2 && col[k] <= NX - 2) {
+                if (cv[k] && col[k] >= 2 && col[k] <= NX - 2 && J <= Jlast) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         R a = dtdx * fx[k].s[w];
@@ -570,7 +600,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
             for (int w = 0; w < MW; w++) dotry[k][w] = R(0);
         }
-        if (J == 0) {
+        if (Jr == 0) {
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
                 fyn[k] = Face{};
@@ -583,7 +613,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 if constexpr (S::NC > 0) S::cell(qm1[k], pp, cdm);
                 else cdm[0] = R(0);
                 S::template rpn2<1>(qm1[k], qc[k], cdm, cdc[k], vcur[k], pp, fyn[k], tyn[k]);
-                if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2) {
+                if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2 && J > J0 && J <= Jlast) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         R a = dtdy * fyn[k].s[w];
@@ -593,7 +623,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
             }
         }
-        if (J >= 2) {
+        if (Jr >= 2) {
             // dot product of face J-1 with face J (both unlimited)
 #pragma unroll
             for (int k = 0; k < CPL; k++)
@@ -605,7 +635,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     dotry[k][w] = d;
                 }
         }
-        if (J >= 3) {
+        if (Jr >= 3) {
             // finalise y-face F = J-1 (2 <= F <= NY-2)
             R urt_m1[CPL], urt_m2[CPL];
 #pragma unroll
@@ -690,7 +720,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 
         // ---- interior cell row R = J-2 ------------------------------------
         const int Rw = J - 2;
-        if (Rw >= 2 && Rw <= NY - 3) {
+        if (Rw >= ra && Rw < rb) {
             // neighbour terms by DPP from every lane first, then one branch
             // per column for the update and its stores
             R e1rv[CPL][ME], e2lv[CPL][ME];
@@ -778,10 +808,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 if (!(cv[k] && c >= 2 && c <= NX - 3)) continue;
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
-                    if (J == 3)  // gm on the bottom edge face (top face of ghost row 1)
+                    if (J == 3 && first_band)  // gm on the bottom edge face (top face of ghost row 1)
                         reg_add(rg + reg_off(A.mx, A.my, ME, 2, 1) + m * A.mx + (c - 2), dtel_x,
                                 (double)(G1[k][m] + fdm[k][m]));
-                    if (J == NY - 1)  // gp on the top edge face (bottom face of ghost row NY-2)
+                    if (J == NY - 1 && last_band)  // gp on the top edge face (bottom face of ghost row NY-2)
                         reg_sub(rg + reg_off(A.mx, A.my, ME, 3, 0) + m * A.mx + (c - 2), dtel_x,
                                 (double)(G1[k][m] + fdp[k][m]));
                 }
@@ -810,14 +840,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     };
 
     if constexpr (UNR == 4) {
-        for (int J = 0; J < NY; J += 4) {
+        for (int J = J0; J <= Jlast; J += 4) {
             row(std::integral_constant<int, 0>{}, J);
             row(std::integral_constant<int, 1>{}, J + 1);
             row(std::integral_constant<int, 2>{}, J + 2);
             row(std::integral_constant<int, 3>{}, J + 3);
         }
     } else {
-        for (int J = 0; J < NY; J++) {
+        for (int J = J0; J <= Jlast; J++) {
             row(std::integral_constant<int, 0>{}, J);
             // rotate the rings: slot 0 holds row J, slots 3 / 2 / 1 are read
             // as rows J-1 / J-2 / J-1 by the next row
@@ -849,25 +879,28 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
 
     // ---- same-level ghost push --------------------------------------------
-    // After the sweep every lane of the wave stores a share of the staged
-    // edge cells of the wave's patches into the ghost cells of their
-    // same-level neighbours across faces and corners.
+    // After the sweep every lane of the wave (of the workgroup, with row
+    // bands) stores a share of the staged edge cells of the wave's patches
+    // into the ghost cells of their same-level neighbours across faces and
+    // corners.
     if constexpr (PUSH) {
         __syncthreads();  // every thread of the block reaches this point
+        const int tl = A.nband > 1 ? (int)threadIdx.x : lane;          // pushing thread
+        const int tn = A.nband > 1 ? (int)blockDim.x : HC_WAVE;        // pushing threads
         for (int sp = 0; sp < A.ppw; sp++) {
-            const int pi = gwave * A.ppw + sp;
+            const int pi = (A.nband > 1 ? (int)blockIdx.x : gwave) * A.ppw + sp;
             if (pi >= A.npatch || pi >= A.ntiles) break;  // uniform across the wave
             const int st = A.slot0 + pi;
             const int* tb = A.push + 8 * st;
             R* ob = static_cast<R*>(A.qout_l[A.levels[st]]);
-            const R* sg = reinterpret_cast<const R*>(hc_smem) + ((threadIdx.x >> 6) * A.ppw + sp) * pstage;
+            const R* sg = reinterpret_cast<const R*>(hc_smem) + (swave * A.ppw + sp) * pstage;
             auto put = [&](int nb, int cc, int rr, int m, R v) {
                 ob[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
             };
             const int nx_items = 4 * ME * A.my;
-            // x edges: a lane takes one row of a 32-row block and one of the
-            // two edge columns (lane pairs store 8 contiguous bytes)
-            const int cb = lane & 1, rl = lane >> 1;
+            // x edges: a thread takes one row of a tn/2-row block and one of
+            // the two edge columns (thread pairs store 8 contiguous bytes)
+            const int cb = tl & 1, rl = tl >> 1;
 #pragma unroll
             for (int side = 0; side < 2; side++) {
                 const int nb = tb[side], nbl = tb[4 + side], nbh = tb[6 + side];
@@ -875,7 +908,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
                     const R* sx = sg + ((side * 2 + cb) * ME + m) * A.my;
-                    for (int r = rl; r < A.my; r += HC_WAVE / 2) {
+                    for (int r = rl; r < A.my; r += tn / 2) {
                         const int Rw = r + 2;
                         const R v = sx[r];
                         if (nb >= 0) put(nb, c, Rw, m, v);
@@ -884,7 +917,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     }
                 }
             }
-            // y edges: a lane takes one column of a ghost row
+            // y edges: a thread takes one column of a ghost row
 #pragma unroll
             for (int side = 0; side < 2; side++) {
                 const int nb = tb[2 + side];
@@ -895,7 +928,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                     for (int m = 0; m < ME; m++) {
                         const R* sy = sg + nx_items + ((side * 2 + rr) * ME + m) * A.mx;
-                        for (int i = lane; i < A.mx; i += HC_WAVE) put(nb, i + 2, Rt, m, sy[i]);
+                        for (int i = tl; i < A.mx; i += tn) put(nb, i + 2, Rt, m, sy[i]);
                     }
                 }
             }
@@ -911,6 +944,18 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     for (int d = 1; d < HC_WAVE; d <<= 1) {
         double t = __shfl_up(cd, d, HC_WAVE);
         if (lp - d >= 0) cd = dmax(cd, t);
+    }
+    if (A.nband > 1) {
+        // the bands of a patch are the waves of one workgroup: max over them
+        __shared__ double cfl_band[4][HC_WAVE];
+        if (lp == A.lpp - 1 && slot < A.ppw) cfl_band[band][slot] = cd;
+        __syncthreads();
+        if (band == 0 && pact && lp == A.lpp - 1 && cflp) {
+            double m = cfl_band[0][slot];
+            for (int b = 1; b < A.nband; b++) m = dmax(m, cfl_band[b][slot]);
+            cflp[tix] = m;
+        }
+        return;
     }
     if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
 }

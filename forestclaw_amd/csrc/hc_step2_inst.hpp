@@ -10,7 +10,10 @@ template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC
 static void launch_k(const StepArgs& a, long long blocks, int wpg, const void* qin, void* qout,
                      const void* aux, double* cfl, hipStream_t s)
 {
-    size_t lds = PUSH ? sizeof(R) * (size_t)wpg * a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) : 0;
+    // push staging per wave's patches (shared by the row bands of a patch),
+    // then the swirl row sines per wave
+    const size_t nstage = a.nband > 1 ? 1 : (size_t)wpg;
+    size_t lds = PUSH ? sizeof(R) * nstage * a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) : 0;
     if (a.swirl) lds = ((lds + 7) & ~(size_t)7) + sizeof(double) * (size_t)wpg * a.ppw * 2 * (a.my + 4);
     hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH>), dim3((unsigned)blocks),
                        dim3(HC_WAVE * wpg), lds, s, a, static_cast<const R*>(qin),
@@ -28,16 +31,29 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     if (a.lpp > HC_WAVE) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patch too wide for one wave");
     a.ppw = HC_WAVE / a.lpp;
     if (a.push) {
-        // the push stage must fit a workgroup's LDS: fewer waves per group if needed
-        const size_t per_wave = sizeof(R) * (size_t)a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) +
-                                (a.swirl ? sizeof(double) * (size_t)a.ppw * 2 * (a.my + 4) + 8 : 0);
-        while (wpg > 1 && per_wave * wpg > 65536) wpg--;
-        if (per_wave > 65536) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patches too large for the ghost push");
+        // the push stage must fit a workgroup's LDS: fewer waves per group
+        // (fewer row bands) if needed
+        const size_t stg = sizeof(R) * (size_t)a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my);
+        const size_t sw = a.swirl ? sizeof(double) * (size_t)a.ppw * 2 * (a.my + 4) + 8 : 0;
+        if (a.nband > 1) {
+            int nb = a.nband;
+            while (nb > 1 && stg + sw * nb > 65536) nb--;
+            if (nb < a.nband) {
+                a.band_rows = nb > 1 ? (a.my + nb - 1) / nb : a.my;
+                a.nband = nb > 1 ? (a.my + a.band_rows - 1) / a.band_rows : 1;
+                wpg = a.nband;
+            }
+        }
+        while (a.nband < 2 && wpg > 1 && (stg + sw) * wpg > 65536) wpg--;
+        if (stg + sw > 65536) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patches too large for the ghost push");
     }
     if (a.swirl && !(SolverT<R>::MAUX > 0)) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the advection solver");
     if (a.swirl && !a.xylower) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the patch corners");
     const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
-    const long long blocks = (waves + wpg - 1) / wpg;
+    // banded: one workgroup of nband waves per wave's worth of patches
+    if (a.nband > 1 && (wpg != a.nband || a.nband > 4 || a.band_rows < 2))
+        return set_error(HIPCLAW_ERR_ARG, "row bands: 2..4 waves of one workgroup, 2 rows or more each");
+    const long long blocks = a.nband > 1 ? waves : (waves + wpg - 1) / wpg;
     if (blocks == 0) return HIPCLAW_OK;
     bool mc22 = a.order2 == 2 && a.transverse == 2;
     for (int w = 0; w < MW; w++) mc22 = mc22 && a.mthlim[w] == 4;

@@ -205,12 +205,13 @@ class Hipclaw46:
         _check(_lib().hipclaw_output_ascii(self.h, iframe, self.t if time is None else time,
                                            directory.encode()))
 
-    def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None):
+    def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None,
+                   row_bands: int = 0):
         """Patch-step launch shape for this context (None: library default)."""
         if cols_per_lane is None:
             _check(_lib().hipclaw_set_launch(self.h, None))
         else:
-            lc = H.Launch(cols_per_lane, waves_per_group)
+            lc = H.Launch(cols_per_lane, waves_per_group, row_bands)
             _check(_lib().hipclaw_set_launch(self.h, ctypes.byref(lc)))
 
     def autotune(self, level: int = -1, reps: int = 3):

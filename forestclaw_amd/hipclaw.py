@@ -42,7 +42,8 @@ class Physics(ctypes.Structure):
 
 
 class Launch(ctypes.Structure):
-    _fields_ = [("cols_per_lane", ctypes.c_int), ("waves_per_group", ctypes.c_int)]
+    _fields_ = [("cols_per_lane", ctypes.c_int), ("waves_per_group", ctypes.c_int),
+                ("row_bands", ctypes.c_int)]
 
 
 class Domain(ctypes.Structure):

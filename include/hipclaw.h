@@ -70,11 +70,16 @@ typedef struct hipclaw_physics {
     double gamma;     /* Euler                                                 */
 } hipclaw_physics;
 
-/* Launch shape of the patch-step kernel (chosen by the autotuner when NULL):
- * cols_per_lane 1 or 2; waves_per_group 1..4. */
+/* Launch shape of the patch-step kernel (library default when NULL):
+ * cols_per_lane 1 or 2; waves_per_group 1..4; row_bands 0 or 1 sweeps every
+ * patch with one wave, 2..4 splits the interior rows of each patch into that
+ * many bands swept by the waves of one workgroup (waves_per_group is then
+ * ignored; launches in place sweep whole patches).  Bands change no bits.  The library default bands launches of at
+ * most 1,024 patches. */
 typedef struct hipclaw_launch {
     int cols_per_lane;
     int waves_per_group;
+    int row_bands;
 } hipclaw_launch;
 
 int hipclaw_abi_version(void);

@@ -241,6 +241,44 @@ def test_autotune_and_launch_shapes_bitwise(dev):
     sim.set_launch(None)
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+@pytest.mark.parametrize("case", ["swe", "swe_push", "swe_time_sync", "swirl"])
+@pytest.mark.parametrize("mx", [8, 10, 16])
+def test_row_bands_change_no_bits(dev, precision, case, mx):
+    """Patches swept in 2, 3 or 4 row bands by the waves of one workgroup give
+    the bits of whole-patch sweeps: interiors, per-patch Courant numbers and,
+    with time sync, the corrections from the flux registers; with the
+    same-level ghost push (swe_push, swirl), the pushed ghost cells (the last band
+    of a 10-row patch holds one row; 4 bands of an 8-row patch hold 2 each).
+    Each banded run is compared with the unbanded one of the same columns per
+    lane (in FP32 the two instantiations contract differently)."""
+    runs = []
+    for cpl in (1, 2):
+        for bands in (0, 2, 3, 4):
+            if case == "swirl":
+                sim = make(H.SOLVER_ADV, mx, 2, 3, precision=precision, box=(0.0, 1.0, 0.0, 0.75))
+                sim.set_swirl(8.0)
+                sim.initialize("swirl", (0.5,))
+                dt = 0.004
+            else:
+                opt = F.Options(minlevel=2, maxlevel=3, mx=mx, my=mx, solver=H.SOLVER_SWE_ROE,
+                                precision=precision, refine_threshold=0.5, coarsen_threshold=0.05,
+                                time_sync=case == "swe_time_sync")
+                sim = F.Hipclaw46(opt)
+                sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+                dt = 0.005
+            sim.set_feature(H.FEATURE_GHOST_PUSH, case in ("swe_push", "swirl"))
+            sim.set_launch(cpl, 1, bands)
+            cfl = [sim.advance(s * dt, dt) for s in range(3)]
+            runs.append(((cpl, bands), cfl, sim.get_q(), sim.num_patches()))
+            sim.close()
+    for shape, cfl, q, n in runs:
+        ref = runs[0] if shape[0] == 1 else runs[4]
+        assert ref[3] > 16
+        assert n == ref[3] and cfl == ref[1], shape
+        assert np.array_equal(interiors(q), interiors(ref[2])), shape
+
+
 @pytest.mark.parametrize("precision,launch", [(H.FP64, None), (H.FP32, None), (H.FP32, (1, 2)),
                                               (H.FP64, (1, 1))])
 def test_merged_launch_and_ghost_push_change_no_bits(dev, precision, launch):
