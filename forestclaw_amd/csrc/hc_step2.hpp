@@ -144,7 +144,7 @@ template <class R, int CPL, bool SYNC>
 constexpr int STEP_WPE = (!SYNC && (sizeof(R) == 4 || CPL == 1)) ? HC_STEP_MIN_WAVES : 1;
 
 template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH,
-          int UNR = STEP_UNR<R, CPL, SYNC>>
+          bool BAND = false, int UNR = STEP_UNR<R, CPL, SYNC>>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEP_WPE<R, CPL, SYNC>)))
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
@@ -158,8 +158,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int gwave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
     const int slot = lane / A.lpp;
     const int lp = lane - slot * A.lpp;
-    const int band = A.nband > 1 ? (int)(threadIdx.x >> 6) : 0;
-    const int pidx = (A.nband > 1 ? (int)blockIdx.x : gwave) * A.ppw + slot;
+    // wave-uniform (scalar) band index: the row loop and its conditions stay
+    // scalar when the band bounds derive from it
+    const int band = BAND ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    const int pidx = (BAND ? (int)blockIdx.x : gwave) * A.ppw + slot;
     const bool pin = (slot < A.ppw) && (pidx < A.npatch);
     const int praw = pin ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
     // a list entry outside the tile range is skipped, never dereferenced
@@ -168,10 +170,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int NX = A.mx + 4, NY = A.my + 4;
     // rows of this wave: interior rows [ra, rb) are written, the sweep runs
     // over rows J0 .. Jlast (the whole tile without bands)
-    const int ra = A.nband > 1 ? 2 + band * A.band_rows : 2;
-    const int rb = A.nband > 1 ? min(2 + (band + 1) * A.band_rows, NY - 2) : NY - 2;
-    const bool first_band = band == 0;
-    const bool last_band = A.nband <= 1 || band == A.nband - 1;
+    const int ra = BAND ? 2 + band * A.band_rows : 2;
+    const int rb = BAND ? min(2 + (band + 1) * A.band_rows, NY - 2) : NY - 2;
+    const bool first_band = !BAND || band == 0;
+    const bool last_band = !BAND || band == A.nband - 1;
     // a band's sweep starts on a multiple of the unroll factor, so every row
     // runs through the unrolled copy (and its FP32 contractions) that a
     // whole-patch sweep runs it through
@@ -195,8 +197,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     // of a patch share its area
     extern __shared__ char hc_smem[];
     const int pstage = 4 * ME * (A.mx + A.my);
-    const int swave = A.nband > 1 ? 0 : (int)(threadIdx.x >> 6);  // staging wave
-    const int nstage = A.nband > 1 ? 1 : (int)(blockDim.x >> 6);   // staging waves per group
+    const int swave = BAND ? 0 : (int)(threadIdx.x >> 6);  // staging wave
+    const int nstage = BAND ? 1 : (int)(blockDim.x >> 6);   // staging waves per group
     R* stage = reinterpret_cast<R*>(hc_smem) + (swave * A.ppw + (slot < A.ppw ? slot : 0)) * pstage;
     const double dx = pact ? A.dxdy[2 * tix] : 1.0;
     const double dy = pact ? A.dxdy[2 * tix + 1] : 1.0;
@@ -885,10 +887,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     // corners.
     if constexpr (PUSH) {
         __syncthreads();  // every thread of the block reaches this point
-        const int tl = A.nband > 1 ? (int)threadIdx.x : lane;          // pushing thread
-        const int tn = A.nband > 1 ? (int)blockDim.x : HC_WAVE;        // pushing threads
+        const int tl = BAND ? (int)threadIdx.x : lane;    // pushing thread
+        const int tn = BAND ? (int)blockDim.x : HC_WAVE;  // pushing threads
         for (int sp = 0; sp < A.ppw; sp++) {
-            const int pi = (A.nband > 1 ? (int)blockIdx.x : gwave) * A.ppw + sp;
+            const int pi = (BAND ? (int)blockIdx.x : gwave) * A.ppw + sp;
             if (pi >= A.npatch || pi >= A.ntiles) break;  // uniform across the wave
             const int st = A.slot0 + pi;
             const int* tb = A.push + 8 * st;
@@ -945,7 +947,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         double t = __shfl_up(cd, d, HC_WAVE);
         if (lp - d >= 0) cd = dmax(cd, t);
     }
-    if (A.nband > 1) {
+    if constexpr (BAND) {
         // the bands of a patch are the waves of one workgroup: max over them
         __shared__ double cfl_band[4][HC_WAVE];
         if (lp == A.lpp - 1 && slot < A.ppw) cfl_band[band][slot] = cd;
@@ -955,9 +957,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             for (int b = 1; b < A.nband; b++) m = dmax(m, cfl_band[b][slot]);
             cflp[tix] = m;
         }
-        return;
+    } else {
+        if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
     }
-    if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
 }
 
 }  // namespace hc

@@ -15,7 +15,16 @@ static void launch_k(const StepArgs& a, long long blocks, int wpg, const void* q
     const size_t nstage = a.nband > 1 ? 1 : (size_t)wpg;
     size_t lds = PUSH ? sizeof(R) * nstage * a.ppw * 4 * SolverT<R>::ME * (a.mx + a.my) : 0;
     if (a.swirl) lds = ((lds + 7) & ~(size_t)7) + sizeof(double) * (size_t)wpg * a.ppw * 2 * (a.my + 4);
-    hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH>), dim3((unsigned)blocks),
+    // row bands are instantiated for one column per lane (the small-launch shape)
+    if constexpr (CPL == 1) {
+        if (a.nband > 1) {
+            hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH, true>), dim3((unsigned)blocks),
+                               dim3(HC_WAVE * wpg), lds, s, a, static_cast<const R*>(qin),
+                               static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_step2<R, SolverT, CPL, MC22, SYNC, PUSH, false>), dim3((unsigned)blocks),
                        dim3(HC_WAVE * wpg), lds, s, a, static_cast<const R*>(qin),
                        static_cast<R*>(qout), static_cast<const R*>(aux), cfl);
 }
@@ -27,6 +36,11 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     constexpr int MW = SolverT<R>::MW;
     StepArgs a = a0;
     const int NX = a.mx + 4;
+    if (CPL != 1 && a.nband > 1) {  // no banded instantiation: whole-patch sweeps
+        wpg = 1;
+        a.nband = 1;
+        a.band_rows = a.my;
+    }
     a.lpp = (NX + CPL - 1) / CPL;
     if (a.lpp > HC_WAVE) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patch too wide for one wave");
     a.ppw = HC_WAVE / a.lpp;
