@@ -44,7 +44,7 @@ HBM_PEAK = 8.0e12
 # SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the level-9 launch
 # (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
 MEASURED_MESH = {7: 10481, 8: 2349, 9: 85052}
-TRAFFIC_PER_LAUNCH = 3.612e9
+TRAFFIC_PER_LAUNCH = 3.547e9
 VALU_ACTIVE = 0.83
 
 
