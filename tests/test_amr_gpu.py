@@ -251,7 +251,8 @@ def test_row_bands_change_no_bits(dev, precision, case, mx):
     same-level ghost push (swe_push, swirl), the pushed ghost cells (the last band
     of a 10-row patch holds one row; 4 bands of an 8-row patch hold 2 each).
     Each banded run is compared with the unbanded one of the same columns per
-    lane (in FP32 the two instantiations contract differently)."""
+    lane (in FP32 the two instantiations contract differently); with two
+    columns per lane the library sweeps whole patches whatever row_bands says."""
     runs = []
     for cpl in (1, 2):
         for bands in (0, 2, 3, 4):
