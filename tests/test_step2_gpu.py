@@ -116,8 +116,10 @@ def _oracle_batch(kind, q, aux, dx, dy, dt, order=(2, 2), mthlim=None):
 @pytest.mark.parametrize("kind,mx,my,n", [("swe", 32, 32, 37), ("swe", 16, 24, 11),
                                           ("euler", 16, 16, 13), ("adv", 32, 32, 9),
                                           ("adv", 20, 12, 7)])
-@pytest.mark.parametrize("launch", [(1, 4), (2, 2)])
+@pytest.mark.parametrize("launch", [(1, 4), (2, 2), (1, 1, 3), (1, 1, 4)])
 def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
+    """Whole-patch sweeps and row bands (3 and 4 waves per patch; the third
+    element of the launch) against the oracle, bitwise."""
     if kind == "euler" and launch[0] == 2:
         pytest.skip("Euler FP64 runs one column per lane")
     rng = np.random.default_rng(100 + n)
@@ -129,14 +131,16 @@ def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
     assert np.array_equal(cfl, rcfl)
 
 
-def test_patch_ids_and_inplace(dev):
+@pytest.mark.parametrize("launch", [None, (1, 1, 4)])
+def test_patch_ids_and_inplace(dev, launch):
+    """In place, row bands asked for or not, the library sweeps whole patches."""
     rng = np.random.default_rng(7)
     q, aux, dx, dy, dt = _batch("swe", rng, 12, 32, 32)
     ref, rcfl = _oracle_batch("swe", q, aux, dx, dy, dt)
     # step only every other tile, in place, in a permuted order; list entries
     # outside the tile range are skipped by the kernel
     ids = [10, 2, 6, 0, 8, 4, 12, -1]
-    got, cfl = run_gpu(dev, "swe", q, aux, dx, dy, dt, inplace=True, patch_ids=ids)
+    got, cfl = run_gpu(dev, "swe", q, aux, dx, dy, dt, inplace=True, patch_ids=ids, launch=launch)
     inner = (slice(None), slice(2, -2), slice(2, -2))
     for t in range(12):
         if t in ids:
