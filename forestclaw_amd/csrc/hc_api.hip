@@ -290,9 +290,10 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     // a launch of fewer patches than the chip has wave slots splits the rows
     // of every patch into bands, one wave each: the sweep, which is latency
     // bound there, gets 2 or 4 times shorter for 4 extra rows per band
-    int nband = launch ? launch->row_bands : (npatch <= 512 ? 4 : (npatch < 2048 ? 2 : 1));
-    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4 || nband < 0 || nband > 4)
-        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..4");
+    int nband = launch ? launch->row_bands
+                       : (npatch <= 256 ? 8 : (npatch <= 512 ? 4 : (npatch < 2048 ? 2 : 1)));
+    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4 || nband < 0 || nband > 8)
+        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..8");
     StepArgs a{};
     a.npatch = npatch;
     a.mx = phys->mx;

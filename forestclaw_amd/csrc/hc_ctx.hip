@@ -1032,8 +1032,8 @@ int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch)
         return HIPCLAW_OK;
     }
     if (launch->cols_per_lane < 1 || launch->cols_per_lane > 2 || launch->waves_per_group < 1 ||
-        launch->waves_per_group > 4 || launch->row_bands < 0 || launch->row_bands > 4)
-        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..4");
+        launch->waves_per_group > 4 || launch->row_bands < 0 || launch->row_bands > 8)
+        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..8");
     ctx->c->launch = *launch;
     ctx->c->clear_graphs();
     return HIPCLAW_OK;

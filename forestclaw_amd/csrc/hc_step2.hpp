@@ -145,7 +145,7 @@ constexpr int STEP_WPE = (!SYNC && (sizeof(R) == 4 || CPL == 1)) ? HC_STEP_MIN_W
 
 template <class R, template <class> class SolverT, int CPL, bool MC22, bool SYNC, bool PUSH,
           bool BAND = false, int UNR = STEP_UNR<R, CPL, SYNC>>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STEP_WPE<R, CPL, SYNC>)))
+__global__ void __launch_bounds__(BAND ? 512 : 256) __attribute__((amdgpu_waves_per_eu(STEP_WPE<R, CPL, SYNC>)))
 k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         const R* __restrict__ aux, double* __restrict__ cflp)
 {
@@ -949,7 +949,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
     if constexpr (BAND) {
         // the bands of a patch are the waves of one workgroup: max over them
-        __shared__ double cfl_band[4][HC_WAVE];
+        __shared__ double cfl_band[8][HC_WAVE];
         if (lp == A.lpp - 1 && slot < A.ppw) cfl_band[band][slot] = cd;
         __syncthreads();
         if (band == 0 && pact && lp == A.lpp - 1 && cflp) {

@@ -65,8 +65,8 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     if (a.swirl && !a.xylower) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the patch corners");
     const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
     // banded: one workgroup of nband waves per wave's worth of patches
-    if (a.nband > 1 && (wpg != a.nband || a.nband > 4 || a.band_rows < 2))
-        return set_error(HIPCLAW_ERR_ARG, "row bands: 2..4 waves of one workgroup, 2 rows or more each");
+    if (a.nband > 1 && (wpg != a.nband || a.nband > 8 || a.band_rows < 2))
+        return set_error(HIPCLAW_ERR_ARG, "row bands: 2..8 waves of one workgroup, 2 rows or more each");
     const long long blocks = a.nband > 1 ? waves : (waves + wpg - 1) / wpg;
     if (blocks == 0) return HIPCLAW_OK;
     bool mc22 = a.order2 == 2 && a.transverse == 2;

@@ -72,11 +72,11 @@ typedef struct hipclaw_physics {
 
 /* Launch shape of the patch-step kernel (library default when NULL):
  * cols_per_lane 1 or 2; waves_per_group 1..4; row_bands 0 or 1 sweeps every
- * patch with one wave, 2..4 splits the interior rows of each patch into that
+ * patch with one wave, 2..8 splits the interior rows of each patch into that
  * many bands swept by the waves of one workgroup (waves_per_group is then
  * ignored; launches in place or with cols_per_lane 2 sweep whole patches).
- * Bands change no bits.  The library default uses 4 bands for launches of at
- * most 512 patches and 2 for fewer than 2,048. */
+ * Bands change no bits.  The library default uses 8 bands for launches of at
+ * most 256 patches, 4 for at most 512 and 2 for fewer than 2,048. */
 typedef struct hipclaw_launch {
     int cols_per_lane;
     int waves_per_group;

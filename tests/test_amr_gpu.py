@@ -245,7 +245,7 @@ def test_autotune_and_launch_shapes_bitwise(dev):
 @pytest.mark.parametrize("case", ["swe", "swe_push", "swe_time_sync", "swirl"])
 @pytest.mark.parametrize("mx", [8, 10, 16])
 def test_row_bands_change_no_bits(dev, precision, case, mx):
-    """Patches swept in 2, 3 or 4 row bands by the waves of one workgroup give
+    """Patches swept in 2, 3, 4 or 8 row bands by the waves of one workgroup give
     the bits of whole-patch sweeps: interiors, per-patch Courant numbers and,
     with time sync, the corrections from the flux registers; with the
     same-level ghost push (swe_push, swirl), the pushed ghost cells (the last band
@@ -255,7 +255,7 @@ def test_row_bands_change_no_bits(dev, precision, case, mx):
     columns per lane the library sweeps whole patches whatever row_bands says."""
     runs = []
     for cpl in (1, 2):
-        for bands in (0, 2, 3, 4):
+        for bands in (0, 2, 3, 4, 8):
             if case == "swirl":
                 sim = make(H.SOLVER_ADV, mx, 2, 3, precision=precision, box=(0.0, 1.0, 0.0, 0.75))
                 sim.set_swirl(8.0)
@@ -274,7 +274,7 @@ def test_row_bands_change_no_bits(dev, precision, case, mx):
             runs.append(((cpl, bands), cfl, sim.get_q(), sim.num_patches()))
             sim.close()
     for shape, cfl, q, n in runs:
-        ref = runs[0] if shape[0] == 1 else runs[4]
+        ref = next(r for r in runs if r[0] == (shape[0], 0))
         assert ref[3] > 16
         assert n == ref[3] and cfl == ref[1], shape
         assert np.array_equal(interiors(q), interiors(ref[2])), shape

@@ -116,9 +116,9 @@ def _oracle_batch(kind, q, aux, dx, dy, dt, order=(2, 2), mthlim=None):
 @pytest.mark.parametrize("kind,mx,my,n", [("swe", 32, 32, 37), ("swe", 16, 24, 11),
                                           ("euler", 16, 16, 13), ("adv", 32, 32, 9),
                                           ("adv", 20, 12, 7)])
-@pytest.mark.parametrize("launch", [(1, 4), (2, 2), (1, 1, 3), (1, 1, 4)])
+@pytest.mark.parametrize("launch", [(1, 4), (2, 2), (1, 1, 3), (1, 1, 4), (1, 1, 8)])
 def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
-    """Whole-patch sweeps and row bands (3 and 4 waves per patch; the third
+    """Whole-patch sweeps and row bands (3, 4 and 8 waves per patch; the third
     element of the launch) against the oracle, bitwise."""
     if kind == "euler" and launch[0] == 2:
         pytest.skip("Euler FP64 runs one column per lane")
