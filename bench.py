@@ -48,15 +48,6 @@ TRAFFIC_PER_LAUNCH = 3.547e9
 VALU_ACTIVE = 0.83
 
 
-def humps(seed=7, n=8):
-    rng = np.random.default_rng(seed)
-    p = [1.0, n]
-    for _ in range(n):
-        p += [rng.uniform(0.15, 0.85), rng.uniform(0.15, 0.85), rng.uniform(0.1, 0.5),
-              rng.uniform(0.02, 0.1)]
-    return p
-
-
 _STREAM = []
 
 
@@ -64,17 +55,14 @@ def make_sim(precision, args, leaves=None, q=None):
     """Simulation on a dedicated (default-priority) stream: graph capture
     needs a stream other than the null stream."""
     import torch
+    from forestclaw_amd import configs as CF
     from forestclaw_amd import forest as F
-    from forestclaw_amd import hipclaw as H
     if not _STREAM:
         _STREAM.append(torch.cuda.Stream())
-    opt = F.Options(minlevel=args.minlevel, maxlevel=args.maxlevel, mx=32, my=32,
-                    solver=H.SOLVER_SWE_ROE, precision=precision, grav=1.0,
-                    refine_threshold=args.refine_threshold, coarsen_threshold=0.0,
-                    subcycle=True)
+    opt = CF.c5_options(precision, args.minlevel, args.maxlevel, args.refine_threshold)
     sim = F.Hipclaw46(opt, stream=_STREAM[0].cuda_stream)
     if leaves is None:
-        sim.initialize("humps", humps(args.seed))
+        sim.initialize("humps", CF.c5_humps(args.seed))
     else:
         sim.init_leaves(leaves["level"], leaves["gx"], leaves["gy"])
         sim.set_q(q)
@@ -468,8 +456,8 @@ def main():
     for key in ("level", "gx", "gy"):
         assert np.array_equal(info32[key], info[key]), "FP32 run is not on the FP64 mesh"
     levels = {int(L): int((info["level"] == L).sum()) for L in np.unique(info["level"])}
-    c_max = 1.0 + np.sqrt(1.0 * 1.6)
-    dt = 0.9 * info["dx"].max() / c_max  # CFL 0.9 at the coarsest level (subcycled)
+    from forestclaw_amd import configs as CF
+    dt = CF.c5_dt(float(info["dx"].max()))  # CFL 0.9 at the coarsest level (subcycled)
 
     # launch shape of the patch-step kernel picked by the autotuner on the
     # level with the most patches (timed into a scratch buffer)

@@ -76,6 +76,28 @@ def c3_dt(coarsest=3):
     return 0.8 * dx / 3.6  # bubble sound speed sqrt(1.4/0.138) = 3.2 plus the flow
 
 
+def c5_humps(seed=7, n=8):
+    """qinit parameters of C5: h = 1 + sum of n Gaussian humps with seeded
+    centres U(0.15, 0.85)^2, amplitudes U(0.1, 0.5) and widths U(0.02, 0.1)."""
+    rng = np.random.default_rng(seed)
+    p = [1.0, n]
+    for _ in range(n):
+        p += [rng.uniform(0.15, 0.85), rng.uniform(0.15, 0.85), rng.uniform(0.1, 0.5),
+              rng.uniform(0.02, 0.1)]
+    return p
+
+
+def c5_options(precision=H.FP64, minlevel=7, maxlevel=9, refine_threshold=0.5):
+    return F.Options(minlevel=minlevel, maxlevel=maxlevel, mx=32, my=32, solver=H.SOLVER_SWE_ROE,
+                     precision=precision, grav=1.0, refine_threshold=refine_threshold,
+                     coarsen_threshold=0.0, subcycle=True)
+
+
+def c5_dt(dx_coarse):
+    # CFL 0.9 at the coarsest level for |u| + sqrt(g h) <= 1 + sqrt(1.6)
+    return 0.9 * dx_coarse / (1.0 + math.sqrt(1.6))
+
+
 def build(name, precision=H.FP64, stream=None):
     """(simulation, coarse dt) with initial data and the initial refinement;
     stream: the HIP stream of the simulation (default: torch's current one;
@@ -94,6 +116,10 @@ def build(name, precision=H.FP64, stream=None):
         sim = F.Hipclaw46(c3_options(precision), stream=stream)
         sim.initialize("shock_bubble", c3_params(), nsub=4)
         return sim, c3_dt(int(sim.patch_info()["level"].min()))
+    if name == "C5":
+        sim = F.Hipclaw46(c5_options(precision), stream=stream)
+        sim.initialize("humps", c5_humps())
+        return sim, c5_dt(float(sim.patch_info()["dx"].max()))
     raise ValueError(name)
 
 

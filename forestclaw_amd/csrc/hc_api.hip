@@ -262,6 +262,37 @@ static int step2_batch_impl(const hipclaw_physics* phys, int npatch, const void*
 
 namespace hc {
 
+// Launch shape of one patch-step launch.  The library default: two columns
+// per lane on FP32 launches of 2,048 patches or more with wide patches (three
+// patches per wave pay there), else one; one wave per workgroup below 1,024
+// patches (a small launch spreads its waves one per workgroup); and below
+// 2,048 patches row bands (8 up to 256 patches, 4 up to 512, else 2): a
+// launch of fewer patches than the chip has wave slots is bound by the
+// latency of one sweep, which bands shorten.  A caller's shape with
+// row_bands = -1 is used for launches of 2,048 patches or more and the
+// default below that; row bands apply to one column per lane only (two
+// columns per lane sweep whole patches with the caller's waves per group).
+int choose_launch(const hipclaw_physics* phys, int npatch, const hipclaw_launch* launch, int* cpl,
+                  int* wpg, int* nband)
+{
+    if (launch && (launch->cols_per_lane < 1 || launch->cols_per_lane > 2 || launch->waves_per_group < 1 ||
+                   launch->waves_per_group > 4 || launch->row_bands < -1 || launch->row_bands > 8))
+        return set_error(HIPCLAW_ERR_ARG,
+                         "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands -1..8");
+    const bool small = npatch < 2048;
+    if (launch && !(launch->row_bands < 0 && small)) {
+        *cpl = launch->cols_per_lane;
+        *wpg = launch->waves_per_group;
+        *nband = launch->row_bands < 0 ? 1 : launch->row_bands;
+    } else {
+        *cpl = (phys->precision == HIPCLAW_FP32 && phys->mx + 4 > 32 && !small) ? 2 : 1;
+        *wpg = npatch < 1024 ? 1 : 4;
+        *nband = npatch <= 256 ? 8 : (npatch <= 512 ? 4 : (small ? 2 : 1));
+    }
+    if (*cpl != 1) *nband = 1;
+    return HIPCLAW_OK;
+}
+
 int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                  const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,
                  double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,
@@ -276,24 +307,9 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     if (!q_in || !q_out || !dxdy) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
     if (phys->maux > 0 && !aux) return set_error(HIPCLAW_ERR_ARG, "aux is NULL but maux > 0");
     if (!(dt >= 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be >= 0");
-    int cpl = 1, wpg = 4;
-    if (launch) {
-        cpl = launch->cols_per_lane;
-        wpg = launch->waves_per_group;
-    } else if (phys->precision == HIPCLAW_FP32 && phys->mx + 4 > 32 && npatch >= 2048) {
-        // two columns per lane (three patches per wave) pays on large
-        // launches; a small one needs every patch on a wave of its own
-        cpl = 2;
-    }
-    // a launch of fewer waves than four per CU spreads them one per workgroup
-    if (!launch && npatch < 1024) wpg = 1;
-    // a launch of fewer patches than the chip has wave slots splits the rows
-    // of every patch into bands, one wave each: the sweep, which is latency
-    // bound there, gets 2 or 4 times shorter for 4 extra rows per band
-    int nband = launch ? launch->row_bands
-                       : (npatch <= 256 ? 8 : (npatch <= 512 ? 4 : (npatch < 2048 ? 2 : 1)));
-    if (cpl < 1 || cpl > 2 || wpg < 1 || wpg > 4 || nband < 0 || nband > 8)
-        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..8");
+    int cpl, wpg, nband;
+    rc = choose_launch(phys, npatch, launch, &cpl, &wpg, &nband);
+    if (rc) return rc;
     StepArgs a{};
     a.npatch = npatch;
     a.mx = phys->mx;
@@ -335,7 +351,7 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     }
     // bands of at least two rows each, none empty; an update in place must
     // not overwrite rows that another band of the patch still reads, so it
-    // sweeps unbanded
+    // sweeps unbanded (choose_launch gives bands to one column per lane only)
     if (nband < 2 || q_in == q_out) {
         a.nband = 1;
         a.band_rows = a.my;
@@ -355,6 +371,26 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
 }  // namespace hc
 
 extern "C" {
+
+int hipclaw_launch_shape(const hipclaw_physics* phys, int npatch, const hipclaw_launch* launch,
+                         hipclaw_launch* out)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (!out || npatch < 0) return set_error(HIPCLAW_ERR_ARG, "bad launch_shape args");
+    int cpl, wpg, nband;
+    rc = choose_launch(phys, npatch, launch, &cpl, &wpg, &nband);
+    if (rc) return rc;
+    // the band count the launch really uses: bands of at least two rows
+    int rows = nband > 1 ? (phys->my + nband - 1) / nband : phys->my;
+    if (rows < 2) rows = 2;
+    nband = nband > 1 ? (phys->my + rows - 1) / rows : 1;
+    if (nband < 2) nband = 1;
+    out->cols_per_lane = cpl;
+    out->waves_per_group = nband > 1 ? nband : wpg;
+    out->row_bands = nband;
+    return HIPCLAW_OK;
+}
 
 int hipclaw_max_reduce(const double* x_dev, int n, double* out_dev, void* stream)
 {

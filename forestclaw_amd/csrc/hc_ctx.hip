@@ -462,7 +462,7 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     for (int cpl = 1; cpl <= 2 && !rc; cpl++)
         for (int wpg : {1, 2, 4}) {
             if (rc) break;
-            hipclaw_launch l{cpl, wpg};
+            hipclaw_launch l{cpl, wpg, 0};
             // one untimed launch, then reps timed ones
             rc = run(l);
             if (!rc) rc = check_hip(hipEventRecord(e0, stream), "event");
@@ -476,7 +476,9 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
             k++;
             if (!rc && t < bestms) {
                 bestms = t;
-                launch = l;
+                // the shape for launches of this level's size; smaller
+                // launches keep the library's size-based shape and bands
+                launch = hipclaw_launch{cpl, wpg, -1};
             }
         }
     if (e0) (void)hipEventDestroy(e0);
@@ -1031,9 +1033,8 @@ int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch)
         ctx->c->clear_graphs();
         return HIPCLAW_OK;
     }
-    if (launch->cols_per_lane < 1 || launch->cols_per_lane > 2 || launch->waves_per_group < 1 ||
-        launch->waves_per_group > 4 || launch->row_bands < 0 || launch->row_bands > 8)
-        return set_error(HIPCLAW_ERR_ARG, "launch: cols_per_lane 1..2, waves_per_group 1..4, row_bands 0..8");
+    int cpl, wpg, nband;
+    HC_TRY(choose_launch(&ctx->c->phys, 1, launch, &cpl, &wpg, &nband));  // validates the shape
     ctx->c->launch = *launch;
     ctx->c->clear_graphs();
     return HIPCLAW_OK;
@@ -1144,6 +1145,14 @@ int hipclaw_tag_stats(hipclaw_ctx* ctx, int* stats6)
     CTX_CHECK(ctx);
     if (!stats6) return set_error(HIPCLAW_ERR_ARG, "stats6 is NULL");
     std::memcpy(stats6, ctx->c->tag_stats, sizeof(ctx->c->tag_stats));
+    return HIPCLAW_OK;
+}
+
+int hipclaw_get_launch(hipclaw_ctx* ctx, hipclaw_launch* launch)
+{
+    CTX_CHECK(ctx);
+    if (!launch) return set_error(HIPCLAW_ERR_ARG, "launch is NULL");
+    *launch = ctx->c->launch;
     return HIPCLAW_OK;
 }
 

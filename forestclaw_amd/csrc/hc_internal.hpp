@@ -52,6 +52,12 @@ int launch_step2_fp32(const StepArgs& a, int solver, int cpl, int wpg, const voi
 // the message set.
 int validate_physics(const hipclaw_physics* p);
 
+// Effective launch shape (columns per lane, waves per workgroup, row bands)
+// of a patch-step launch of npatch patches for a caller's shape (NULL: the
+// library default).  Host-only.
+int choose_launch(const hipclaw_physics* phys, int npatch, const hipclaw_launch* launch, int* cpl,
+                  int* wpg, int* nband);
+
 // Patch-step launch behind hipclaw_step2_batch(_sync); with ml, one launch
 // over the slots [ml->slot0, ml->slot0 + npatch) of several levels (q_in /
 // q_out unused, dxdy / aux / cfl / reg addressed by slot).

@@ -36,11 +36,8 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
     constexpr int MW = SolverT<R>::MW;
     StepArgs a = a0;
     const int NX = a.mx + 4;
-    if (CPL != 1 && a.nband > 1) {  // no banded instantiation: whole-patch sweeps
-        wpg = 1;
-        a.nband = 1;
-        a.band_rows = a.my;
-    }
+    if (CPL != 1 && a.nband > 1)  // choose_launch gives bands to one column per lane only
+        return set_error(HIPCLAW_ERR_ARG, "row bands need one column per lane");
     a.lpp = (NX + CPL - 1) / CPL;
     if (a.lpp > HC_WAVE) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patch too wide for one wave");
     a.ppw = HC_WAVE / a.lpp;

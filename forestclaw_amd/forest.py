@@ -206,13 +206,21 @@ class Hipclaw46:
                                            directory.encode()))
 
     def set_launch(self, cols_per_lane: int | None = None, waves_per_group: int | None = None,
-                   row_bands: int = 0):
-        """Patch-step launch shape for this context (None: library default)."""
+                   row_bands: int = -1):
+        """Patch-step launch shape for this context (None: library default).
+        row_bands -1 keeps the library's shape for launches of fewer than
+        2,048 patches; 0 sweeps whole patches; 2..8 bands."""
         if cols_per_lane is None:
             _check(_lib().hipclaw_set_launch(self.h, None))
         else:
             lc = H.Launch(cols_per_lane, waves_per_group, row_bands)
             _check(_lib().hipclaw_set_launch(self.h, ctypes.byref(lc)))
+
+    def get_launch(self):
+        """(cols_per_lane, waves_per_group, row_bands); cols_per_lane 0 = library default."""
+        lc = H.Launch()
+        _check(_lib().hipclaw_get_launch(self.h, ctypes.byref(lc)))
+        return lc.cols_per_lane, lc.waves_per_group, lc.row_bands
 
     def autotune(self, level: int = -1, reps: int = 3):
         """Time the six launch shapes on one level and keep the fastest.

@@ -75,6 +75,8 @@ _SIGS = {
                                   _VP, ctypes.c_int, ctypes.c_double, _VP, _VP,
                                   ctypes.POINTER(Launch), _VP], ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_launch_shape": ([ctypes.POINTER(Physics), ctypes.c_int, ctypes.c_void_p,
+                              ctypes.POINTER(Launch)], ctypes.c_int),
     "hipclaw_cnn_create": ([_VP, ctypes.POINTER(_VP)], ctypes.c_int),
     "hipclaw_cnn_destroy": ([_VP], ctypes.c_int),
     "hipclaw_cnn_num_params": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
@@ -116,6 +118,7 @@ _SIGS.update({
     "hipclaw_init_uniform": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
     "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
+    "hipclaw_get_launch": ([_VP, ctypes.POINTER(Launch)], ctypes.c_int),
     "hipclaw_set_time_sync": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_src2": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_feature": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),
@@ -337,6 +340,17 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def launch_shape(phys: Physics, npatch: int, launch=None):
+    """(cols_per_lane, waves_per_group, row_bands) a patch-step launch of
+    npatch patches really uses for the shape `launch` (None: library
+    default).  Host-only."""
+    ln = Launch(*launch) if launch is not None else None
+    out = Launch()
+    _check(lib().hipclaw_launch_shape(ctypes.byref(phys), npatch,
+                                      ctypes.byref(ln) if ln is not None else None, ctypes.byref(out)))
+    return out.cols_per_lane, out.waves_per_group, out.row_bands
 
 
 def max_reduce(x, n: int, out, stream=None) -> None:

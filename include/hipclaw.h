@@ -74,9 +74,15 @@ typedef struct hipclaw_physics {
  * cols_per_lane 1 or 2; waves_per_group 1..4; row_bands 0 or 1 sweeps every
  * patch with one wave, 2..8 splits the interior rows of each patch into that
  * many bands swept by the waves of one workgroup (waves_per_group is then
- * ignored; launches in place or with cols_per_lane 2 sweep whole patches).
- * Bands change no bits.  The library default uses 8 bands for launches of at
- * most 256 patches, 4 for at most 512 and 2 for fewer than 2,048. */
+ * ignored; launches in place or with cols_per_lane 2 sweep whole patches
+ * with the given waves_per_group).  row_bands = -1: the given columns per
+ * lane and waves per group for launches of 2,048 patches or more, the library
+ * default below that.  Launch shapes change no bits in FP64; FP32 results of
+ * one and two columns per lane differ by contraction rounding.  The library
+ * default: two columns per lane for FP32 launches of 2,048 patches or more
+ * (mx > 28), else one; one wave per workgroup below 1,024 patches; 8 bands
+ * for launches of at most 256 patches, 4 for at most 512 and 2 for fewer
+ * than 2,048. */
 typedef struct hipclaw_launch {
     int cols_per_lane;
     int waves_per_group;
@@ -88,6 +94,14 @@ const char* hipclaw_last_error(void);
 
 /* Fill in the defaults of the reference options for a solver family. */
 int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out);
+
+/* The launch shape a patch-step launch of npatch patches uses for the
+ * caller's shape launch (NULL: the library default), with row_bands the
+ * band count it really sweeps (1 = whole patches) and waves_per_group the
+ * workgroup size in waves.  Host-only: no device call.  No reference
+ * counterpart (the reference has one fixed loop nest per patch). */
+int hipclaw_launch_shape(const hipclaw_physics* phys, int npatch, const hipclaw_launch* launch,
+                         hipclaw_launch* out);
 
 /* ------------------------------------------------------------------------
  * Batched single-step update.
@@ -277,12 +291,15 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
 int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on);
 /* Coarse steps captured as graphs and replays of captured graphs so far. */
 int hipclaw_graph_stats(hipclaw_ctx* ctx, long long* captures, long long* replays);
-/* Patch-step launch shape used by this context (NULL = library default). */
+/* Patch-step launch shape used by this context (NULL = library default);
+ * _get returns it (cols_per_lane 0 = the library default). */
 int hipclaw_set_launch(hipclaw_ctx* ctx, const hipclaw_launch* launch);
+int hipclaw_get_launch(hipclaw_ctx* ctx, hipclaw_launch* launch);
 /* Autotuner: times every launch shape (cols_per_lane 1..2 x waves_per_group
  * 1, 2, 4; reps timed launches each, after one untimed) on the current data of
  * one level (level < 0: the level with the most patches), writing into a
- * temporary buffer, and keeps the fastest for this context.  The kernel
+ * temporary buffer, and keeps the fastest for this context with row_bands =
+ * -1 (launches of fewer than 2,048 patches keep the library default).  The kernel
  * variant timed is the one the level updates launch (with the same-level
  * ghost push when that is on).  ms (optional) receives the 6 average times in
  * that order.  No reference counterpart: the reference has one fixed loop

@@ -63,3 +63,28 @@ def test_bad_limiter_and_launch_rejected():
     p = H.physics(H.SOLVER_SWE_ROE, 32, 32)
     with pytest.raises(H.HipclawError):
         H.step2_batch(p, 4, 1, 1, None, 1, 0.1, launch=(3, 4), stream=0)
+
+
+def test_launch_shape_defaults_and_tuned_shapes():
+    """Host-side launch shapes: the library default bands small launches and
+    gives FP32 launches of 2,048 wide patches or more two columns per lane; a
+    tuned shape with row_bands -1 applies only to launches of 2,048 patches
+    or more; two columns per lane never band and keep their waves per group."""
+    p64 = H.physics(H.SOLVER_SWE_ROE, 32, 32)
+    p32 = H.physics(H.SOLVER_SWE_ROE, 32, 32, precision=H.FP32)
+    assert H.launch_shape(p64, 200) == (1, 8, 8)
+    assert H.launch_shape(p64, 400) == (1, 4, 4)
+    assert H.launch_shape(p64, 1500) == (1, 2, 2)
+    assert H.launch_shape(p64, 5000) == (1, 4, 1)
+    assert H.launch_shape(p32, 5000) == (2, 4, 1)
+    for n in (200, 1500):
+        assert H.launch_shape(p64, n, (2, 2, -1)) == H.launch_shape(p64, n)
+    assert H.launch_shape(p64, 5000, (2, 2, -1)) == (2, 2, 1)
+    assert H.launch_shape(p64, 200, (2, 4, 4)) == (2, 4, 1)
+    assert H.launch_shape(p64, 200, (1, 1, 0)) == (1, 1, 1)
+    assert H.launch_shape(p64, 200, (1, 1, 3)) == (1, 3, 3)
+    # bands of at least two rows: an 8-row patch takes at most 4
+    p8 = H.physics(H.SOLVER_SWE_ROE, 8, 8)
+    assert H.launch_shape(p8, 10) == (1, 4, 4)
+    with pytest.raises(H.HipclawError):
+        H.launch_shape(p64, 10, (1, 1, -2))

@@ -119,14 +119,114 @@ def test_interpolate2fine_then_average2coarse_is_identity_on_linear_data(L):
     assert np.array_equal(back[:, 2:-2, 2:-2], qc[:, 2:-2, 2:-2])
 
 
-def test_slope_limiter_closed_form():
+def _minmod(*a):
+    """Textbook minmod: the argument of least magnitude when all share a
+    sign, else 0."""
+    if all(x > 0 for x in a):
+        return min(a)
+    if all(x < 0 for x in a):
+        return max(a)
+    return 0.0
+
+
+def _mc(sl, sr):
+    """Monotonized-central slope (van Leer 1977; LeVeque, Finite Volume
+    Methods for Hyperbolic Problems, eq. 6.29): minmod(2 sl, 2 sr, (sl+sr)/2)."""
+    return _minmod(2 * sl, 2 * sr, (sl + sr) / 2)
+
+
+def _slopes():
     s = O.lib().orc_compute_slopes
     s.argtypes = [ctypes.c_double, ctypes.c_double]
     s.restype = ctypes.c_double
-    for sl, sr in [(1.0, 1.0), (1.0, 3.0), (3.0, 1.0), (-1.0, 2.0), (0.0, 5.0), (-2.0, -0.5)]:
-        sc = (sl + sr) / 2
-        expect = min(2 * abs(sl), 2 * abs(sr), abs(sc)) * (1.0 if sl * sr > 0 else 0.0) * np.sign(sc)
-        assert s(sl, sr) == pytest.approx(expect, abs=0)
+    return s
+
+
+def test_interpolation_slope_is_the_mc_slope():
+    """The AMRClaw slope of the interpolation (mth = 5) is the MC slope.
+    The table separates it from minmod and from the unlimited centred slope:
+    (1, 3) gives 2 (minmod 1), (1, 5) gives 2 (centred 3), opposite signs
+    give 0 (centred nonzero)."""
+    s = _slopes()
+    table = {(1.0, 3.0): 2.0, (3.0, 1.0): 2.0, (1.0, 5.0): 2.0, (5.0, 1.0): 2.0,
+             (-1.0, -5.0): -2.0, (-1.0, 2.0): 0.0, (2.0, -1.0): 0.0, (0.0, 5.0): 0.0,
+             (1.0, 1.0): 1.0, (-2.0, -0.5): -1.0, (0.25, 0.5): 0.375}
+    for (sl, sr), want in table.items():
+        assert s(sl, sr) == want, (sl, sr)
+        assert _mc(sl, sr) == want, (sl, sr)
+    rng = np.random.default_rng(9)
+    for sl, sr in rng.standard_normal((2000, 2)) * np.array([1.0, 3.0]):
+        assert s(sl, sr) == _mc(sl, sr)
+
+
+def _centres(level, gx, gy, mx, my):
+    """Cell-centre coordinates (X, Y) of a tile (ghost cells included) of the
+    level-`level` patch (gx, gy) of the unit square."""
+    h = 2.0 ** (-level)
+    x = gx * h + (np.arange(mx + 4) - 2 + 0.5) * (h / mx)
+    y = gy * h + (np.arange(my + 4) - 2 + 0.5) * (h / my)
+    return np.meshgrid(x, y)
+
+
+def _expected_interp(qc, level, cg, X, Y, mx, my, slope):
+    """Value at fine-cell centre (X, Y): the coarse cell containing it, plus
+    the limited x and y slopes times the offset in coarse-cell units."""
+    h = 2.0 ** (-level)
+    hc = h / mx
+    fi = (X - cg[0] * h) / hc      # position in coarse cells from the patch corner
+    fj = (Y - cg[1] * h) / hc
+    ic, jc = int(np.floor(fi)), int(np.floor(fj))  # 0-based interior index
+    ox, oy = fi - (ic + 0.5), fj - (jc + 0.5)      # +-1/4
+    I, J = ic + 2, jc + 2                          # tile index
+    v = qc[J, I]
+    gx = slope(v - qc[J, I - 1], qc[J, I + 1] - v)
+    gy = slope(v - qc[J - 1, I], qc[J + 1, I] - v)
+    return v + gx * ox + gy * oy
+
+
+@pytest.mark.parametrize("side", range(8))
+def test_interpolation_of_nonlinear_data_by_coordinates(L, side):
+    """Coarse patch (level 1, (1, 1)) with random data; a half-size
+    neighbour across one of its faces (sides 0-3, the lower or left half)
+    or corners (4-7).  Every fine ghost cell the interpolation writes equals
+    the coarse cell that contains its centre plus MC-limited slopes times its
+    offset, all located from coordinates.  The same data with minmod or
+    unlimited slopes gives different values, so the check separates the
+    limiters."""
+    mx = my = 8
+    cg = (1, 1)
+    if side < 4:
+        iface = side
+        fg = [(1, 2), (4, 2), (2, 1), (2, 4)][iface]  # level 2, across coarse face iface
+    else:
+        icorner = side - 4
+        fg = [(1, 1), (4, 1), (1, 4), (4, 4)][icorner]
+    ox, oy = (2 * cg[0] - fg[0]) * mx, (2 * cg[1] - fg[1]) * my
+    X, Y = _centres(2, fg[0], fg[1], mx, my)
+    # the fine patch's ghost band facing the coarse patch
+    band = {0: (slice(2, -2), slice(-2, None)), 1: (slice(2, -2), slice(0, 2)),
+            2: (slice(-2, None), slice(2, -2)), 3: (slice(0, 2), slice(2, -2)),
+            4: (slice(-2, None), slice(-2, None)), 5: (slice(-2, None), slice(0, 2)),
+            6: (slice(0, 2), slice(-2, None)), 7: (slice(0, 2), slice(0, 2))}[side]
+    diff = {"minmod": 0, "centred": 0}
+    for seed in range(6):
+        qc = np.random.default_rng(30 + 8 * seed + side).standard_normal((1, my + 4, mx + 4))
+        qf = np.full((1, my + 4, mx + 4), np.nan)
+        if side < 4:
+            L.orc_interpolate_face(mx, my, 2, 1, P(qc), P(qf), iface, ox, oy)
+        else:
+            L.orc_interpolate_corner(mx, my, 2, 1, P(qc), P(qf), icorner, ox, oy)
+        written = ~np.isnan(qf[0])
+        assert written[band].all()
+        assert not written[2:-2, 2:-2].any()  # interior untouched
+        for j, i in zip(*np.nonzero(written)):
+            want = _expected_interp(qc[0], 1, cg, X[j, i], Y[j, i], mx, my, _mc)
+            assert qf[0, j, i] == want, (side, j, i)
+            diff["minmod"] += qf[0, j, i] != _expected_interp(
+                qc[0], 1, cg, X[j, i], Y[j, i], mx, my, lambda a, b: _minmod(a, b))
+            diff["centred"] += qf[0, j, i] != _expected_interp(
+                qc[0], 1, cg, X[j, i], Y[j, i], mx, my, lambda a, b: (a + b) / 2)
+    assert diff["minmod"] > 0 and diff["centred"] > 0, diff
 
 
 def _uniform_oracle(level, mx=8, meqn=3):
