@@ -46,9 +46,24 @@ HBM_PEAK = 8.0e12
 MEASURED_MESH = {7: 10481, 8: 2349, 9: 85052}
 TRAFFIC_PER_LAUNCH = 3.547e9
 VALU_ACTIVE = 0.83
+# FP32 VALU work of the level-9 step launch per cell update: wave-level VALU
+# instructions, and FP32 operations issued (64 lanes x (ADD + MUL + 2 FMA +
+# TRANS) wave instructions, SQ_INSTS_VALU_*_F32;
+# profiles/k_step2_swe_humps_l7-9_valu_mix.md).
+VALU_INSTS_PER_CELL = 14.43
+FLOP_PER_CELL = 950.4
+# MI355X FP32 vector peak without packed math: 256 CUs x 4 SIMDs x 16 lanes
+# x 2 (FMA) x 2.4 GHz.  The step kernels are built without packed FP32.
+FP32_VALU_PEAK = 78.6e12
 
 
 _STREAM = []
+_T0 = time.perf_counter()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def make_sim(precision, args, leaves=None, q=None):
@@ -156,6 +171,17 @@ def _ref_worker(args):
     return cells, time.perf_counter() - t0
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(sim64, budget_s):
     """CPU rate of the same patch step on the GPU box's host cores, on a sample
     of level-7 patches of the workload.  With oracle/_ref built: the reference
@@ -196,13 +222,18 @@ def cpu_baseline(sim64, budget_s):
     t0 = time.perf_counter()
     with mp.get_context("spawn").Pool(len(args)) as pool:
         res = pool.map(_ref_worker, args)
+        # the same step on one core, over the whole sample
+        one = pool.apply(_ref_worker, ((REF_LIB, sample, float(dx[0]), float(dy[0]), dt, budget_s / 4),))
     wall = time.perf_counter() - t0
     rcells = sum(r[0] for r in res)
     busy = max(r[1] for r in res)
     ref = dict(value=rcells / busy, unit="cell-updates/s", cores=len(args), kind="reference",
                sample=f"{len(idx)} level-7 patches (32x32, meqn 3) split over {len(args)} processes, "
                       f"the reference Fortran CLAWPACK46_STEP2_WRAP (oracle/_ref, amdflang -O2) "
-                      f"repeated for {busy:.1f} s per process ({wall:.1f} s wall with start-up)")
+                      f"repeated for {busy:.1f} s per process ({wall:.1f} s wall with start-up)",
+               one_core={"value": one[0] / one[1], "unit": "cell-updates/s", "cores": 1,
+                         "seconds": one[1]},
+               cpu_model=_cpu_model(), host_threads=os.cpu_count())
     return ref, port
 
 
@@ -447,6 +478,7 @@ def main():
     # The mesh is built once, by the FP64 run (tagging sees FP64 data, as in
     # the reference); the FP32 run starts from the same leaves and the same
     # initial data rounded to FP32.
+    log("building the mesh (FP64 initial refinement)")
     sim64 = make_sim(H.FP64, args)
     info = sim64.patch_info()
     q0 = sim64.get_q()
@@ -461,16 +493,19 @@ def main():
 
     # launch shape of the patch-step kernel picked by the autotuner on the
     # level with the most patches (timed into a scratch buffer)
+    log(f"mesh {levels}; autotuning")
     tuned = {}
     for name, sim in (("fp32", sim32), ("fp64", sim64)):
         best, sweep = sim.autotune(-1, 3)
         tuned[name] = {"cols_per_lane": best[0], "waves_per_group": best[1],
                        "sweep_ms": {f"cpl{c}_wpg{w}": round(v, 4) for (c, w), v in sweep.items()}}
 
+    log("timing FP32 batched")
     r32 = timed(sim32, args, dt, not args.no_graph)
     r64 = r64p = None
     err = None
     if not args.no_fp64:
+        log("timing FP64 batched")
         r64 = timed(sim64, args, dt, not args.no_graph)
         a = sim32.get_q()[:, :, 2:-2, 2:-2]
         b = sim64.get_q()[:, :, 2:-2, 2:-2]
@@ -484,10 +519,12 @@ def main():
         best, sweep = sim64.autotune(-1, 3)
         tuned["fp64_plain"] = {"cols_per_lane": best[0], "waves_per_group": best[1],
                                "sweep_ms": {f"cpl{c}_wpg{w}": round(v, 4) for (c, w), v in sweep.items()}}
+        log("timing FP64 plain")
         r64p = timed(sim64, args, dt, False, plain=True)
 
     cpu = port = None
     if not args.no_cpu:
+        log("CPU baseline")
         cpu, port = cpu_baseline(sim64, args.cpu_seconds)
     hp = host_path(sim64, args.host_path, dt) if args.host_path > 0 else None
 
@@ -495,7 +532,10 @@ def main():
     T, N, meqn = 36 * 36, 32 * 32, 3
     b_upd = w * meqn * (T + N) / N  # algorithmic bytes per cell update (read tile, write interior)
     k_time = r32["kernel_ms"] / 1e3
-    achieved = b_upd * r32["kcells"] / k_time if k_time > 0 else 0.0
+    k_rate = r32["kcells"] / k_time if k_time > 0 else 0.0  # cell updates per second of step kernel
+    hbm_achieved = b_upd * k_rate
+    measured = levels == MEASURED_MESH
+    flops = FLOP_PER_CELL * k_rate if (FLOP_PER_CELL is not None and measured) else None
     value = r32["cells"] / r32["seconds"]
     res = {
         "metric": METRIC,
@@ -516,15 +556,23 @@ def main():
                    "cell_updates_per_step": r32["cells"] // args.steps,
                    "hipgraph": {"captures": r32["graph_captures"], "replays": r32["graph_replays"]},
                    "launch": tuned},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK,
-                     "traffic": TRAFFIC_PER_LAUNCH if levels == MEASURED_MESH else None,
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+        # The step kernel is bound by VALU issue (SQ counters), not HBM: the
+        # roofline is the FP32 vector peak, with the VALU busy fraction and
+        # the HBM fraction beside it.
+        "roofline": {"bound": "valu",
+                     "achieved": flops / 1e12 if flops is not None else None,
+                     "peak": FP32_VALU_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / FP32_VALU_PEAK if flops is not None else None,
+                     "flop_per_cell_update": FLOP_PER_CELL,
+                     "valu_busy": VALU_ACTIVE if measured else None,
+                     "valu_insts_per_cell_update": VALU_INSTS_PER_CELL if measured else None,
+                     "traffic": TRAFFIC_PER_LAUNCH if measured else None,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
                      "algorithmic_per_launch": b_upd * r32["kcells"] / max(r32["launches"], 1),
-                     "valu_active": VALU_ACTIVE if levels == MEASURED_MESH else None,
+                     "hbm": {"achieved": hbm_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                             "frac": hbm_achieved / HBM_PEAK, "bytes_per_cell_update": b_upd},
                      "kernel": "k_step2<float,Swe>",
-                     "bytes_per_cell_update": b_upd,
-                     "kernel_only_rate": r32["kcells"] / k_time if k_time > 0 else None,
+                     "kernel_only_rate": k_rate,
                      "kernel_launches": r32["launches"]},
         "cpu_baseline": cpu,
     }
@@ -538,8 +586,6 @@ def main():
         res["cnn_c4"] = cnn_c4(args.cnn)
     if args.no_push or args.no_merge:
         res["config"]["features"] = {"ghost_push": not args.no_push, "merge_levels": not args.no_merge}
-    res["roofline"]["achieved"] = achieved / 1e9
-    res["roofline"]["peak"] = HBM_PEAK / 1e9
     if r64 is not None:
         def rate(r, what):
             k = r["kernel_ms"] / 1e3
