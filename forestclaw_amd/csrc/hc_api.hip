@@ -3,11 +3,47 @@
 // velocities, physical boundary conditions and the level CFL reduction.
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "hc_internal.hpp"
 
 namespace hc {
+
+// Per-tile Courant numbers of hipclaw_step2_batch_noalloc: a file-scope
+// device array (each GPU holds its own copy until the process ends).  Host
+// code never passes the array's name to a kernel: its device address on the
+// current GPU comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) double scratch[HIPCLAW_NOALLOC_MAX_TILES];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (looked up on the first
+// call per device).
+static int scratch_ptr(double** out)
+{
+    int dev = -1;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return set_error(HIPCLAW_ERR_UNSUPPORTED, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        if (!p) return set_error(HIPCLAW_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HIPCLAW_OK;
+}
+
+__global__ void k_fill_zero(double* __restrict__ x, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
 
 static thread_local std::string g_last_error;
 
@@ -235,6 +271,35 @@ int hipclaw_step2_batch(const hipclaw_physics* phys, int npatch, const void* q_i
 {
     return step2_batch_impl(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt,
                             cfl_patch, nullptr, launch, stream);
+}
+
+int hipclaw_step2_batch_noalloc(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
+                                const void* aux, const double* dxdy, const int* patch_ids, int ntiles,
+                                double dt, double* maxcfl, const hipclaw_launch* launch, void* stream)
+{
+    int rc = validate_physics(phys);
+    if (rc) return rc;
+    if (!maxcfl) return set_error(HIPCLAW_ERR_ARG, "maxcfl is NULL");
+    if (npatch < 0 || ntiles < 0) return set_error(HIPCLAW_ERR_ARG, "npatch, ntiles must be >= 0");
+    if (!patch_ids && ntiles < npatch) return set_error(HIPCLAW_ERR_ARG, "ntiles < npatch without a patch list");
+    if (ntiles > HIPCLAW_NOALLOC_MAX_TILES || npatch > HIPCLAW_NOALLOC_MAX_TILES)
+        return set_error(HIPCLAW_ERR_ARG, "step2_batch_noalloc: at most " +
+                                              std::to_string(HIPCLAW_NOALLOC_MAX_TILES) + " tiles per call");
+    double* cfl = nullptr;
+    rc = scratch_ptr(&cfl);
+    if (rc) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // the array keeps the previous call's values: clear the tiles this call
+    // reduces over (a fill kernel, so the call can be captured in a graph)
+    const int nclear = ntiles > 0 ? ntiles : 1;
+    hipLaunchKernelGGL(k_fill_zero, dim3((nclear + 255) / 256), dim3(256), 0, s, cfl, nclear);
+    rc = check_hip(hipGetLastError(), "k_fill_zero launch");
+    if (rc) return rc;
+    rc = step2_batch_impl(phys, npatch, q_in, q_out, aux, dxdy, patch_ids, ntiles, dt, cfl, nullptr,
+                          launch, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(1024), 0, s, cfl, ntiles, maxcfl);
+    return check_hip(hipGetLastError(), "k_max_reduce launch");
 }
 
 int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void* q_in,

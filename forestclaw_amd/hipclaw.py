@@ -17,6 +17,7 @@ SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
 FP64, FP32 = 0, 1
 FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
 FEATURE_FUSE_AUX = 5
+NOALLOC_MAX_TILES = 262144  # HIPCLAW_NOALLOC_MAX_TILES
 TAG_VALUE, TAG_DIFFERENCE, TAG_MINMAX, TAG_GRADIENT = 0, 1, 2, 3
 SRC2_NONE, SRC2_AXISYMMETRIC = 0, 1
 CLAW5 = 0x100   # solver flag: the Clawpack 5 solver and host layout q(meqn, i, j)
@@ -75,6 +76,9 @@ _SIGS = {
                                   _VP, ctypes.c_int, ctypes.c_double, _VP, _VP,
                                   ctypes.POINTER(Launch), _VP], ctypes.c_int),
     "hipclaw_max_reduce": ([_VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
+    "hipclaw_step2_batch_noalloc": ([ctypes.POINTER(Physics), ctypes.c_int, _VP, _VP, _VP, _VP, _VP,
+                                     ctypes.c_int, ctypes.c_double, _VP, ctypes.POINTER(Launch), _VP],
+                                    ctypes.c_int),
     "hipclaw_launch_shape": ([ctypes.POINTER(Physics), ctypes.c_int, ctypes.c_void_p,
                               ctypes.POINTER(Launch)], ctypes.c_int),
     "hipclaw_cnn_create": ([_VP, ctypes.POINTER(_VP)], ctypes.c_int),
@@ -241,6 +245,19 @@ def step2_batch(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, c
         ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy),
         _ptr(patch_ids), int(ntiles), float(dt), _ptr(cfl), ctypes.byref(ln) if ln is not None else None,
         _stream(stream)))
+
+
+def step2_batch_noalloc(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, maxcfl,
+                        patch_ids=None, launch=None, stream=None, ntiles: int | None = None) -> None:
+    """step2_batch that allocates nothing and writes the batch's maximum
+    Courant number to maxcfl (one device double)."""
+    if ntiles is None:
+        ntiles = npatch
+    ln = Launch(*launch) if launch is not None else None
+    _check(lib().hipclaw_step2_batch_noalloc(
+        ctypes.byref(phys), npatch, _ptr(q_in), _ptr(q_out), _ptr(aux), _ptr(dxdy),
+        _ptr(patch_ids), int(ntiles), float(dt), _ptr(maxcfl),
+        ctypes.byref(ln) if ln is not None else None, _stream(stream)))
 
 
 def step2_batch_sync(phys: Physics, npatch: int, q_in, q_out, aux, dxdy, dt: float, cfl,

@@ -148,6 +148,24 @@ int hipclaw_step2_batch_sync(const hipclaw_physics* phys, int npatch, const void
                              const int* patch_ids, int ntiles, double dt, double* cfl_patch,
                              double* registers, const hipclaw_launch* launch, void* stream);
 
+/* hipclaw_step2_batch that allocates nothing and returns the batch's maximum
+ * Courant number, the value fclaw2d_update_single_step takes over a level,
+ * in *maxcfl (one device double), instead of per-tile values.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The per-tile Courant numbers, which
+ * hipclaw_step2_batch writes to the caller's cfl_patch workspace, live in a
+ * static device array of HIPCLAW_NOALLOC_MAX_TILES doubles (2 MiB per GPU,
+ * held until the process ends): ntiles and npatch are limited to that many
+ * (HIPCLAW_ERR_ARG beyond it).  Every call on one GPU shares the array, so
+ * calls that overlap on one GPU (other streams, other threads) race; calls
+ * ordered on one stream do not.  Other arguments as hipclaw_step2_batch. */
+#define HIPCLAW_NOALLOC_MAX_TILES 262144
+int hipclaw_step2_batch_noalloc(const hipclaw_physics* phys, int npatch, const void* q_in,
+                                void* q_out, const void* aux, const double* dxdy,
+                                const int* patch_ids, int ntiles, double dt, double* maxcfl,
+                                const hipclaw_launch* launch, void* stream);
+
 /* ------------------------------------------------------------------------
  * Host-resident patches: deferred single-step update.
  *
