@@ -10,7 +10,8 @@ and the FP32-vs-FP64 error report of the precision mode.
       subcycled, time sync (applications/clawpack/shallow/2d/radialdam).
   C3  Euler shock-bubble, gamma 1.4, [0, 2] x [0, 0.5] as a 4x1 brick, AMR
       levels 3-7, 16x16 patches, Mach 1.22 shock at x = 0.2 meeting a bubble
-      of density 0.138, radius 0.2 at (0.5, 0.25) (applications/clawpack/euler/2d/shockbubble).
+      of density 0.138, radius 0.2 at (0.5, 0.25) (applications/clawpack/euler/2d/shockbubble);
+      Roe (C3) or HLLE (C3-HLLE) normal solver.
   C5  shallow water on the unit square, level 7 uniform refined to level 9
       around 8 seeded Gaussian humps (the throughput workload of bench.py).
 
@@ -65,9 +66,9 @@ def c3_params(gamma=1.4):
     return (gamma, 0.2, rinf, vinf, pinf, 0.5, 0.25, 0.2, 0.138)
 
 
-def c3_options(precision=H.FP64):
+def c3_options(precision=H.FP64, solver=H.SOLVER_EULER5_ROE):
     return F.Options(minlevel=3, maxlevel=7, mx=16, my=16, mi=4, mj=1, ax=0.0, bx=2.0, ay=0.0,
-                     by=0.5, solver=H.SOLVER_EULER5_ROE, precision=precision, gamma=1.4,
+                     by=0.5, solver=solver, precision=precision, gamma=1.4,
                      refine_threshold=0.1, coarsen_threshold=0.025, subcycle=True)
 
 
@@ -112,8 +113,9 @@ def build(name, precision=H.FP64, stream=None):
         sim = F.Hipclaw46(c2_options(precision), stream=stream)
         sim.initialize("radial_dam", c2_case(), nsub=4)
         return sim, c2_dt(int(sim.patch_info()["level"].min()))
-    if name == "C3":
-        sim = F.Hipclaw46(c3_options(precision), stream=stream)
+    if name in ("C3", "C3-HLLE"):
+        solver = H.SOLVER_EULER_HLLE if name == "C3-HLLE" else H.SOLVER_EULER5_ROE
+        sim = F.Hipclaw46(c3_options(precision, solver), stream=stream)
         sim.initialize("shock_bubble", c3_params(), nsub=4)
         return sim, c3_dt(int(sim.patch_info()["level"].min()))
     if name == "C5":

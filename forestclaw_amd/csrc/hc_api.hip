@@ -86,6 +86,12 @@ int validate_physics(const hipclaw_physics* p)
             return set_error(HIPCLAW_ERR_ARG, "euler5 needs meqn=mwaves=5, maux=0|1");
         if (!(p->gamma > 1)) return set_error(HIPCLAW_ERR_ARG, "gamma must be > 1");
         break;
+    case HIPCLAW_SOLVER_EULER_HLLE:
+        if (p->meqn != 5 || p->mwaves != 2 || p->maux < 0 || p->maux > 1)
+            return set_error(HIPCLAW_ERR_ARG, "euler HLLE needs meqn=5, mwaves=2, maux=0|1");
+        if (!(p->gamma > 1)) return set_error(HIPCLAW_ERR_ARG, "gamma must be > 1");
+        if (is_claw5(p->solver)) return set_error(HIPCLAW_ERR_UNSUPPORTED, "HLLE has no Clawpack 5 variant");
+        break;
     default:
         return set_error(HIPCLAW_ERR_ARG, "unknown solver");
     }
@@ -254,6 +260,7 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
     case HIPCLAW_SOLVER_ADV: out->meqn = out->mwaves = 1; out->maux = 2; break;
     case HIPCLAW_SOLVER_SWE_ROE: out->meqn = out->mwaves = 3; break;
     case HIPCLAW_SOLVER_EULER5_ROE: out->meqn = out->mwaves = 5; break;
+    case HIPCLAW_SOLVER_EULER_HLLE: out->meqn = 5; out->mwaves = 2; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown solver");
     }
     for (int w = 0; w < 5; w++) out->mthlim[w] = w < out->mwaves ? 4 : 0;

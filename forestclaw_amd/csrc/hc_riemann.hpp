@@ -28,7 +28,7 @@
 
 namespace hc {
 
-enum { SOLVER_ADV = 0, SOLVER_SWE = 1, SOLVER_EULER5 = 2 };
+enum { SOLVER_ADV = 0, SOLVER_SWE = 1, SOLVER_EULER5 = 2, SOLVER_EULER_HLLE = 3 };
 
 template <class R, int ME_, int MW_> struct FaceT {
     static constexpr int ME = ME_, MW = MW_;
@@ -416,6 +416,130 @@ template <class R, bool C5> struct Euler5T {
                     b1 = b1 + sb[w] * wb[w][m];
             }
             bm[m] = b0; bp[m] = b1;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Euler equations + conserved tracer, HLLE: two waves with Einfeldt's speed
+// bounds (Harten, Lax and van Leer 1983; Einfeldt 1988; LeVeque, Finite
+// Volume Methods for Hyperbolic Problems, section 15.3.7).  The reference has
+// no HLLE (applications/clawpack/euler/2d/rp/README); this solver takes the
+// interface of clawpack46_rpn2_euler5.f / clawpack46_rpt2_euler5.f; its
+// transverse split is the HLL form of rpt2_euler5's Roe split (rpt2 below).
+//   s1 = min(u_a - c_a, u_roe - a_roe),  s2 = max(u_b + c_b, u_roe + a_roe)
+//   W1 = (s2 (q_b - q_a) - (f(q_b) - f(q_a))) / (s2 - s1),  W2 = (q_b - q_a) - W1
+// Equal states give exactly zero waves; s1 W1 + s2 W2 = f(q_b) - f(q_a).
+// Field 5 is a conserved tracer density with flux u q_5.
+// ---------------------------------------------------------------------------
+template <class R> struct EulerHlle {
+    static constexpr int ME = 5, MW = 2, MAUX = 0;
+    static constexpr bool HAS_CONS = false;
+    using Face = FaceT<R, 5, 2>;
+    using Trans = typename Euler5T<R, false>::Trans;
+
+    static constexpr int NC = 0;  // no per-cell data
+    __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+
+    // normal flux, normal velocity and sound speed of one state
+    template <int DIR>
+    __device__ static inline void flux(const R* q, R g1, R gam, R* f, R& un, R& c)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R u = MM::divp(q[mu], q[0]);
+        const R p = g1 * (q[3] - MM::divp(R(0.5) * (q[1] * q[1] + q[2] * q[2]), q[0]));
+        f[0] = q[mu];
+        f[mu] = q[mu] * u + p;
+        f[mv] = q[mv] * u;
+        f[3] = u * (q[3] + p);
+        f[4] = q[4] * u;
+        un = u;
+        c = MM::sqrt_(MM::divp(gam * p, q[0]));
+    }
+
+    template <int DIR>
+    __device__ static inline void rpn2(const R* qa, const R* qb, const R*, const R*, R,
+                                       const PhysParams& pp, Face& f, Trans& t)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R gam = R(pp.gamma);
+        const R g1 = R(pp.gamma - 1.0);
+        // Roe averages (the operations of Euler5T::rpn2)
+        R rl = MM::sqrt_(qa[0]);
+        R rr = MM::sqrt_(qb[0]);
+        R pl = g1 * (qa[3] - MM::divp(R(0.5) * (qa[1] * qa[1] + qa[2] * qa[2]), qa[0]));
+        R pr = g1 * (qb[3] - MM::divp(R(0.5) * (qb[1] * qb[1] + qb[2] * qb[2]), qb[0]));
+        R rsq2 = rl + rr;
+        R u = MM::divp(MM::divp(qa[mu], rl) + MM::divp(qb[mu], rr), rsq2);
+        R v = MM::divp(MM::divp(qa[mv], rl) + MM::divp(qb[mv], rr), rsq2);
+        R enth = MM::divp(MM::divp(qa[3] + pl, rl) + MM::divp(qb[3] + pr, rr), rsq2);
+        R u2v2 = u * u + v * v;
+        R a2c = g1 * (enth - R(.5) * u2v2);
+        R a = MM::sqrt_(a2c);
+        t.u = u; t.v = v; t.enth = enth; t.a = a; t.euv = enth - u2v2; t.u2v2 = u2v2;
+        t.g1a2 = MM::divp(g1, a2c);
+
+        R fa[5], fb[5], ua, ca, ub, cb;
+        flux<DIR>(qa, g1, gam, fa, ua, ca);
+        flux<DIR>(qb, g1, gam, fb, ub, cb);
+        const R s1 = dmin(ua - ca, u - a);
+        const R s2 = dmax(ub + cb, u + a);
+        f.s[0] = s1;
+        f.s[1] = s2;
+        for (int m = 0; m < 5; m++) {
+            const R dq = qb[m] - qa[m];
+            const R w1 = MM::divp(s2 * dq - (fb[m] - fa[m]), s2 - s1);
+            const R w2 = dq - w1;
+            f.w[0][m] = w1;
+            f.w[1][m] = w2;
+            f.am[m] = dmin(s1, R(0)) * w1 + dmin(s2, R(0)) * w2;
+            f.ap[m] = dmax(s1, R(0)) * w1 + dmax(s2, R(0)) * w2;
+        }
+    }
+
+    // Transverse split: the HLL form of the Roe transverse matrix B at the
+    // Roe average.  B asdq is the Roe split summed over its waves; with the
+    // bounds t1 = v - a, t2 = v + a,
+    //   V1 = (t2 asdq - B asdq) / (t2 - t1),  V2 = asdq - V1,
+    //   bm = min(t1, 0) V1 + min(t2, 0) V2,  bp = max(t1, 0) V1 + max(t2, 0) V2,
+    // so bm + bp = B asdq, and a fluctuation the Roe split gives no transverse
+    // speed (a contact's) still spreads with the acoustic speeds, as the
+    // normal solver spreads it (with the Roe split alone the scheme is
+    // unstable at Courant numbers above about 0.5).
+    template <int DIR>
+    __device__ static inline void rpt2(int, const R* asdq, const Trans& t, const PhysParams&, R* bm,
+                                       R* bp)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        using MM = M<R>;
+        const R u = t.u, v = t.v, enth = t.enth, a = t.a, g1a2 = t.g1a2, euv = t.euv, u2v2 = t.u2v2;
+        R a3 = g1a2 * (euv * asdq[0] + u * asdq[mu] + v * asdq[mv] - asdq[3]);
+        R a2 = asdq[mu] - u * asdq[0];
+        R a4 = MM::divp(asdq[mv] + (a - v) * asdq[0] - a * a3, R(2) * a);
+        R a1 = asdq[0] - a3 - a4;
+        R wb[4][5], sb[4];
+        wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
+        wb[0][3] = a1 * (enth - v * a); wb[0][4] = R(0); sb[0] = v - a;
+        wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v;
+        wb[1][3] = a3 * R(0.5) * u2v2 + a2 * u; wb[1][4] = R(0); sb[1] = v;
+        wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a);
+        wb[2][3] = a4 * (enth + v * a); wb[2][4] = R(0); sb[2] = v + a;
+        wb[3][0] = R(0); wb[3][mu] = R(0); wb[3][mv] = R(0); wb[3][3] = R(0);
+        wb[3][4] = asdq[4]; sb[3] = v;
+        const R t1 = v - a, t2 = v + a;
+        for (int m = 0; m < 5; m++) {
+            R bq = R(0);
+            for (int w = 0; w < 4; w++) bq = bq + sb[w] * wb[w][m];
+            const R d = asdq[m];
+            const R v1 = MM::divp(t2 * d - bq, t2 - t1);
+            const R v2 = d - v1;
+            bm[m] = dmin(t1, R(0)) * v1 + dmin(t2, R(0)) * v2;
+            bp[m] = dmax(t1, R(0)) * v1 + dmax(t2, R(0)) * v2;
         }
     }
 };

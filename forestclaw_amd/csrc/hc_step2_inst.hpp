@@ -123,12 +123,14 @@ static int launch_step2_t(const StepArgs& a, int solver_id, int cpl, int wpg, co
         case SOLVER_ADV: return launch_one<R, Adv, 1>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_SWE: return launch_one<R, Swe, 1>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER5: return launch_one<R, Euler5, 1>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_EULER_HLLE: return launch_one<R, EulerHlle, 1>(a, wpg, qin, qout, aux, cfl, s);
         }
     } else if (cpl == 2) {
         switch (solver) {
         case SOLVER_ADV: return launch_one<R, Adv, 2>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_SWE: return launch_one<R, Swe, 2>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER5: return launch_one<R, Euler5, 2>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_EULER_HLLE: return launch_one<R, EulerHlle, 2>(a, wpg, qin, qout, aux, cfl, s);
         }
     }
     return set_error(HIPCLAW_ERR_ARG, "unsupported solver / cols_per_lane");

@@ -38,6 +38,13 @@ extern "C" {
 #define HIPCLAW_SOLVER_EULER5_ROE 2 /* Euler + tracer Roe + entropy fix (meqn 5; maux
                                        0, or 1 = the radius for the axisymmetric
                                        source, not read by the step)             */
+#define HIPCLAW_SOLVER_EULER_HLLE 3 /* Euler + conserved tracer, HLLE normal solver with
+                                       Einfeldt's speeds (2 waves; meqn 5, mwaves 2)
+                                       and the HLL form of rpt2_euler5's Roe split.
+                                       No counterpart in the reference (its Euler
+                                       solver is Roe only); same interface as
+                                       clawpack46_rpn2_euler5.f:1-280 /
+                                       clawpack46_rpt2_euler5.f:1-122             */
 
 /* Flag OR-ed into the solver id: the Clawpack 5 variant of the solver family
  * (the fc2d_clawpack5 solver, src/solvers/fc2d_clawpack5/, with the

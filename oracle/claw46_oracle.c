@@ -335,6 +335,76 @@ static void rpn2_euler5(const orc_cfg* c, int ixy, slice_t* sl)
         }
 }
 
+/* HLLE (Harten, Lax and van Leer 1983, with the wave-speed bounds of
+ * Einfeldt 1988; LeVeque, Finite Volume Methods for Hyperbolic Problems,
+ * section 15.3.7).  No counterpart in the reference (its Euler solver is
+ * Roe only, applications/clawpack/euler/2d/rp/README); the interface is that
+ * of clawpack46_rpn2_euler5.f.  Two waves W1 = q_m - q_a, W2 = q_b - q_m
+ * with the HLL middle state q_m; W1 is formed from the jumps, so equal
+ * states give exactly zero waves:
+ *   s1 = min(u_a - c_a, u_roe - a_roe),  s2 = max(u_b + c_b, u_roe + a_roe)
+ *   W1 = (s2 (q_b - q_a) - (f(q_b) - f(q_a))) / (s2 - s1),  W2 = (q_b - q_a) - W1
+ * The tracer (field 5) is a conserved density with flux u q_5. */
+static void euler_flux(double g1, int mu, int mv, const double* q, double* f, double* un, double* c,
+                       double gam)
+{
+    const double u = q[mu] / q[0];
+    const double p = g1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    f[0] = q[mu];
+    f[mu] = q[mu] * u + p;
+    f[mv] = q[mv] * u;
+    f[3] = u * (q[3] + p);
+    f[4] = q[4] * u;
+    *un = u;
+    *c = sqrt(gam * p / q[0]);
+}
+
+void orc_rpn2_hlle_face(double gamma, int ixy, const double* qa, const double* qb, double* w,
+                        double* s, double* amdq, double* apdq, double* r)
+{
+    orc_cfg c;
+    memset(&c, 0, sizeof(c));
+    c.gamma = gamma;
+    const int mu = (ixy == 1) ? 1 : 2;
+    const int mv = (ixy == 1) ? 2 : 1;
+    const double g1 = gamma - 1.0;
+    euler_roe(&c, mu, mv, qb, qa, r);
+    double fa[5], fb[5], ua, ca, ub, cb;
+    euler_flux(g1, mu, mv, qa, fa, &ua, &ca, gamma);
+    euler_flux(g1, mu, mv, qb, fb, &ub, &cb, gamma);
+    const double s1 = dmin(ua - ca, r[0] - r[3]);
+    const double s2 = dmax(ub + cb, r[0] + r[3]);
+    for (int m = 0; m < 5; m++) {
+        const double dq = qb[m] - qa[m];
+        const double w1 = (s2 * dq - (fb[m] - fa[m])) / (s2 - s1);
+        const double w2 = dq - w1;
+        w[m] = w1;
+        w[5 + m] = w2;
+        amdq[m] = dmin(s1, 0.0) * w1 + dmin(s2, 0.0) * w2;
+        apdq[m] = dmax(s1, 0.0) * w1 + dmax(s2, 0.0) * w2;
+    }
+    s[0] = s1;
+    s[1] = s2;
+}
+
+static void rpn2_hlle(const orc_cfg* c, int ixy, slice_t* sl)
+{
+    for (int k = 1; k < sl->N; k++) {
+        double qa[5], qb[5], w[10], sp[2], am[5], ap[5], r[ROE_N];
+        for (int m = 0; m < 5; m++) { qa[m] = Q1(m, k - 1); qb[m] = Q1(m, k); }
+        orc_rpn2_hlle_face(c->gamma, ixy, qa, qb, w, sp, am, ap, r);
+        for (int t = 0; t < ROE_N; t++) ROE(t, k) = r[t];
+        for (int m = 0; m < 5; m++) {
+            WV(0, m, k) = w[m];
+            WV(1, m, k) = w[5 + m];
+            AM(m, k) = am[m];
+            AP(m, k) = ap[m];
+        }
+        SP(0, k) = sp[0];
+        SP(1, k) = sp[1];
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Transverse solvers: split asdq(face k) into down/up-going parts.          */
 /* ------------------------------------------------------------------------ */
@@ -431,11 +501,67 @@ static void rpt2_euler5(const orc_cfg* c, int ixy, int imp, const double* asdq,
     }
 }
 
+/* Transverse split of the HLLE pair: the HLL form of the Roe transverse
+ * matrix B at the Roe average of the face.  B asdq is the Roe split summed
+ * over its waves (sb_w wb_w, as rpt2_euler5 forms them); with the bounds
+ * t1 = v - a, t2 = v + a (v the transverse Roe velocity):
+ *   V1 = (t2 asdq - B asdq) / (t2 - t1),  V2 = asdq - V1
+ *   bmasdq = min(t1, 0) V1 + min(t2, 0) V2,  bpasdq = max(t1, 0) V1 + max(t2, 0) V2
+ * so bmasdq + bpasdq = B asdq.  A fluctuation that the Roe split gives no
+ * transverse speed (a contact's) still spreads with the acoustic speeds, as
+ * the normal HLLE solver spreads it. */
+void orc_rpt2_hlle_face(int ixy, const double* r, const double* d, double* bm, double* bp)
+{
+    const int mu = (ixy == 1) ? 1 : 2;
+    const int mv = (ixy == 1) ? 2 : 1;
+    double u = r[0], v = r[1], enth = r[2], a = r[3], g1a2 = r[4], euv = r[5], u2v2 = r[6];
+    double a3 = g1a2 * (euv * d[0] + u * d[mu] + v * d[mv] - d[3]);
+    double a2 = d[mu] - u * d[0];
+    double a4 = (d[mv] + (a - v) * d[0] - a * a3) / (2.0 * a);
+    double a1 = d[0] - a3 - a4;
+    double wb[4][5], sb[4];
+    wb[0][0] = a1; wb[0][mu] = a1 * u; wb[0][mv] = a1 * (v - a);
+    wb[0][3] = a1 * (enth - v * a); wb[0][4] = 0.0; sb[0] = v - a;
+    wb[1][0] = a3; wb[1][mu] = a3 * u + a2; wb[1][mv] = a3 * v;
+    wb[1][3] = a3 * 0.5 * u2v2 + a2 * u; wb[1][4] = 0.0; sb[1] = v;
+    wb[2][0] = a4; wb[2][mu] = a4 * u; wb[2][mv] = a4 * (v + a);
+    wb[2][3] = a4 * (enth + v * a); wb[2][4] = 0.0; sb[2] = v + a;
+    wb[3][0] = 0.0; wb[3][mu] = 0.0; wb[3][mv] = 0.0; wb[3][3] = 0.0;
+    wb[3][4] = d[4]; sb[3] = v;
+    const double t1 = v - a, t2 = v + a;
+    for (int m = 0; m < 5; m++) {
+        double bq = 0.0;
+        for (int w = 0; w < 4; w++) bq = bq + sb[w] * wb[w][m];
+        const double v1 = (t2 * d[m] - bq) / (t2 - t1);
+        const double v2 = d[m] - v1;
+        bm[m] = dmin(t1, 0.0) * v1 + dmin(t2, 0.0) * v2;
+        bp[m] = dmax(t1, 0.0) * v1 + dmax(t2, 0.0) * v2;
+    }
+}
+
+static void rpt2_hlle(const orc_cfg* c, int ixy, int imp, const double* asdq, slice_t* sl)
+{
+    (void)c;
+    (void)imp;
+    const int N = sl->N;
+    for (int k = 1; k < N; k++) {
+        double r[ROE_N], d[5], bm[5], bp[5];
+        for (int t = 0; t < ROE_N; t++) r[t] = ROE(t, k);
+        for (int m = 0; m < 5; m++) d[m] = asdq[m * N + k];
+        orc_rpt2_hlle_face(ixy, r, d, bm, bp);
+        for (int m = 0; m < 5; m++) {
+            BM(m, k) = bm[m];
+            BP(m, k) = bp[m];
+        }
+    }
+}
+
 static void rpn2(const orc_cfg* c, int ixy, slice_t* sl)
 {
     switch (c->solver) {
     case ORC_SOLVER_ADV: rpn2_adv(c, ixy, sl); break;
     case ORC_SOLVER_SWE_ROE: rpn2_swe(c, ixy, sl); break;
+    case ORC_SOLVER_EULER_HLLE: rpn2_hlle(c, ixy, sl); break;
     default: rpn2_euler5(c, ixy, sl); break;
     }
 }
@@ -445,6 +571,7 @@ static void rpt2(const orc_cfg* c, int ixy, int imp, const double* asdq, slice_t
     switch (c->solver) {
     case ORC_SOLVER_ADV: rpt2_adv(c, ixy, imp, asdq, sl); break;
     case ORC_SOLVER_SWE_ROE: rpt2_swe(c, ixy, imp, asdq, sl); break;
+    case ORC_SOLVER_EULER_HLLE: rpt2_hlle(c, ixy, imp, asdq, sl); break;
     default: rpt2_euler5(c, ixy, imp, asdq, sl); break;
     }
 }
@@ -594,6 +721,7 @@ static int check_cfg(const orc_cfg* c)
     case ORC_SOLVER_ADV: if (c->maux < 2) return 1; break;
     case ORC_SOLVER_SWE_ROE: if (c->meqn != 3 || c->mwaves != 3) return 1; break;
     case ORC_SOLVER_EULER5_ROE: if (c->meqn != 5 || c->mwaves != 5) return 1; break;
+    case ORC_SOLVER_EULER_HLLE: if (c->meqn != 5 || c->mwaves != 2 || c->claw5) return 1; break;
     default: return 1;
     }
     return 0;

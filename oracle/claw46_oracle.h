@@ -20,7 +20,9 @@ extern "C" {
 enum {
     ORC_SOLVER_ADV = 0,        /* clawpack46_rpn2adv / rpt2adv            */
     ORC_SOLVER_SWE_ROE = 1,    /* shallow/2d/rp clawpack46_rpn2 / rpt2    */
-    ORC_SOLVER_EULER5_ROE = 2  /* euler/2d/rp clawpack46_rpn2/rpt2_euler5 */
+    ORC_SOLVER_EULER5_ROE = 2, /* euler/2d/rp clawpack46_rpn2/rpt2_euler5 */
+    ORC_SOLVER_EULER_HLLE = 3  /* Euler + tracer, HLLE normal solver (2 waves) and
+                                  the HLL form of the Roe transverse split        */
 };
 
 typedef struct {
@@ -38,6 +40,21 @@ typedef struct {
                           scale by 0.5/a; Euler5: clawpack5_rpt2_euler5.f90:67
                           takes gamma1/c**2 for the transverse split)           */
 } orc_cfg;
+
+/* HLLE normal solver at one face for the Euler equations with a conserved
+ * tracer (meqn 5: rho, rho u, rho v, E, rho phi), gamma-law gas.  qa is the
+ * state on the low side, qb on the high side, ixy 1 = x face, 2 = y face.
+ * Two waves w[2][5] with speeds s[2] (Einfeldt's bounds), fluctuations
+ * amdq / apdq.  Returns the Roe averages r[7] (u, v, enth, a, g1/a^2,
+ * enth - u^2 - v^2, u^2 + v^2) that the transverse split uses. */
+void orc_rpn2_hlle_face(double gamma, int ixy, const double* qa, const double* qb, double* w,
+                        double* s, double* amdq, double* apdq, double* r);
+
+/* Transverse split of the HLLE pair at one face: d (5) the fluctuation, r
+ * the Roe averages orc_rpn2_hlle_face returned, ixy the normal direction of
+ * the face (the split is along the other one).  bm + bp = B d with B the Roe
+ * transverse matrix; the split is its HLL form with the bounds v -+ a. */
+void orc_rpt2_hlle_face(int ixy, const double* r, const double* d, double* bm, double* bp);
 
 /* One patch step in place (clawpack46_step2_wrap semantics, mcapa = 0).
  * Returns 0 on success, nonzero on bad configuration.  *cfl receives the

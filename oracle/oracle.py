@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE = 0, 1, 2
+SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE, SOLVER_EULER_HLLE = 0, 1, 2, 3
 
 
 class OrcCfg(ctypes.Structure):
@@ -72,8 +72,8 @@ def _dp(a: np.ndarray):
 def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = None,
              order=(2, 2), mthlim=None, grav: float = 1.0, gamma: float = 1.4,
              efix: int = 1, claw5: bool = False) -> OrcCfg:
-    meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5}[solver]
-    mwaves = meqn
+    meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5, SOLVER_EULER_HLLE: 5}[solver]
+    mwaves = 2 if solver == SOLVER_EULER_HLLE else meqn
     if maux is None:
         maux = 2 if solver == SOLVER_ADV else 0
     c = OrcCfg()
@@ -161,6 +161,36 @@ def bc2(q: np.ndarray, mx, my, mbc, mthbc):
     if rc != 0:
         raise ValueError("bc2: mthbc 0 (user) and 2 (periodic) are rejected")
     return qn
+
+
+def rpn2_hlle_face(gamma: float, ixy: int, qa, qb):
+    """HLLE at one face (qa low side, qb high side; ixy 1 x, 2 y): returns
+    (waves (2, 5), speeds (2,), amdq, apdq)."""
+    L = lib()
+    L.orc_rpn2_hlle_face.argtypes = [ctypes.c_double, ctypes.c_int] + [_DPT] * 7
+    L.orc_rpn2_hlle_face.restype = None
+    qa = np.ascontiguousarray(qa, dtype=np.float64)
+    qb = np.ascontiguousarray(qb, dtype=np.float64)
+    w, sp, am, ap, r = np.zeros(10), np.zeros(2), np.zeros(5), np.zeros(5), np.zeros(7)
+    L.orc_rpn2_hlle_face(gamma, ixy, _dp(qa), _dp(qb), _dp(w), _dp(sp), _dp(am), _dp(ap), _dp(r))
+    return w.reshape(2, 5), sp, am, ap
+
+
+def rpt2_hlle_face(gamma: float, ixy: int, qa, qb, asdq):
+    """Transverse split of the HLLE pair of the face (qa, qb): (bmasdq, bpasdq)
+    of the fluctuation asdq."""
+    L = lib()
+    L.orc_rpn2_hlle_face.argtypes = [ctypes.c_double, ctypes.c_int] + [_DPT] * 7
+    L.orc_rpt2_hlle_face.argtypes = [ctypes.c_int] + [_DPT] * 4
+    L.orc_rpt2_hlle_face.restype = None
+    qa = np.ascontiguousarray(qa, dtype=np.float64)
+    qb = np.ascontiguousarray(qb, dtype=np.float64)
+    d = np.ascontiguousarray(asdq, dtype=np.float64)
+    w, sp, am, ap, r = np.zeros(10), np.zeros(2), np.zeros(5), np.zeros(5), np.zeros(7)
+    L.orc_rpn2_hlle_face(gamma, ixy, _dp(qa), _dp(qb), _dp(w), _dp(sp), _dp(am), _dp(ap), _dp(r))
+    bm, bp = np.zeros(5), np.zeros(5)
+    L.orc_rpt2_hlle_face(ixy, _dp(r), _dp(d), _dp(bm), _dp(bp))
+    return bm, bp
 
 
 def limiter_phi(mthlim: int, r: float) -> float:

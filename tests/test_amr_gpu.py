@@ -110,14 +110,16 @@ def test_coarsest_level_above_minlevel_bitwise(dev, subcycle):
     assert sim.cell_updates() == amr.cell_updates == 2 * per_coarse
 
 
-def test_euler_brick_bitwise(dev):
+@pytest.mark.parametrize("solver", [(H.SOLVER_EULER5_ROE, O.SOLVER_EULER5_ROE),
+                                    (H.SOLVER_EULER_HLLE, O.SOLVER_EULER_HLLE)])
+def test_euler_brick_bitwise(dev, solver):
     g = 1.4
     pinf = 1.5698
     rinf = ((g - 1) + (g + 1) * pinf) / ((g + 1) + (g - 1) * pinf)
     vinf = (1 / np.sqrt(g)) * (pinf - 1) / np.sqrt(0.5 * ((g + 1) / g) * pinf + 0.5 * (g - 1) / g)
-    sim = make(H.SOLVER_EULER5_ROE, 16, 1, 2, mi=4, mj=1, box=(0, 2, 0, 0.5), refine=2.0)
+    sim = make(solver[0], 16, 1, 2, mi=4, mj=1, box=(0, 2, 0, 0.5), refine=2.0)
     sim.initialize("shock_bubble", (g, 0.2, rinf, vinf, pinf, 0.5, 0.25, 0.2, 0.138), nsub=4)
-    amr = oracle_from(sim, O.SOLVER_EULER5_ROE)
+    amr = oracle_from(sim, solver[1])
     dt = 0.002
     for s in range(2):
         assert sim.advance(s * dt, dt) == amr.advance(s * dt, dt)

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("name,checkpoints", [("C1", (1, 50, 200)), ("C2", (1, 40, 100)),
-                                              ("C3", (1, 10, 30))])
+                                              ("C3", (1, 10, 30)), ("C3-HLLE", (1, 10, 30))])
 def test_fp32_error_per_config(dev, name, checkpoints):
     rep = C.precision_report(name, checkpoints)
     for r in rep:

@@ -21,6 +21,7 @@ FP32_REL_L1 = 1e-5
 
 SOLVERS = {"swe": (H.SOLVER_SWE_ROE, O.SOLVER_SWE_ROE),
            "euler": (H.SOLVER_EULER5_ROE, O.SOLVER_EULER5_ROE),
+           "hlle": (H.SOLVER_EULER_HLLE, O.SOLVER_EULER_HLLE),
            "adv": (H.SOLVER_ADV, O.SOLVER_ADV)}
 
 
@@ -84,7 +85,7 @@ def _batch(kind, rng, n, mx, my):
             q, dx, dy = C.swe_state(rng, mx, my, 2)
             dts.append(C.swe_dt(q, dx, dy))
             aux = None
-        elif kind == "euler":
+        elif kind in ("euler", "hlle"):
             q, dx, dy = C.euler_state(rng, mx, my, 2)
             dts.append(C.euler_dt(q, dx, dy))
             aux = None
@@ -115,7 +116,8 @@ def _oracle_batch(kind, q, aux, dx, dy, dt, order=(2, 2), mthlim=None):
 
 @pytest.mark.parametrize("kind,mx,my,n", [("swe", 32, 32, 37), ("swe", 16, 24, 11),
                                           ("euler", 16, 16, 13), ("adv", 32, 32, 9),
-                                          ("adv", 20, 12, 7)])
+                                          ("adv", 20, 12, 7), ("hlle", 16, 16, 13),
+                                          ("hlle", 20, 12, 9)])
 @pytest.mark.parametrize("launch", [(1, 4), (2, 2), (1, 1, 3), (1, 1, 4), (1, 1, 8)])
 def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
     """Whole-patch sweeps and row bands (3, 4 and 8 waves per patch; the third
@@ -164,7 +166,7 @@ def test_fp64_methods_and_limiters(dev, order, mthlim):
 
 
 @pytest.mark.parametrize("kind,mx,n", [("swe", 32, 64), ("euler", 16, 32), ("adv", 32, 16),
-                                       ("swe", 16, 37)])
+                                       ("swe", 16, 37), ("hlle", 16, 32)])
 @pytest.mark.parametrize("launch", [None, (1, 4), (2, 4), (2, 1)])
 def test_fp32_error_vs_fp64_oracle(dev, kind, mx, n, launch):
     rng = np.random.default_rng(1000 + n)
