@@ -768,6 +768,49 @@ int Ctx::advance_body(double t, double dt)
     return HIPCLAW_OK;
 }
 
+// Copies of every level's current tiles (interiors and ghost cells) into the
+// save pool and back, as clawpatch_save_step / clawpatch_restore_step do per
+// patch (griddata_save = griddata, src/patches/clawpatch/fclaw2d_clawpatch.cpp:445-471)
+// for save_time_step / restore_time_step (src/fclaw2d_run.c:155-158, 220-236).
+// Each level's slots are contiguous, so a level is one device copy.  The
+// time-sync registers need no copy: the final time sync of a coarse step
+// resets every register, so they are zero at every step boundary.
+int Ctx::save_step()
+{
+    if (n <= 0) return set_error(HIPCLAW_ERR_STATE, "save_step: no mesh");
+    if (!save) HC_TRY(dmalloc(&save, (size_t)n * tile * esz));
+    for (int L = 0; L < MAXL; L++) {
+        const int s0 = plan[L].s0, np = plan[L].s1 - s0;
+        if (np <= 0) continue;
+        const size_t off = (size_t)s0 * tile * esz, bytes = (size_t)np * tile * esz;
+        HC_TRY(check_hip(hipMemcpyAsync(static_cast<char*>(save) + off,
+                                        static_cast<const char*>(pool[parity[L]]) + off, bytes,
+                                        hipMemcpyDeviceToDevice, stream),
+                         "save_step copy"));
+    }
+    std::memcpy(saved_pushed, pushed, sizeof(pushed));
+    saved = true;
+    saved_epoch = mesh_epoch;
+    return HIPCLAW_OK;
+}
+
+int Ctx::restore_step()
+{
+    if (!saved || saved_epoch != mesh_epoch || !save)
+        return set_error(HIPCLAW_ERR_STATE, "restore_step: no step saved on the current mesh");
+    for (int L = 0; L < MAXL; L++) {
+        const int s0 = plan[L].s0, np = plan[L].s1 - s0;
+        if (np <= 0) continue;
+        const size_t off = (size_t)s0 * tile * esz, bytes = (size_t)np * tile * esz;
+        HC_TRY(check_hip(hipMemcpyAsync(static_cast<char*>(pool[parity[L]]) + off,
+                                        static_cast<const char*>(save) + off, bytes,
+                                        hipMemcpyDeviceToDevice, stream),
+                         "restore_step copy"));
+    }
+    std::memcpy(pushed, saved_pushed, sizeof(pushed));
+    return HIPCLAW_OK;
+}
+
 int Ctx::qinit(int kind, const QinitParams& prm)
 {
     clear_pushed();
@@ -1146,6 +1189,18 @@ int hipclaw_tag_stats(hipclaw_ctx* ctx, int* stats6)
     if (!stats6) return set_error(HIPCLAW_ERR_ARG, "stats6 is NULL");
     std::memcpy(stats6, ctx->c->tag_stats, sizeof(ctx->c->tag_stats));
     return HIPCLAW_OK;
+}
+
+int hipclaw_save_step(hipclaw_ctx* ctx)
+{
+    CTX_READY(ctx);
+    return ctx->c->save_step();
+}
+
+int hipclaw_restore_step(hipclaw_ctx* ctx)
+{
+    CTX_READY(ctx);
+    return ctx->c->restore_step();
 }
 
 int hipclaw_get_launch(hipclaw_ctx* ctx, hipclaw_launch* launch)

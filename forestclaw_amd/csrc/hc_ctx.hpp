@@ -138,6 +138,13 @@ struct Ctx {
     std::vector<GraphEntry> graphs;
     long long graph_captures = 0, graph_replays = 0;
     void clear_graphs();
+    // save_time_step / restore_time_step of the run loop (clawpatch
+    // griddata_save): every level's current tiles copied on the device
+    bool saved = false;
+    long long saved_epoch = -1;
+    bool saved_pushed[MAXL] = {};
+    int save_step();
+    int restore_step();
     int qinit(int kind, const QinitParams& prm);
     int tag_and_adapt(int init_flag, int* changed);
     int mass(double* sums);

@@ -112,6 +112,8 @@ class Hipclaw46:
         # and regrids that changed the mesh
         self.count_amr_advance = 0
         self.count_amr_new_domain = 0
+        self.retakes = 0
+        self.dt_history = []
 
     def close(self):
         if self.h:
@@ -288,6 +290,14 @@ class Hipclaw46:
         self.count_amr_new_domain += int(ch.value != 0)
         return bool(ch.value)
 
+    def save_step(self):
+        """save_time_step: device copy of the current solution."""
+        _check(_lib().hipclaw_save_step(self.h))
+
+    def restore_step(self):
+        """restore_time_step: back to the last saved solution (same mesh)."""
+        _check(_lib().hipclaw_restore_step(self.h))
+
     def mass(self) -> np.ndarray:
         s = np.zeros(self.phys.meqn)
         _check(_lib().hipclaw_mass(self.h, s.ctypes.data_as(_DP)))
@@ -312,7 +322,10 @@ class Hipclaw46:
             on_step=None):
         """fclaw2d_run outstyle 1 (src/fclaw2d_run.c:119-297): variable dt from the
         CFL number unless use_fixed_dt; reject and retake a step whose maxcfl
-        exceeds max_cfl (the solution is restored from a saved copy)."""
+        exceeds max_cfl (the solution is restored from the device copy that
+        save_step made before the step, src/fclaw2d_run.c:155-158, 220-236).
+        self.retakes counts rejected steps; self.dt_history records every
+        attempted (dt, Courant number)."""
         o = self.opt
         dt_min = dt0 if dt0 is not None else o.initial_dt
         t0 = self.t
@@ -321,7 +334,8 @@ class Hipclaw46:
         for n in range(nout):
             tend = t0 + (n + 1) * dt_outer
             while self.t < tend and steps < max_steps:
-                saved = None if o.use_fixed_dt else self.get_q()
+                if not o.use_fixed_dt:
+                    self.save_step()
                 dt_step = dt_min
                 tol = 1e-2 * dt_step
                 took_small = False
@@ -331,8 +345,10 @@ class Hipclaw46:
                         dt_step = tend - self.t
                         took_small = small < 0
                 cfl = self.advance(self.t, dt_step)
+                self.dt_history.append((dt_step, cfl))
                 if cfl > o.max_cfl and not o.use_fixed_dt:
-                    self.set_q(saved)
+                    self.restore_step()
+                    self.retakes += 1
                     dt_min = dt_min * o.desired_cfl / cfl
                     continue
                 self.n_inner += 1

@@ -123,6 +123,8 @@ _SIGS.update({
     "hipclaw_init_leaves": ([_VP, ctypes.c_int, _IP, _IP, _IP], ctypes.c_int),
     "hipclaw_set_launch": ([_VP, ctypes.c_void_p], ctypes.c_int),
     "hipclaw_get_launch": ([_VP, ctypes.POINTER(Launch)], ctypes.c_int),
+    "hipclaw_save_step": ([_VP], ctypes.c_int),
+    "hipclaw_restore_step": ([_VP], ctypes.c_int),
     "hipclaw_set_time_sync": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_src2": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_feature": ([_VP, ctypes.c_int, ctypes.c_int], ctypes.c_int),

@@ -368,6 +368,18 @@ int hipclaw_ghost_update(hipclaw_ctx* ctx, int minlevel, int maxlevel, double sy
  * every patch of the level in one batched launch; *cfl = level maximum. */
 int hipclaw_update_level(hipclaw_ctx* ctx, int level, double t, double dt, double* cfl);
 
+/* save_time_step / restore_time_step of the run loop (src/fclaw2d_run.c:155-158,
+ * 220-236; patch_vt->save_step / restore_step = clawpatch_save_step /
+ * clawpatch_restore_step, src/patches/clawpatch/fclaw2d_clawpatch.cpp:445-471):
+ * _save copies every patch's current solution (ghost cells included) into a
+ * device save pool, _restore copies it back, so a step whose Courant number
+ * exceeds max_cfl is retaken without leaving the device.  Both are
+ * stream-ordered copies.  The save pool is allocated on the first save after
+ * each mesh change; restore fails (HIPCLAW_ERR_STATE) when nothing was saved
+ * on the current mesh. */
+int hipclaw_save_step(hipclaw_ctx* ctx);
+int hipclaw_restore_step(hipclaw_ctx* ctx);
+
 /* fclaw2d_timeinterp (src/fclaw2d_timeinterp.c:51-57). */
 int hipclaw_timeinterp(hipclaw_ctx* ctx, int level, double alpha);
 

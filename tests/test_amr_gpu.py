@@ -368,3 +368,50 @@ def test_mass_is_the_interior_sum(dev, precision):
     ghost = math.fsum((q[:, 0] * area[:, None, None]).ravel()) - math.fsum(
         (q[:, 0, 2:-2, 2:-2] * area[:, None, None]).ravel())
     assert abs(ghost) > 1e6 * 1e-14 * abs(got[0])
+
+
+def test_save_and_restore_step_on_the_device(dev):
+    """hipclaw_save_step / hipclaw_restore_step: a restore after two coarse
+    steps gives back the saved solution bit for bit (ghost cells included);
+    restore without a save on the current mesh is an error."""
+    sim = make(H.SOLVER_SWE_ROE, 8, 2, 4, refine=0.5)
+    sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+    with pytest.raises(H.HipclawError):
+        sim.restore_step()
+    q0 = sim.get_q()
+    sim.save_step()
+    for s in range(2):
+        sim.advance(s * 0.01, 0.01)
+    assert not np.array_equal(sim.get_q(), q0)
+    sim.restore_step()
+    assert np.array_equal(sim.get_q(), q0)
+    # the restored state steps like the original one
+    c1 = sim.advance(0.0, 0.01)
+    q1 = sim.get_q()
+    sim.restore_step()
+    assert sim.advance(0.0, 0.01) == c1 and np.array_equal(sim.get_q(), q1)
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_rejected_step_is_retaken_from_the_device_copy(dev, precision):
+    """fclaw2d_run with variable dt: an initial dt whose Courant number exceeds
+    max_cfl is rejected, the solution restored on the device and the step
+    retaken at dt * desired_cfl / cfl.  The run ends bitwise equal to one
+    started at that smaller dt, whose attempts are the first run's after its
+    first rejection (later rejections, if the flow or a regrid raise the
+    Courant number, happen in both)."""
+    def run(dt0):
+        opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_SWE_ROE,
+                        precision=precision, refine_threshold=0.5, coarsen_threshold=0.05,
+                        use_fixed_dt=False, max_cfl=1.0, desired_cfl=0.9, regrid_interval=2)
+        sim = F.Hipclaw46(opt)
+        sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        sim.run(0.2, nout=1, dt0=dt0, max_steps=5)
+        return sim
+    a = run(0.05)
+    assert a.retakes >= 1 and a.dt_history[0][1] > 1.0
+    b = run(a.dt_history[1][0])
+    assert b.retakes == a.retakes - 1
+    assert b.dt_history == a.dt_history[1:]
+    assert np.array_equal(a.get_q(), b.get_q())
+    assert a.t == b.t
