@@ -414,6 +414,7 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         a.push = ml->push;
         a.swirl = ml->swirl;
         a.xylower = ml->xylower;
+        a.par = ml->par;
         for (int L = 0; L < STEP_MAXL; L++) a.vtl[L] = ml->vtl[L];
         for (int L = 0; L < STEP_MAXL; L++) {
             a.dtl[L] = ml->dtl[L];

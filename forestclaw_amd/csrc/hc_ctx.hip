@@ -48,6 +48,7 @@ Ctx::Ctx(const hipclaw_physics& p, const hipclaw_domain& d)
 Ctx::~Ctx()
 {
     clear_graphs();
+    free_par();
     prof_clear();
     release_pools();
     if (d_tiles) (void)hipFree(d_tiles);
@@ -58,6 +59,47 @@ void Ctx::clear_graphs()
 {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     graphs.clear();
+}
+
+int Ctx::ensure_par()
+{
+    if (d_par) return HIPCLAW_OK;
+    const size_t n = (size_t)PAR_MAXLAUNCH * PAR_STRIDE;
+    if (hipMalloc(reinterpret_cast<void**>(&d_par), sizeof(double) * n) != hipSuccess) {
+        d_par = nullptr;
+        return set_error(HIPCLAW_ERR_NOMEM, "hipMalloc of the step parameter block failed");
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&h_par), sizeof(double) * n * PAR_RING, hipHostMallocDefault) !=
+        hipSuccess) {
+        h_par = nullptr;
+        free_par();
+        return set_error(HIPCLAW_ERR_NOMEM, "hipHostMalloc of the step parameter ring failed");
+    }
+    for (int k = 0; k < PAR_RING; k++) {
+        if (hipEventCreateWithFlags(&par_ev[k], hipEventDisableTiming) != hipSuccess) {
+            par_ev[k] = nullptr;
+            free_par();
+            return set_error(HIPCLAW_ERR_HIP, "event creation failed");
+        }
+        par_ev_used[k] = false;
+    }
+    return HIPCLAW_OK;
+}
+
+void Ctx::free_par()
+{
+    for (int k = 0; k < PAR_RING; k++) {
+        if (par_ev[k]) {
+            if (par_ev_used[k]) (void)hipEventSynchronize(par_ev[k]);
+            (void)hipEventDestroy(par_ev[k]);
+        }
+        par_ev[k] = nullptr;
+        par_ev_used[k] = false;
+    }
+    if (h_par) (void)hipHostFree(h_par);
+    if (d_par) (void)hipFree(d_par);
+    h_par = nullptr;
+    d_par = nullptr;
 }
 
 void Ctx::prof_clear()
@@ -296,6 +338,7 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
 // ---------------------------------------------------------------------------
 int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
 {
+    if (par_dry) return HIPCLAW_OK;
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
     const int mincoarse = minlevel, maxcoarse = maxlevel - 1;
     const int tl = minlevel - 1;
@@ -508,7 +551,7 @@ int Ctx::set_time_sync(bool on)
 // [minlevel, maxlevel-1], physical faces of [minlevel, maxlevel].
 int Ctx::time_sync(int minlevel, int maxlevel)
 {
-    if (!tsync) return HIPCLAW_OK;
+    if (!tsync || par_dry) return HIPCLAW_OK;
     clear_pushed();
     if (minlevel < 0 || maxlevel >= MAXL || minlevel > maxlevel)
         return set_error(HIPCLAW_ERR_ARG, "time_sync: bad level range");
@@ -533,6 +576,25 @@ int Ctx::time_sync(int minlevel, int maxlevel)
 
 int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
 {
+    // graph mode: this launch's dt and vt per level into the parameter block
+    // (written in the dry pass; the capture that follows only takes the
+    // launch's slot address, since the slot's upload may still be in flight)
+    const double* par_dev = nullptr;
+    if (par_host) {
+        if (par_n >= PAR_MAXLAUNCH) return set_error(HIPCLAW_ERR_STATE, "too many step launches for a graph");
+        par_dev = d_par + (size_t)par_n * PAR_STRIDE;
+        if (par_dry) {
+            double* hp = par_host + (size_t)par_n * PAR_STRIDE;
+            const bool fz = fused_swirl();
+            for (int L = lo; L <= hi; L++) {
+                hp[L] = dtl[L];
+                hp[STEP_MAXL + L] = fz ? swirl_vt(tl[L], dtl[L]) : 0.0;
+            }
+            par_n++;
+            return HIPCLAW_OK;
+        }
+        par_n++;
+    }
     if (!merge_levels) {
         for (int L = hi; L >= lo; L--) HC_TRY(update_level(L, tl[L], dtl[L]));
         return HIPCLAW_OK;
@@ -569,11 +631,13 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
     const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
     const bool push = push_ghosts && !tsync && !src2 && phys.mx >= 4 && phys.my >= 4;
     ml.push = push ? d_push : nullptr;
+    ml.par = par_dev;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
     if (src2) {
         Src2Args sa = src2_args(s0, np);
         sa.levels = d_levels;
+        sa.par = par_dev;
         for (int L = lo; L <= hi; L++) {
             sa.q_l[L] = ml.qout_l[L];
             sa.dtl[L] = dtl[L];
@@ -597,6 +661,7 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
 
 int Ctx::timeinterp(int level, double alpha)
 {
+    if (par_dry) return HIPCLAW_OK;
     const LevelPlan& lp = plan[level];
     if (esz == 8)
         launch_timeinterp<double>(lp.ti.p, lp.ti.n, pools<double>(), phys.mx, phys.my, phys.meqn, alpha, stream);
@@ -620,53 +685,98 @@ static unsigned level_bits(const bool* f)
 }
 
 // Coarse step with the Courant-number reduction; eager or as a hipGraph.
+// A graph records the device work of one coarse step; what depends on (t, dt),
+// each step launch's dt and swirl vt per level, comes from the parameter
+// block d_par, which a dry pass of the schedule fills on the host (the same
+// arithmetic as the eager path) and one copy uploads before each launch.  The
+// schedule itself (which levels step when, the time-interpolation weights)
+// depends only on the step counters, so a graph serves every dt; it is keyed
+// by the pool parities at entry and the mesh.
 int Ctx::advance(double t, double dt, double* maxcfl)
 {
-    const bool graph_ok = use_graph && stream != nullptr && !prof &&
-                          !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV);
+    const bool graph_ok = use_graph && stream != nullptr && !prof && merge_levels &&
+                          !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV &&
+                            !fused_swirl());
     if (!graph_ok) {
         HC_TRY(advance_body(t, dt));
         if (maxcfl) HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
     } else {
-        // Key: dt, pool parities, mesh.  The ghost-push flags at entry do not
+        HC_TRY(ensure_par());
+        // host parameters of this coarse step into a free ring slot
+        const int slot = par_slot;
+        par_slot = (par_slot + 1) % PAR_RING;
+        if (par_ev_used[slot]) HC_TRY(check_hip(hipEventSynchronize(par_ev[slot]), "parameter slot"));
+        par_host = h_par + (size_t)slot * PAR_MAXLAUNCH * PAR_STRIDE;
+        par_n = 0;
+        par_dry = true;
+        const int cnt0 = cflacc_count;
+        int rc = advance_body(t, dt);
+        par_dry = false;
+        cflacc_count = cnt0;
+        const int nlaunch = par_n;
+        if (rc) {
+            par_host = nullptr;
+            return rc;
+        }
+        rc = check_hip(hipMemcpyAsync(d_par, par_host, sizeof(double) * (size_t)nlaunch * PAR_STRIDE,
+                                      hipMemcpyHostToDevice, stream),
+                       "parameter upload");
+        if (!rc) rc = check_hip(hipEventRecord(par_ev[slot], stream), "parameter event");
+        if (rc) {
+            par_host = nullptr;
+            return rc;
+        }
+        par_ev_used[slot] = true;
+        // Key: pool parities, mesh.  The ghost-push flags at entry do not
         // change the recorded work: the first fine step updates every level,
         // which sets them, before any ghost fill reads them.
         const unsigned pin = level_bits(parity), qin = level_bits(pushed);
         GraphEntry* e = nullptr;
         for (auto& g : graphs)
-            if (g.dt == dt && g.parity_in == pin && g.epoch == mesh_epoch) e = &g;
+            if (g.parity_in == pin && g.epoch == mesh_epoch) e = &g;
         if (!e) {
             // capture: the body runs its host-side bookkeeping as usual while the
-            // device work is recorded; restore the bookkeeping if capture fails
+            // device work is recorded (its kernels read d_par); restore the
+            // bookkeeping if capture fails
             int par0[MAXL];
             bool push0[MAXL];
             std::memcpy(par0, parity, sizeof(par0));
             std::memcpy(push0, pushed, sizeof(push0));
             const long long cells0 = cell_updates;
-            HC_TRY(check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture"));
-            int rc = advance_body(t, dt);
-            if (!rc) rc = hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream);
-            hipGraph_t g = nullptr;
-            const hipError_t ec = hipStreamEndCapture(stream, &g);
-            hipGraphExec_t ex = nullptr;
-            if (!rc && ec == hipSuccess && g) rc = check_hip(hipGraphInstantiateWithFlags(&ex, g, 0), "instantiate");
-            else if (!rc) rc = check_hip(ec, "end capture");
-            if (g) (void)hipGraphDestroy(g);
+            par_n = 0;
+            rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            if (!rc) {
+                rc = advance_body(t, dt);
+                if (!rc) rc = hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream);
+                if (!rc && par_n != nlaunch) rc = set_error(HIPCLAW_ERR_STATE, "graph schedule mismatch");
+                hipGraph_t g = nullptr;
+                const hipError_t ec = hipStreamEndCapture(stream, &g);
+                hipGraphExec_t ex = nullptr;
+                if (!rc && ec == hipSuccess && g)
+                    rc = check_hip(hipGraphInstantiateWithFlags(&ex, g, 0), "instantiate");
+                else if (!rc)
+                    rc = check_hip(ec, "end capture");
+                if (g) (void)hipGraphDestroy(g);
+                if (!rc) {
+                    if (graphs.size() >= 4) {
+                        (void)hipGraphExecDestroy(graphs.front().exec);
+                        graphs.erase(graphs.begin());
+                    }
+                    graphs.push_back({pin, qin, mesh_epoch, ex, level_bits(parity), level_bits(pushed),
+                                      cell_updates - cells0, cflacc_count});
+                    e = &graphs.back();
+                    graph_captures++;
+                }
+            }
+            par_host = nullptr;
             if (rc) {
                 std::memcpy(parity, par0, sizeof(par0));
                 std::memcpy(pushed, push0, sizeof(push0));
                 cell_updates = cells0;
                 return rc;
             }
-            if (graphs.size() >= 4) {
-                (void)hipGraphExecDestroy(graphs.front().exec);
-                graphs.erase(graphs.begin());
-            }
-            graphs.push_back({dt, pin, qin, mesh_epoch, ex, level_bits(parity), level_bits(pushed),
-                              cell_updates - cells0, cflacc_count});
-            e = &graphs.back();
-            graph_captures++;
         } else {
+            par_host = nullptr;
             graph_replays++;
             for (int L = 0; L < MAXL; L++) {
                 parity[L] = (e->parity_out >> L) & 1u;
@@ -1034,6 +1144,8 @@ int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, v
         return set_error(HIPCLAW_ERR_ARG, e.what());
     }
     h->c->stream = static_cast<hipStream_t>(stream);
+    // whole coarse steps replay as graphs where the stream can be captured
+    h->c->use_graph = h->c->stream != nullptr;
     *out = h;
     return HIPCLAW_OK;
 }

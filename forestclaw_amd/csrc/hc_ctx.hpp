@@ -127,7 +127,6 @@ struct Ctx {
     bool use_graph = false;
     long long mesh_epoch = 0;
     struct GraphEntry {
-        double dt;
         unsigned parity_in, pushed_in;
         long long epoch;
         hipGraphExec_t exec;
@@ -138,6 +137,23 @@ struct Ctx {
     std::vector<GraphEntry> graphs;
     long long graph_captures = 0, graph_replays = 0;
     void clear_graphs();
+    // Step parameters of graph replays: per step launch of a coarse step,
+    // dt and the swirl vt of every level (PAR_STRIDE doubles), computed on
+    // the host by a dry pass of the schedule, staged in a pinned ring slot
+    // and copied to d_par ahead of each graph launch on the context stream.
+    static constexpr int PAR_STRIDE = 2 * STEP_MAXL;
+    static constexpr int PAR_MAXLAUNCH = 4096;
+    static constexpr int PAR_RING = 4;
+    double* d_par = nullptr;
+    double* h_par = nullptr;          // pinned, PAR_RING slots
+    hipEvent_t par_ev[PAR_RING] = {};
+    bool par_ev_used[PAR_RING] = {};
+    int par_slot = 0;
+    double* par_host = nullptr;       // slot being filled (nullptr: no recording)
+    int par_n = 0;                    // launches recorded in this coarse step
+    bool par_dry = false;             // schedule only: record parameters, launch nothing
+    int ensure_par();
+    void free_par();
     // save_time_step / restore_time_step of the run loop (clawpatch
     // griddata_save): every level's current tiles copied on the device
     bool saved = false;

@@ -73,6 +73,9 @@ struct MultiLevel {
     int swirl;
     const double* xylower;
     double vtl[STEP_MAXL];
+    // optional device parameter block: dt of level L at par[L], vt at
+    // par[STEP_MAXL + L] (graph replays; see StepArgs::par)
+    const double* par;
 };
 int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                  const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,
@@ -91,6 +94,7 @@ struct Src2Args {
     const double* dxdy;
     int mx, my, meqn, maux;
     double gamma1;
+    const double* par;  // optional device parameter block: dt of level L at par[L]
 };
 int src2_launch(const Src2Args& a, int kind, int precision, const void* aux, hipStream_t s);
 

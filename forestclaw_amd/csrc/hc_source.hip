@@ -31,7 +31,7 @@ __global__ void k_src2_axisym(Src2Args a, const R* __restrict__ aux)
     const int k = blockIdx.x;
     const int slot = a.slot0 + k;
     const int L = a.levels ? a.levels[slot] : 0;
-    const double dt = a.dtl[L];
+    const double dt = a.par ? a.par[L] : a.dtl[L];
     R* q = static_cast<R*>(a.q_l[L]);
     const int NX = a.mx + 2 * MBC, NY = a.my + 2 * MBC;
     const long long T = (long long)NX * NY;

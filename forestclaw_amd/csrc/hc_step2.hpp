@@ -68,6 +68,10 @@ struct StepArgs {
     const double* xylower;
     double vt;
     double vtl[STEP_MAXL];
+    // Multi-level launches replayed from a hipGraph: dt and vt of level L
+    // read from the device at par[L] and par[STEP_MAXL + L] (written before
+    // each replay) instead of dtl / vtl, so one graph serves every dt.
+    const double* par;
     // Row bands (nband > 1, q_out != q_in): a workgroup of
     // nband waves takes the same ppw patches, wave b the interior rows
     // [2 + b*band_rows, 2 + (b+1)*band_rows) (clipped to my), sweeping from
@@ -186,7 +190,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const int lev = (A.levels && pact) ? A.levels[tix] : 0;
     const R* qbase = A.levels ? static_cast<const R*>(A.qin_l[lev]) : qin;
     R* obase = A.levels ? static_cast<R*>(A.qout_l[lev]) : qout;
-    const double dtp = A.levels ? A.dtl[lev] : A.dt;
+    const double dtp = A.levels ? (A.par ? A.par[lev] : A.dtl[lev]) : A.dt;
     const R* __restrict__ qp = qbase + (long long)tix * A.qstride;
     R* __restrict__ qo = obase + (long long)tix * A.qstride;
     const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)tix * A.astride : nullptr;
@@ -287,7 +291,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
             }
             swy = ys;
-            const double vt = A.levels ? A.vtl[lev] : A.vt;
+            const double vt = A.levels ? (A.par ? A.par[STEP_MAXL + lev] : A.vtl[lev]) : A.vt;
             swvty = vt / dy;
             swvtx = vt / dx;
             __syncthreads();

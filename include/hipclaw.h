@@ -298,10 +298,14 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
  *   GHOST_PUSH    the step writes updated edge cells into the ghost cells of
  *                 same-level neighbours, replacing the same-level copies of the
  *                 next ghost fill (not used while time sync is on)
- *   GRAPH         (off by default) each coarse step is captured once as a
- *                 hipGraph and replayed while dt, the pool parities and the mesh
- *                 are unchanged; needs a non-default stream, not used while
- *                 profiling or with time-dependent swirl velocities */
+ *   GRAPH         (on by default when the context's stream is not the null
+ *                 stream) each coarse step is captured once as a hipGraph and
+ *                 replayed while the pool parities, the mesh and the features
+ *                 are unchanged, for every t and dt: the step launches read dt
+ *                 and the swirl time factor from a device parameter block
+ *                 uploaded before each replay.  Not used while profiling,
+ *                 with MERGE_LEVELS off, or with the swirl field stored in
+ *                 aux (FUSE_AUX off). */
 #define HIPCLAW_FEATURE_MERGE_LEVELS 1
 #define HIPCLAW_FEATURE_GHOST_PUSH 2
 #define HIPCLAW_FEATURE_GRAPH 3

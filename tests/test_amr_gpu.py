@@ -341,6 +341,56 @@ def test_graph_replay_matches_eager(dev, ts):
     assert sims[0].graph_stats() == (0, 0)
 
 
+@pytest.mark.parametrize("case", ["swe", "swe_time_sync", "swirl", "euler_src2"])
+def test_graph_replays_serve_every_dt(dev, case):
+    """One captured coarse step per pool-parity state serves every (t, dt):
+    with dt changing from step to step (as under CFL control), and for the
+    swirl field whose velocities depend on t and dt, graph replays give the
+    eager bits after every step."""
+    import torch
+    stream = torch.cuda.Stream()
+    g = 1.4
+    pinf = 1.5698
+    rinf = ((g - 1) + (g + 1) * pinf) / ((g + 1) + (g - 1) * pinf)
+    vinf = (1 / np.sqrt(g)) * (pinf - 1) / np.sqrt(0.5 * ((g + 1) / g) * pinf + 0.5 * (g - 1) / g)
+    sims = []
+    for graph in (False, True):
+        if case == "swirl":
+            opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_ADV,
+                            refine_threshold=0.2, coarsen_threshold=0.05,
+                            refinement_criteria="minmax")
+            sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+            sim.set_swirl(8.0)
+            sim.initialize("swirl", (0.5,))
+        elif case == "euler_src2":
+            opt = F.Options(minlevel=1, maxlevel=3, mx=8, my=8, mi=4, mj=1, ax=0, bx=2, ay=0, by=0.5,
+                            solver=H.SOLVER_EULER5_ROE, src2=H.SRC2_AXISYMMETRIC,
+                            refine_threshold=0.1, coarsen_threshold=0.02, mthbc=(1, 1, 3, 1))
+            sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+            sim.initialize("shock_bubble", (g, 0.2, rinf, vinf, pinf, 0.5, 0.25, 0.2, 0.138), nsub=4)
+        else:
+            opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_SWE_ROE,
+                            refine_threshold=0.5, coarsen_threshold=0.05,
+                            time_sync=case == "swe_time_sync")
+            sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+            sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        sim.set_feature(H.FEATURE_GRAPH, graph)
+        sims.append(sim)
+    lev = sims[0].patch_info()["level"]
+    assert len(set(lev)) >= 2, f"{case}: needs two levels"
+    base = 0.004 if case != "euler_src2" else 0.002
+    t = 0.0
+    for s, f in enumerate((1.0, 0.8, 0.93, 0.71, 1.0, 0.84, 0.77)):
+        dt = base * f
+        c = [sim.advance(t, dt) for sim in sims]
+        t += dt
+        assert c[0] == c[1], (case, s)
+        assert np.array_equal(sims[0].get_q(), sims[1].get_q()), (case, s)
+    caps, reps = sims[1].graph_stats()
+    assert caps <= 2 and caps + reps == 7 and reps >= 5, (caps, reps)
+    assert sims[0].graph_stats() == (0, 0)
+
+
 @pytest.mark.parametrize("precision", [H.FP64, H.FP32])
 def test_mass_is_the_interior_sum(dev, precision):
     """hipclaw_mass (the conservation check of
