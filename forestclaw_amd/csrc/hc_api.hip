@@ -395,6 +395,7 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         for (int w = 0; w < 5; w++)
             if (a.mthlim[w] == 6) a.mthlim[w] = 1;
     a.pp.grav = phys->grav;
+    a.pp.sgrav = sqrt(phys->grav);
     a.pp.gamma = phys->gamma;
     a.pp.efix = phys->efix;
     a.dt = dt;

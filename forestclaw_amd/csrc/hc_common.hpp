@@ -41,6 +41,11 @@ template <> struct M<float> {
 
 template <class R> __device__ inline R dmin(R a, R b) { return a < b ? a : b; }
 template <class R> __device__ inline R dmax(R a, R b) { return a > b ? a : b; }
+// FP32: the hardware v_min_f32 / v_max_f32 (one instruction instead of a
+// compare and a select; equal for the finite data of a step up to the sign
+// of a zero).  FP64 keeps the reference's comparisons.
+template <> __device__ inline float dmin<float>(float a, float b) { return __builtin_fminf(a, b); }
+template <> __device__ inline float dmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
 
 // Whole-wave neighbour exchange (lane l receives lane l-1 / l+1) with the
 // GFX9 DPP wave_shr:1 / wave_shl:1 row controls: one VALU move, no LDS round

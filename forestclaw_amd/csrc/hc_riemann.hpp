@@ -42,6 +42,7 @@ struct PhysParams {
     double grav;
     double gamma;
     int efix;
+    double sgrav;  // sqrt(grav), for the FP32 shallow-water solver
 };
 
 // ---------------------------------------------------------------------------
@@ -58,6 +59,10 @@ template <class R> struct Adv {
 
     static constexpr int NC = 0;  // no per-cell data
     __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+    // component m of wave w is zero by construction (none here)
+    template <int DIR> __device__ static constexpr bool wz(int, int) { return false; }
+    static constexpr bool RPT2_SCALED = false;
+    template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     template <int DIR>
     __device__ static inline void rpn2(const R* qa, const R* qb, const R*, const R*, R sface,
@@ -114,6 +119,12 @@ template <class R, bool C5> struct SweT {
     // operations, so no bit changes); the FP32 kernels (NC = 0) evaluate them
     // per face, which keeps their register footprint within two waves per SIMD.
     static constexpr int NC = sizeof(R) == 8 ? 6 : 0;
+    // component m of wave w is zero by construction: the shear wave carries
+    // only the transverse momentum
+    template <int DIR> __device__ static constexpr bool wz(int w, int m)
+    {
+        return w == 1 && m != (DIR == 0 ? 2 : 1);
+    }
     __device__ static inline void cell(const R* q, const PhysParams& pp, R* cd)
     {
         using MM = M<R>;
@@ -127,12 +138,122 @@ template <class R, bool C5> struct SweT {
         cd[5] = MM::sqrt_(g * q[0]);
     }
 
+    // FP32 form of rpn2: the same waves, speeds and entropy fix, evaluated
+    // with fewer transcendental instructions, rounding aside.  One reciprocal
+    // square root per state gives sqrt(h) = h r, hu/sqrt(h) = hu r,
+    // u = hu r^2 and sqrt(g h) = sqrt(g) sqrt(h); one of g hbar gives both
+    // a = g hbar t and 1/(2a) = t/2; the shear wave's two zero components are
+    // left out of the sums.
+    template <int DIR>
+    __device__ static inline void rpn2_f32(const R* qa, const R* qb, const PhysParams& pp, Face& f,
+                                           Trans& roe)
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        const R g = R(pp.grav), sg = R(pp.sgrav);
+        const R ra = __builtin_amdgcn_rsqf(qa[0]), rb = __builtin_amdgcn_rsqf(qb[0]);
+        const R hsl = qa[0] * ra, hsr = qb[0] * rb;
+        const R iq = __builtin_amdgcn_rcpf(hsl + hsr);
+        const R u = (qa[mu] * ra + qb[mu] * rb) * iq;
+        const R v = (qa[mv] * ra + qb[mv] * rb) * iq;
+        const R gh = g * ((qa[0] + qb[0]) * R(0.5));
+        const R t = __builtin_amdgcn_rsqf(gh);
+        const R a = gh * t, i2a = R(0.5) * t;
+        roe.u = u; roe.v = v; roe.a = a;
+        const R d0 = qb[0] - qa[0], d1 = qb[mu] - qa[mu], d2 = qb[mv] - qa[mv];
+        const R a1 = ((u + a) * d0 - d1) * i2a;
+        const R a3 = (d1 - (u - a) * d0) * i2a;
+        const R a2 = d2 - v * d0;
+        const R s0 = u - a, s2 = u + a;
+        f.w[0][0] = a1; f.w[0][mu] = a1 * s0; f.w[0][mv] = a1 * v; f.s[0] = s0;
+        f.w[1][0] = R(0); f.w[1][mu] = R(0); f.w[1][mv] = a2; f.s[1] = u;
+        f.w[2][0] = a3; f.w[2][mu] = a3 * s2; f.w[2][mv] = a3 * v; f.s[2] = s2;
+        if (!pp.efix) {
+            const R m0 = dmin(s0, R(0)), m2 = dmin(s2, R(0)), m1 = dmin(u, R(0));
+            const R p0 = dmax(s0, R(0)), p2 = dmax(s2, R(0)), p1 = dmax(u, R(0));
+            for (int m = 0; m < 3; m++) {
+                f.am[m] = m0 * f.w[0][m] + m2 * f.w[2][m];
+                f.ap[m] = p0 * f.w[0][m] + p2 * f.w[2][m];
+            }
+            f.am[mv] = f.am[mv] + m1 * a2;
+            f.ap[mv] = f.ap[mv] + p1 * a2;
+            return;
+        }
+        // entropy fix (the branches of rpn2): amdq from the left-going parts
+        R am0 = R(0), amu = R(0), amv = R(0);
+        do {
+            const R sa = qa[mu] * (ra * ra) - sg * hsl;  // u - c of the low state
+            if (sa > R(0) && s0 > R(0)) break;
+            const R h1 = qa[0] + a1, hu1 = qa[mu] + a1 * s0;
+            const R r1 = __builtin_amdgcn_rsqf(h1);
+            const R s1 = hu1 * (r1 * r1) - sg * (h1 * r1);
+            R sfract;
+            if (sa < R(0) && s1 > R(0))
+                sfract = sa * ((s1 - s0) / (s1 - sa));
+            else if (s0 < R(0))
+                sfract = s0;
+            else
+                sfract = R(0);
+            am0 = sfract * a1; amu = sfract * f.w[0][mu]; amv = sfract * f.w[0][mv];
+            if (u > R(0)) break;
+            amv = amv + u * a2;
+            const R sb = qb[mu] * (rb * rb) + sg * hsr;  // u + c of the high state
+            const R h3 = qb[0] - a3, hu3 = qb[mu] - a3 * s2;
+            const R r3 = __builtin_amdgcn_rsqf(h3);
+            const R s3 = hu3 * (r3 * r3) + sg * (h3 * r3);
+            if (s3 < R(0) && sb > R(0))
+                sfract = s3 * ((sb - s2) / (sb - s3));
+            else if (s2 < R(0))
+                sfract = s2;
+            else
+                break;
+            am0 = am0 + sfract * a3; amu = amu + sfract * f.w[2][mu]; amv = amv + sfract * f.w[2][mv];
+        } while (0);
+        f.am[0] = am0; f.am[mu] = amu; f.am[mv] = amv;
+        f.ap[0] = (s0 * a1 + s2 * a3) - am0;
+        f.ap[mu] = (s0 * f.w[0][mu] + s2 * f.w[2][mu]) - amu;
+        f.ap[mv] = (s0 * f.w[0][mv] + u * a2 + s2 * f.w[2][mv]) - amv;
+    }
+
+    // FP32 form of rpt2: bm = sum_w min(sb_w, 0) alpha_w r_w with the
+    // eigenvectors r_1 = (1, u, v - a), r_2 = (0, 1, 0), r_3 = (1, u, v + a)
+    // written out, so the zero components cost nothing.
+    // FP32 split scaled by sc (the step's dt/(2 dx)): sc enters the three
+    // wave coefficients instead of the six outputs
+    static constexpr bool RPT2_SCALED = sizeof(R) == 4;
+    template <int DIR>
+    __device__ static inline void rpt2_scaled(const R* asdq, const Trans& roe, R sc, R* bm, R* bp)
+    {
+        rpt2_f32<DIR>(asdq, roe, bm, bp, sc);
+    }
+
+    template <int DIR>
+    __device__ static inline void rpt2_f32(const R* asdq, const Trans& roe, R* bm, R* bp, R sc = R(1))
+    {
+        constexpr int mu = DIR == 0 ? 1 : 2;
+        constexpr int mv = DIR == 0 ? 2 : 1;
+        const R u = roe.u, v = roe.v, a = roe.a;
+        const R i2a = (R(0.5) * sc) * __builtin_amdgcn_rcpf(a);
+        const R sm = v - a, sp = v + a;
+        const R a1 = (sp * asdq[0] - asdq[mv]) * i2a;
+        const R a3 = (asdq[mv] - sm * asdq[0]) * i2a;
+        const R a2 = (asdq[mu] - u * asdq[0]) * sc;
+        const R m1 = dmin(sm, R(0)) * a1, m2 = dmin(v, R(0)) * a2, m3 = dmin(sp, R(0)) * a3;
+        const R p1 = dmax(sm, R(0)) * a1, p2 = dmax(v, R(0)) * a2, p3 = dmax(sp, R(0)) * a3;
+        bm[0] = m1 + m3; bm[mu] = u * (m1 + m3) + m2; bm[mv] = sm * m1 + sp * m3;
+        bp[0] = p1 + p3; bp[mu] = u * (p1 + p3) + p2; bp[mv] = sm * p1 + sp * p3;
+    }
+
     template <int DIR>
     __device__ static inline void rpn2(const R* qa, const R* qb, const R* ca, const R* cb, R,
                                        const PhysParams& pp, Face& f, Trans& roe)
     {
         constexpr int mu = DIR == 0 ? 1 : 2;
         constexpr int mv = DIR == 0 ? 2 : 1;
+        if constexpr (sizeof(R) == 4) {
+            rpn2_f32<DIR>(qa, qb, pp, f, roe);
+            return;
+        }
         using MM = M<R>;
         const R g = R(pp.grav);
         R h = (qa[0] + qb[0]) / R(2);
@@ -233,6 +354,10 @@ template <class R, bool C5> struct SweT {
     {
         constexpr int mu = DIR == 0 ? 1 : 2;
         constexpr int mv = DIR == 0 ? 2 : 1;
+        if constexpr (sizeof(R) == 4) {
+            rpt2_f32<DIR>(asdq, roe, bm, bp);
+            return;
+        }
         using MM = M<R>;
         const R u = roe.u, v = roe.v, a = roe.a;
         R a1, a3;
@@ -271,6 +396,15 @@ template <class R, bool C5> struct Euler5T {
 
     static constexpr int NC = 0;  // no per-cell data
     __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+    // component m of wave w is zero by construction: the acoustic and
+    // entropy waves carry no tracer, the shear wave only the transverse
+    // momentum and energy, the tracer wave only the tracer
+    template <int DIR> __device__ static constexpr bool wz(int w, int m)
+    {
+        return w == 4 ? m != 4 : (m == 4 || (w == 1 && (m == 0 || m == (DIR == 0 ? 1 : 2))));
+    }
+    static constexpr bool RPT2_SCALED = false;
+    template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     template <int DIR>
     __device__ static inline void rpn2(const R* qa, const R* qb, const R*, const R*, R,
@@ -440,6 +574,9 @@ template <class R> struct EulerHlle {
 
     static constexpr int NC = 0;  // no per-cell data
     __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+    template <int DIR> __device__ static constexpr bool wz(int, int) { return false; }
+    static constexpr bool RPT2_SCALED = false;
+    template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     // normal flux, normal velocity and sound speed of one state
     template <int DIR>

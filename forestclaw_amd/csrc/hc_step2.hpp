@@ -107,6 +107,20 @@ __device__ inline void reg_sub(double* r, double s, double v)
     *r = *r - s * v;
 }
 
+// 0 + x and 0 - x: the reference accumulates fluxes into zeroed arrays, and
+// FP64 keeps those additions (they decide the sign of a zero result); FP32
+// takes x and -x, which drops an instruction each.
+template <class R> __device__ inline R zadd(R x)
+{
+    if constexpr (sizeof(R) == 4) return x;
+    else return R(0) + x;
+}
+template <class R> __device__ inline R zsub(R x)
+{
+    if constexpr (sizeof(R) == 4) return -x;
+    else return R(0) - x;
+}
+
 template <int CPL, class T> __device__ inline T from_left(const T (&v)[CPL], int k)
 {
     T s = shl_up(v[CPL - 1]);
@@ -157,6 +171,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     constexpr int ME = S::ME, MW = S::MW;
     using Face = typename S::Face;
     using Trans = typename S::Trans;
+    // FP32 sums over a wave's components skip those that are zero by
+    // construction (FP64 keeps every term of the reference's sums)
+    constexpr bool SKIPZ = sizeof(R) == 4;
 
     const int lane = threadIdx.x & (HC_WAVE - 1);
     const int gwave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
@@ -453,12 +470,17 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     tx[k] = Trans{vl_cur[k], vl_next[k], vcur[k], vnext[k]};
                 }
                 // Courant number on faces i = 1 .. mx+1 of every swept row
+                // (FP32: max(dt/dx s, -dt/dx s) as dt/dx |s|, dt/dx > 0)
                 if (cv[k] && col[k] >= 2 && col[k] <= NX - 2 && J <= Jlast) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
-                        R a = dtdx * fx[k].s[w];
-                        R b = -dtdx * fx[k].s[w];
-                        cfl[k] = dmax(dmax(cfl[k], a), b);
+                        if constexpr (sizeof(R) == 4) {
+                            cfl[k] = dmax(cfl[k], dtdx * M<R>::abs_(fx[k].s[w]));
+                        } else {
+                            R a = dtdx * fx[k].s[w];
+                            R b = -dtdx * fx[k].s[w];
+                            cfl[k] = dmax(dmax(cfl[k], a), b);
+                        }
                     }
                 }
             }
@@ -485,7 +507,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     for (int k = 0; k < CPL; k++) {
                         R d = R(0);
 #pragma unroll
-                        for (int m = 0; m < ME; m++) d = d + fx[k].w[w][m] * wr[k][m];
+                        for (int m = 0; m < ME; m++)
+                            if (!(SKIPZ && S::template wz<0>(w, m))) d = d + fx[k].w[w][m] * wr[k][m];
                         dotr[k][w] = d;
                     }
                 }
@@ -500,12 +523,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         if (mth[w] != 0 && col[k] >= 2 && col[k] <= NX - 2) {
                             R wn = R(0);
 #pragma unroll
-                            for (int m = 0; m < ME; m++) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
+                            for (int m = 0; m < ME; m++)
+                                if (!(SKIPZ && S::template wz<0>(w, m))) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
                             if (M<R>::lim_ok(wn)) {
                                 R r = M<R>::divl(fx[k].s[w] > R(0) ? dotl : dotr[k][w], wn);
                                 R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
-                                for (int m = 0; m < ME; m++) fx[k].w[w][m] = phi * fx[k].w[w][m];
+                                for (int m = 0; m < ME; m++)
+                                    if (!(SKIPZ && S::template wz<0>(w, m))) fx[k].w[w][m] = phi * fx[k].w[w][m];
                             }
                         }
                     }
@@ -517,6 +542,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R c = R(0);
 #pragma unroll
                         for (int w = 0; w < MW; w++) {
+                            if (SKIPZ && S::template wz<0>(w, m)) continue;
                             R as = M<R>::abs_(fx[k].s[w]);
                             c = c + as * (R(1) - as * dtdx) * fx[k].w[w][m];
                         }
@@ -528,8 +554,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             for (int k = 0; k < CPL; k++)
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
-                    R a = R(0) + fx[k].am[m];
-                    R b = R(0) - fx[k].ap[m];
+                    R a = zadd(fx[k].am[m]);
+                    R b = zsub(fx[k].ap[m]);
                     if (second) {
                         a = a + R(0.5) * cq[k][m];
                         b = b + R(0.5) * cq[k][m];
@@ -547,17 +573,22 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         am2[m] = cqsplit ? fx[k].am[m] + cq[k][m] : fx[k].am[m];
                         ap2[m] = cqsplit ? fx[k].ap[m] - cq[k][m] : fx[k].ap[m];
                     }
-                    S::template rpt2<0>(1, am2, tx[k], pp, bm, bp);
+                    if constexpr (S::RPT2_SCALED) {
+                        S::template rpt2_scaled<0>(am2, tx[k], hdtdx, hmm[k], hpm[k]);
+                        S::template rpt2_scaled<0>(ap2, tx[k], hdtdx, hmp[k], hpp[k]);
+                    } else {
+                        S::template rpt2<0>(1, am2, tx[k], pp, bm, bp);
 #pragma unroll
-                    for (int m = 0; m < ME; m++) {
-                        hmm[k][m] = hdtdx * bm[m];
-                        hpm[k][m] = hdtdx * bp[m];
-                    }
-                    S::template rpt2<0>(2, ap2, tx[k], pp, bm, bp);
+                        for (int m = 0; m < ME; m++) {
+                            hmm[k][m] = hdtdx * bm[m];
+                            hpm[k][m] = hdtdx * bp[m];
+                        }
+                        S::template rpt2<0>(2, ap2, tx[k], pp, bm, bp);
 #pragma unroll
-                    for (int m = 0; m < ME; m++) {
-                        hmp[k][m] = hdtdx * bm[m];
-                        hpp[k][m] = hdtdx * bp[m];
+                        for (int m = 0; m < ME; m++) {
+                            hmp[k][m] = hdtdx * bm[m];
+                            hpp[k][m] = hdtdx * bp[m];
+                        }
                     }
                 } else {
 #pragma unroll
@@ -575,11 +606,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
 #pragma unroll
                 for (int k = 0; k < CPL; k++) {
-                    R d1 = trans ? (R(0) - from_right<CPL>(t1, k)) - hmp[k][m] : R(0);
-                    R d2 = trans ? (R(0) - from_right<CPL>(t2, k)) - hpp[k][m] : R(0);
-                    fmxn[k][m] = R(0) + from_right<CPL>(t3, k);
-                    fpxn[k][m] = R(0) + fap[k][m];
-                    Gn[k][m] = (R(0) + d2p[k][m]) + d1;
+                    R d1 = trans ? zsub(from_right<CPL>(t1, k)) - hmp[k][m] : R(0);
+                    R d2 = trans ? zsub(from_right<CPL>(t2, k)) - hpp[k][m] : R(0);
+                    fmxn[k][m] = zadd(from_right<CPL>(t3, k));
+                    fpxn[k][m] = zadd(fap[k][m]);
+                    Gn[k][m] = zadd(d2p[k][m]) + d1;
                     d2n[k][m] = d2;
                 }
             }
@@ -622,9 +653,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2 && J > J0 && J <= Jlast) {
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
-                        R a = dtdy * fyn[k].s[w];
-                        R b = -dtdy * fyn[k].s[w];
-                        cfl[k] = dmax(dmax(cfl[k], a), b);
+                        if constexpr (sizeof(R) == 4) {
+                            cfl[k] = dmax(cfl[k], dtdy * M<R>::abs_(fyn[k].s[w]));
+                        } else {
+                            R a = dtdy * fyn[k].s[w];
+                            R b = -dtdy * fyn[k].s[w];
+                            cfl[k] = dmax(dmax(cfl[k], a), b);
+                        }
                     }
                 }
             }
@@ -637,7 +672,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 for (int w = 0; w < MW; w++) {
                     R d = R(0);
 #pragma unroll
-                    for (int m = 0; m < ME; m++) d = d + fyp[k].w[w][m] * fyn[k].w[w][m];
+                    for (int m = 0; m < ME; m++)
+                        if (!(SKIPZ && S::template wz<1>(w, m))) d = d + fyp[k].w[w][m] * fyn[k].w[w][m];
                     dotry[k][w] = d;
                 }
         }
@@ -661,12 +697,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         if (mth[w] != 0) {
                             R wn = R(0);
 #pragma unroll
-                            for (int m = 0; m < ME; m++) wn = wn + f.w[w][m] * f.w[w][m];
+                            for (int m = 0; m < ME; m++)
+                                if (!(SKIPZ && S::template wz<1>(w, m))) wn = wn + f.w[w][m] * f.w[w][m];
                             if (M<R>::lim_ok(wn)) {
                                 R r = M<R>::divl(f.s[w] > R(0) ? dotly[k][w] : dotry[k][w], wn);
                                 R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
-                                for (int m = 0; m < ME; m++) f.w[w][m] = phi * f.w[w][m];
+                                for (int m = 0; m < ME; m++)
+                                    if (!(SKIPZ && S::template wz<1>(w, m))) f.w[w][m] = phi * f.w[w][m];
                             }
                         }
                     }
@@ -675,6 +713,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         R c = R(0);
 #pragma unroll
                         for (int w = 0; w < MW; w++) {
+                            if (SKIPZ && S::template wz<1>(w, m)) continue;
                             R as = M<R>::abs_(f.s[w]);
                             c = c + as * (R(1) - as * dtdy) * f.w[w][m];
                         }
@@ -683,8 +722,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
-                    R a = R(0) + f.am[m];
-                    R b = R(0) - f.ap[m];
+                    R a = zadd(f.am[m]);
+                    R b = zsub(f.ap[m]);
                     if (second) {
                         a = a + R(0.5) * cq[m];
                         b = b + R(0.5) * cq[m];
@@ -705,20 +744,28 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         am2[m] = cqsplit ? f.am[m] + cq[m] : f.am[m];
                         ap2[m] = cqsplit ? f.ap[m] - cq[m] : f.ap[m];
                     }
-                    S::template rpt2<1>(1, am2, t, pp, bm, bp);
                     R hmm_y[ME], hpm_y[ME];
+                    if constexpr (S::RPT2_SCALED) {
+                        S::template rpt2_scaled<1>(am2, t, hdtdy, hmm_y, hpm_y);
+                        S::template rpt2_scaled<1>(ap2, t, hdtdy, hmp_new[k], hpp_new[k]);
+                    } else {
+                        S::template rpt2<1>(1, am2, t, pp, bm, bp);
 #pragma unroll
-                    for (int m = 0; m < ME; m++) {
-                        hmm_y[m] = hdtdy * bm[m];
-                        hpm_y[m] = hdtdy * bp[m];
+                        for (int m = 0; m < ME; m++) {
+                            hmm_y[m] = hdtdy * bm[m];
+                            hpm_y[m] = hdtdy * bp[m];
+                        }
+                        S::template rpt2<1>(2, ap2, t, pp, bm, bp);
+#pragma unroll
+                        for (int m = 0; m < ME; m++) {
+                            hmp_new[k][m] = hdtdy * bm[m];
+                            hpp_new[k][m] = hdtdy * bp[m];
+                        }
                     }
-                    S::template rpt2<1>(2, ap2, t, pp, bm, bp);
 #pragma unroll
                     for (int m = 0; m < ME; m++) {
-                        hmp_new[k][m] = hdtdy * bm[m];
-                        hpp_new[k][m] = hdtdy * bp[m];
-                        e1[k][m] = (R(0) - hmm_y[m]) - hmp_prev[k][m];
-                        e2[k][m] = (R(0) - hpm_y[m]) - hpp_prev[k][m];
+                        e1[k][m] = zsub(hmm_y[m]) - hmp_prev[k][m];
+                        e2[k][m] = zsub(hpm_y[m]) - hpp_prev[k][m];
                     }
                 }
             }

@@ -223,9 +223,12 @@ def test_wall_boundaries_conserve_mass(dev):
     assert abs(m1[0] - m0[0]) <= 1e-13 * abs(m0[0])
 
 
-def test_autotune_and_launch_shapes_bitwise(dev):
-    """Every launch shape gives the same FP64 bits; the autotuner keeps one of them."""
-    sim = make(H.SOLVER_SWE_ROE, 16, 2, 3, refine=0.5)
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_autotune_and_launch_shapes_bitwise(dev, precision):
+    """Every launch shape gives the same bits (FP32 too: its step contracts
+    within source expressions only, so every instantiation rounds alike); the
+    autotuner keeps one of them."""
+    sim = make(H.SOLVER_SWE_ROE, 16, 2, 3, refine=0.5, precision=precision)
     sim.initialize("radial_dam", SMALL_DAM, nsub=4)
     q0 = sim.get_q()
     ref = None
