@@ -287,30 +287,66 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const double* swy = nullptr;
     double swvtx = 0.0, swvty = 0.0;
     const double swpi = 4.0 * atan(1.0);
+    // FP32: psi = A(x) B(y) / pi is separable.  Per column A = sin^2(pi xll)
+    // and dA = A(xll + dx) - A, per row (in LDS) B and dB likewise, and the
+    // velocities are u = vt/(pi dy) A dB, v = -vt/(pi dx) B dA: two
+    // multiplications each instead of the FP64 differences of psi, which
+    // FP64 keeps so that its velocities equal the stored field bit for bit.
+    float swa0[CPL], swda[CPL];
+    const float* swyf = nullptr;
+    float swcu = 0.0f, swcv = 0.0f;
     if constexpr (S::MAUX > 0) {
         if (A.swirl) {
             const double x0 = pact ? A.xylower[2 * tix] : 0.0;
             const double y0 = pact ? A.xylower[2 * tix + 1] : 0.0;
+            if constexpr (sizeof(R) == 8) {
 #pragma unroll
-            for (int k = 0; k < CPL; k++) {
-                const double xll = x0 + (double)(col[k] - 2) * dx;
-                swx0[k] = sin(swpi * xll);
-                swx1[k] = sin(swpi * (xll + dx));
+                for (int k = 0; k < CPL; k++) {
+                    const double xll = x0 + (double)(col[k] - 2) * dx;
+                    swx0[k] = sin(swpi * xll);
+                    swx1[k] = sin(swpi * (xll + dx));
+                }
             }
             const size_t stage_bytes = PUSH ? sizeof(R) * (size_t)nstage * A.ppw * pstage : 0;
             double* ys = reinterpret_cast<double*>(hc_smem + ((stage_bytes + 7) & ~(size_t)7)) +
                          ((threadIdx.x >> 6) * A.ppw + (slot < A.ppw ? slot : 0)) * 2 * NY;
-            if (slot < A.ppw) {
-                for (int t = lp; t < 2 * NY; t += A.lpp) {
-                    const int j = t < NY ? t : t - NY;
-                    const double yll = y0 + (double)(j - 2) * dy;
-                    ys[t] = sin(swpi * (t < NY ? yll : yll + dy));
-                }
-            }
-            swy = ys;
             const double vt = A.levels ? (A.par ? A.par[STEP_MAXL + lev] : A.vtl[lev]) : A.vt;
-            swvty = vt / dy;
-            swvtx = vt / dx;
+            if constexpr (sizeof(R) == 4) {
+                // sin(pi z) by the hardware sine, which takes revolutions
+                // (v_sin_f32(z/2)); the corner coordinates are dyadic, so
+                // their halves are exact in FP32
+                auto sinpi = [](double z) { return __builtin_amdgcn_sinf((float)(0.5 * z)); };
+                float* yf = reinterpret_cast<float*>(ys);
+                if (slot < A.ppw) {
+                    for (int j = lp; j < NY; j += A.lpp) {
+                        const double yll = y0 + (double)(j - 2) * dy;
+                        const float s0 = sinpi(yll), s1 = sinpi(yll + dy);
+                        yf[j] = s0 * s0;
+                        yf[NY + j] = s1 * s1 - s0 * s0;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    const double xll = x0 + (double)(col[k] - 2) * dx;
+                    const float s0 = sinpi(xll), s1 = sinpi(xll + dx);
+                    swa0[k] = s0 * s0;
+                    swda[k] = s1 * s1 - s0 * s0;
+                }
+                swyf = yf;
+                swcu = (float)(vt / (swpi * dy));
+                swcv = (float)(-vt / (swpi * dx));
+            } else {
+                if (slot < A.ppw) {
+                    for (int t = lp; t < 2 * NY; t += A.lpp) {
+                        const int j = t < NY ? t : t - NY;
+                        const double yll = y0 + (double)(j - 2) * dy;
+                        ys[t] = sin(swpi * (t < NY ? yll : yll + dy));
+                    }
+                }
+                swy = ys;
+                swvty = vt / dy;
+                swvtx = vt / dx;
+            }
             __syncthreads();
         }
     }
@@ -367,7 +403,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             for (int m = 0; m < ME; m++)
                 qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
             if constexpr (S::MAUX > 0) {
-                if (A.swirl) {
+                if (A.swirl && sizeof(R) == 4) {
+                    const float b0 = rowin ? swyf[Jo] : 0.0f, db = rowin ? swyf[NY + Jo] : 0.0f;
+                    const float bn = J + 1 < NY ? swyf[Jo + 1] : 0.0f;
+                    ucur[k] = (cv[k] && rowin) ? R(swcu * (swa0[k] * db)) : R(0);
+                    vcur[k] = (cv[k] && rowin) ? R(swcv * (b0 * swda[k])) : R(0);
+                    vnext[k] = (cv[k] && J + 1 < NY) ? R(swcv * (bn * swda[k])) : R(0);
+                } else if (A.swirl) {
                     // velocities of (c, J) and the v of (c, J+1), as k_swirl_aux
                     ucur[k] = vcur[k] = vnext[k] = R(0);
                     const double a0 = swx0[k] * swx0[k], a1 = swx1[k] * swx1[k];

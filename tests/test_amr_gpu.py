@@ -131,7 +131,9 @@ def test_euler_brick_bitwise(dev, solver):
 def test_swirl_fused_velocities_bitwise(dev, precision, merge):
     """The step kernel's own swirl velocities (HIPCLAW_FEATURE_FUSE_AUX) give
     the bits of the velocity-field kernel followed by the step, on a
-    subcycled two-level mesh with non-square cells."""
+    subcycled two-level mesh with non-square cells (FP64).  FP32 evaluates
+    the separable stream function with FP32 factors instead of rounding the
+    FP64 field: there the two agree to FP32 rounding (relative L1 1e-6)."""
     runs = []
     for fuse in (False, True):
         sim = make(H.SOLVER_ADV, 16, 2, 3, precision=precision, box=(0.0, 1.0, 0.0, 0.75))
@@ -144,8 +146,13 @@ def test_swirl_fused_velocities_bitwise(dev, precision, merge):
         runs.append((sim.get_q(), cfl, sim.num_patches()))
         sim.close()
     assert runs[0][2] == runs[1][2] and runs[0][2] > 16
-    assert runs[0][1] == runs[1][1]
-    assert np.array_equal(interiors(runs[0][0]), interiors(runs[1][0]))
+    if precision == H.FP64:
+        assert runs[0][1] == runs[1][1]
+        assert np.array_equal(interiors(runs[0][0]), interiors(runs[1][0]))
+    else:
+        np.testing.assert_allclose(runs[1][1], runs[0][1], rtol=1e-5)
+        a, b = interiors(runs[1][0]), interiors(runs[0][0])
+        assert np.abs(a - b).sum() <= 1e-6 * np.abs(b).sum()
 
 
 def test_swirl_advection_device_velocities(dev):
