@@ -56,6 +56,52 @@ struct TsOp {
     int nbr[4][2];      // same size: [0]; finer: igrid 0, 1
 };
 
+// Several operation lists in one launch: list s holds the operations
+// [start[s], start[s+1]) of the launch (one list per level of a ghost-fill
+// phase or a time sync, whose levels are independent of each other).
+#if defined(__HIPCC__)
+#define HC_HD __host__ __device__
+#else
+#define HC_HD
+#endif
+constexpr int SEG_MAX = MAXL + 1;
+template <class OP, int CAP = SEG_MAX> struct Segs {
+    const OP* ops[CAP];
+    int start[CAP + 1];
+    int n;
+    bool ok;            // false once a list did not fit (the launch is refused)
+    HC_HD Segs() : n(0), ok(true) { start[0] = 0; }
+    // host: append a list (empty lists are skipped)
+    void add(const OP* p, int cnt)
+    {
+        if (cnt <= 0 || !p) return;
+        if (n >= CAP) { ok = false; return; }
+        ops[n] = p;
+        start[n + 1] = start[n] + cnt;
+        n++;
+    }
+    HC_HD int total() const { return start[n]; }
+#ifdef __HIPCC__
+    // device: list holding operation o of the launch (o < total())
+    __device__ int seg(int o) const
+    {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (o >= start[mid]) lo = mid; else hi = mid - 1;
+        }
+        return lo;
+    }
+    __device__ const OP& at(int o) const
+    {
+        const int s = seg(o);
+        return ops[s][o - start[s]];
+    }
+#endif
+};
+// register resets of a time sync: three face lists per level
+using ResetSegs = Segs<int, 3 * MAXL>;
+
 struct XferOp {         // regrid transfer into the new pools
     int kind;           // 0 keep (copy), 1 interpolate2fine, 2 average2coarse
     int dst;            // new slot

@@ -43,17 +43,16 @@ template <class R> __device__ inline R amr_slope(R sl, R sr)
 // (interpolation).  All meqn fields are handled by the thread.
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void k_ghost(const GhostOp* __restrict__ ops, int nops, Pools<R> P, int mx, int my,
-                        int meqn)
+__global__ void k_ghost(const Segs<GhostOp> SG, Pools<R> P, int mx, int my, int meqn)
 {
     const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
     const long long T = (long long)NX * NY;
     const int items = MBC * (mx > my ? mx : my);
     const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (gid >= (long long)nops * items) return;
+    if (gid >= (long long)SG.total() * items) return;
     const int o = (int)(gid / items);
     const int k = (int)(gid - (long long)o * items);
-    const GhostOp op = ops[o];
+    const GhostOp op = SG.at(o);
     R* __restrict__ D = pool_ptr(P, op.dlev, op.flags & 1) + (long long)op.dst * T * meqn;
     const R* __restrict__ S = pool_ptr(P, op.slev, op.flags & 2) + (long long)op.src * T * meqn;
 #define AT(B, i, j, m) B[((long long)(m) * NY + (j)) * NX + (i)]
@@ -148,12 +147,11 @@ __global__ void k_ghost(const GhostOp* __restrict__ ops, int nops, Pools<R> P, i
 // height, then bottom/top over the full width).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void k_bc_list(const BcOp* __restrict__ ops, int nops, Pools<R> P, int mx, int my,
-                          int meqn)
+__global__ void k_bc_list(const Segs<BcOp> S, Pools<R> P, int mx, int my, int meqn)
 {
     const int o = blockIdx.x;
-    if (o >= nops) return;
-    const BcOp op = ops[o];
+    if (o >= S.total()) return;
+    const BcOp op = S.at(o);
     const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
     R* qp = pool_ptr(P, op.level, op.flags & 1) + (long long)op.slot * NX * NY * meqn;
     const int i1 = MBC, imx = mx + MBC - 1, j1 = MBC, jmy = my + MBC - 1;
@@ -653,19 +651,35 @@ __global__ void k_copy_tiles(const int* __restrict__ slots, int n, const R* __re
 static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <class R>
+void launch_ghost(const Segs<GhostOp>& S, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
+{
+    if (S.total() <= 0) return;
+    const long long n = (long long)S.total() * MBC * (mx > my ? mx : my);
+    hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, S, P, mx, my, meqn);
+}
+
+template <class R>
 void launch_ghost(const GhostOp* ops, int nops, const Pools<R>& P, int mx, int my, int meqn,
                   hipStream_t s)
 {
-    if (nops <= 0) return;
-    const long long n = (long long)nops * MBC * (mx > my ? mx : my);
-    hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, ops, nops, P, mx, my, meqn);
+    Segs<GhostOp> S;
+    S.add(ops, nops);
+    launch_ghost<R>(S, P, mx, my, meqn, s);
+}
+
+template <class R>
+void launch_bc(const Segs<BcOp>& S, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
+{
+    if (S.total() <= 0) return;
+    hipLaunchKernelGGL(k_bc_list<R>, dim3(S.total()), dim3(64), 0, s, S, P, mx, my, meqn);
 }
 
 template <class R>
 void launch_bc(const BcOp* ops, int nops, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
 {
-    if (nops <= 0) return;
-    hipLaunchKernelGGL(k_bc_list<R>, dim3(nops), dim3(64), 0, s, ops, nops, P, mx, my, meqn);
+    Segs<BcOp> S;
+    S.add(ops, nops);
+    launch_bc<R>(S, P, mx, my, meqn, s);
 }
 
 template <class R>
@@ -736,6 +750,8 @@ void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long 
 #define HC_INST(R)                                                                                \
     template void launch_ghost<R>(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t); \
     template void launch_bc<R>(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);       \
+    template void launch_ghost<R>(const Segs<GhostOp>&, const Pools<R>&, int, int, int, hipStream_t); \
+    template void launch_bc<R>(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);       \
     template void launch_timeinterp<R>(const TiOp*, int, const Pools<R>&, int, int, int, double,     \
                                        hipStream_t);                                              \
     template void launch_xfer<R>(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t); \

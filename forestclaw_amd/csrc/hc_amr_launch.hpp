@@ -22,6 +22,9 @@ struct QinitParams {
 
 template <class R> void launch_ghost(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t);
 template <class R> void launch_bc(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);
+// the same over several lists in one launch
+template <class R> void launch_ghost(const Segs<GhostOp>&, const Pools<R>&, int, int, int, hipStream_t);
+template <class R> void launch_bc(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);
 template <class R>
 void launch_timeinterp(const TiOp*, int, const Pools<R>&, int, int, int, double, hipStream_t);
 template <class R> void launch_xfer(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t);
@@ -42,6 +45,11 @@ template <class R>
 void launch_ts_correct(const TsOp*, int, R*, const double*, long long, const double*, int, int, int,
                        hipStream_t);
 void launch_ts_reset(const int*, int, double*, long long, int, int, int, hipStream_t);
+// several levels in one launch: list s corrects patches whose q is qs[s]
+template <class R>
+void launch_ts_correct(const Segs<TsOp>&, R* const* qs, const double*, long long, const double*, int, int,
+                       int, hipStream_t);
+void launch_ts_reset(const ResetSegs&, double*, long long, int, int, int, hipStream_t);
 // Network input tiles of n patches (bf16 [n][36][36][8]); returns HIPCLAW_*.
 template <class R>
 int launch_make_tiles(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double,

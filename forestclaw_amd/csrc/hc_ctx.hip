@@ -346,6 +346,34 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
     auto run = [&](auto tag) -> int {
         using R = decltype(tag);
         const Pools<R> P = pools<R>();
+        if (merge_levels) {
+            // one launch per phase: within the average, copy and BC phases
+            // every operation writes ghost cells of its own patch from
+            // interior cells only, so the levels of a phase are independent;
+            // interpolation reads coarse ghost cells that the interpolation
+            // of the next coarser level writes and keeps its level order
+            Segs<GhostOp> avg, copy;
+            Segs<BcOp> bcc, bca;
+            for (int L = maxcoarse; L >= mincoarse; L--) avg.add(plan[L].avg.p, plan[L].avg.n);
+            if (use_ti) avg.add(plan[tl].avg_ti.p, plan[tl].avg_ti.n);
+            for (int L = maxlevel; L >= minlevel; L--)
+                if (!pushed[L]) copy.add(plan[L].copy.p, plan[L].copy.n);
+            if (use_ti) copy.add(plan[tl].copy_ti.p, plan[tl].copy_ti.n);
+            for (int L = maxcoarse; L >= mincoarse; L--) bcc.add(plan[L].bc.p, plan[L].bc.n);
+            if (use_ti) bcc.add(plan[tl].bc_ti.p, plan[tl].bc_ti.n);
+            for (int L = maxlevel; L >= minlevel; L--) bca.add(plan[L].bc.p, plan[L].bc.n);
+            if (use_ti) bca.add(plan[tl].bc_ti.p, plan[tl].bc_ti.n);
+            if (!avg.ok || !copy.ok || !bcc.ok || !bca.ok)
+                return set_error(HIPCLAW_ERR_UNSUPPORTED, "ghost update: too many operation lists");
+            launch_ghost<R>(avg, P, mx, my, meqn, stream);
+            launch_ghost<R>(copy, P, mx, my, meqn, stream);
+            launch_bc<R>(bcc, P, mx, my, meqn, stream);
+            for (int L = maxcoarse; L >= mincoarse; L--)
+                launch_ghost<R>(plan[L].interp.p, plan[L].interp.n, P, mx, my, meqn, stream);
+            if (use_ti) launch_ghost<R>(plan[tl].interp_ti.p, plan[tl].interp_ti.n, P, mx, my, meqn, stream);
+            launch_bc<R>(bca, P, mx, my, meqn, stream);
+            return check_hip(hipGetLastError(), "ghost update");
+        }
         for (int L = maxcoarse; L >= mincoarse; L--)
             launch_ghost<R>(plan[L].avg.p, plan[L].avg.n, P, mx, my, meqn, stream);
         if (use_ti) launch_ghost<R>(plan[tl].avg_ti.p, plan[tl].avg_ti.n, P, mx, my, meqn, stream);
@@ -556,6 +584,31 @@ int Ctx::time_sync(int minlevel, int maxlevel)
     if (minlevel < 0 || maxlevel >= MAXL || minlevel > maxlevel)
         return set_error(HIPCLAW_ERR_ARG, "time_sync: bad level range");
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
+    if (merge_levels) {
+        // one correction launch over the levels (each operation adds to the
+        // interior of its own patch and only reads registers) and one reset
+        // launch after it
+        Segs<TsOp> cs;
+        ResetSegs rs;
+        void* qs[SEG_MAX];
+        for (int L = minlevel; L <= maxlevel; L++) {
+            if (plan[L].ts.n <= 0) continue;
+            qs[cs.n] = esz == 8 ? (void*)pools<double>().q[L] : (void*)pools<float>().q[L];
+            cs.add(plan[L].ts.p, plan[L].ts.n);
+        }
+        for (int L = minlevel; L <= maxlevel; L++) {
+            rs.add(plan[L].ts_same.p, plan[L].ts_same.n);
+            if (L < maxlevel) rs.add(plan[L].ts_f2c.p, plan[L].ts_f2c.n);
+            rs.add(plan[L].ts_phys.p, plan[L].ts_phys.n);
+        }
+        if (!cs.ok || !rs.ok) return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync: too many face lists");
+        if (esz == 8)
+            launch_ts_correct<double>(cs, (double* const*)qs, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
+        else
+            launch_ts_correct<float>(cs, (float* const*)qs, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
+        launch_ts_reset(rs, d_reg, regstride, mx, my, meqn, stream);
+        return check_hip(hipGetLastError(), "time sync");
+    }
     for (int L = minlevel; L <= maxlevel; L++) {
         const LevelPlan& lp = plan[L];
         if (esz == 8)

@@ -54,17 +54,23 @@ __device__ inline double ts_delta_f2c(const double* rc, const double* rf, int mx
     return (R(rc, 3, 2, k) - P(2, 3)) - (P(2, 1) - R(rc, 3, 1, k));
 }
 
+// q pool of each list of a launch (one per level)
+template <class R> struct QList {
+    R* q[SEG_MAX];
+};
+
 // One workgroup (one wave) per patch.  LDS: add[f][g][m][k], f < 4, g < 2.
 template <class R>
 __global__ void __launch_bounds__(64)
-k_ts_correct(const TsOp* __restrict__ ops, int nops, R* __restrict__ q, const double* __restrict__ reg,
-             long long regstride, const double* __restrict__ dxdy, int mx, int my, int meqn)
+k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ reg, long long regstride,
+             const double* __restrict__ dxdy, int mx, int my, int meqn)
 {
 #pragma clang fp contract(off)
     extern __shared__ double add[];
     const int op = blockIdx.x;
-    if (op >= nops) return;
-    const TsOp o = ops[op];
+    if (op >= S.total()) return;
+    const TsOp o = S.at(op);
+    R* __restrict__ q = qs.q[S.seg(op)];
     const int maxlen = mx > my ? mx : my;
     const int NX = mx + 4, NY = my + 4;
     const double area = dxdy[2 * o.slot] * dxdy[2 * o.slot + 1];
@@ -117,40 +123,64 @@ k_ts_correct(const TsOp* __restrict__ ops, int nops, R* __restrict__ q, const do
 }
 
 // Zero the four register blocks of listed faces (entry = slot*4 + face).
-__global__ void k_ts_reset(const int* __restrict__ faces, int n, double* __restrict__ reg,
-                           long long regstride, int mx, int my, int meqn)
+__global__ void k_ts_reset(const ResetSegs S, double* __restrict__ reg, long long regstride, int mx, int my,
+                           int meqn)
 {
     const int maxblk = 4 * meqn * (mx > my ? mx : my);
-    const long long tot = (long long)n * maxblk;
+    const long long tot = (long long)S.total() * maxblk;
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
          t += (long long)gridDim.x * blockDim.x) {
         const int e = (int)(t / maxblk), r = (int)(t - (long long)e * maxblk);
-        const int slot = faces[e] >> 2, f = faces[e] & 3;
+        const int fe = S.at(e);
+        const int slot = fe >> 2, f = fe & 3;
         const int len = f < 2 ? my : mx;
         if (r < 4 * meqn * len) reg[(long long)slot * regstride + reg_off(mx, my, meqn, f, 0) + r] = 0.0;
     }
 }
 
 template <class R>
+void launch_ts_correct(const Segs<TsOp>& S, R* const* qs, const double* reg, long long regstride,
+                       const double* dxdy, int mx, int my, int meqn, hipStream_t s)
+{
+    if (S.total() <= 0) return;
+    QList<R> ql{};
+    for (int k = 0; k < S.n; k++) ql.q[k] = qs[k];
+    const size_t lds = sizeof(double) * 8 * meqn * (mx > my ? mx : my);
+    hipLaunchKernelGGL(k_ts_correct<R>, dim3(S.total()), dim3(64), lds, s, S, ql, reg, regstride, dxdy, mx,
+                       my, meqn);
+}
+
+template <class R>
 void launch_ts_correct(const TsOp* ops, int nops, R* q, const double* reg, long long regstride,
                        const double* dxdy, int mx, int my, int meqn, hipStream_t s)
 {
-    if (nops <= 0) return;
-    const size_t lds = sizeof(double) * 8 * meqn * (mx > my ? mx : my);
-    hipLaunchKernelGGL(k_ts_correct<R>, dim3(nops), dim3(64), lds, s, ops, nops, q, reg, regstride,
-                       dxdy, mx, my, meqn);
+    Segs<TsOp> S;
+    S.add(ops, nops);
+    launch_ts_correct<R>(S, &q, reg, regstride, dxdy, mx, my, meqn, s);
+}
+
+void launch_ts_reset(const ResetSegs& S, double* reg, long long regstride, int mx, int my, int meqn,
+                     hipStream_t s)
+{
+    if (S.total() <= 0) return;
+    const long long tot = (long long)S.total() * 4 * meqn * (mx > my ? mx : my);
+    const long long nb = (tot + 255) / 256;
+    hipLaunchKernelGGL(k_ts_reset, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, s, S, reg,
+                       regstride, mx, my, meqn);
 }
 
 void launch_ts_reset(const int* faces, int n, double* reg, long long regstride, int mx, int my,
                      int meqn, hipStream_t s)
 {
-    if (n <= 0) return;
-    const long long tot = (long long)n * 4 * meqn * (mx > my ? mx : my);
-    const long long nb = (tot + 255) / 256;
-    hipLaunchKernelGGL(k_ts_reset, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, s, faces,
-                       n, reg, regstride, mx, my, meqn);
+    ResetSegs S;
+    S.add(faces, n);
+    launch_ts_reset(S, reg, regstride, mx, my, meqn, s);
 }
 
+template void launch_ts_correct<double>(const Segs<TsOp>&, double* const*, const double*, long long,
+                                        const double*, int, int, int, hipStream_t);
+template void launch_ts_correct<float>(const Segs<TsOp>&, float* const*, const double*, long long,
+                                       const double*, int, int, int, hipStream_t);
 template void launch_ts_correct<double>(const TsOp*, int, double*, const double*, long long,
                                         const double*, int, int, int, hipStream_t);
 template void launch_ts_correct<float>(const TsOp*, int, float*, const double*, long long,
