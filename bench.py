@@ -281,8 +281,8 @@ BF16_PEAK = 2.5e15
 def small_configs(args):
     """The other configurations (forestclaw_amd.configs C1 swirl, C2 radial
     dam, C3 shock-bubble) on their meshes after the initial refinement, no
-    regrid: FP32 batched (merged launches, ghost push, graph replays where
-    allowed) against FP64 plain (one launch per level step, eager), each
+    regrid: FP32 batched (the library defaults: merged launches, ghost push,
+    graph replays of subcycled coarse steps) against FP64 plain (one launch per level step, eager), each
     timed over its stated run length after two warm-up steps."""
     import torch
     from forestclaw_amd import configs as CF
@@ -295,9 +295,10 @@ def small_configs(args):
         res = {}
         for label, prec, plain in (("fp32_batched", H.FP32, False), ("fp64_plain", H.FP64, True)):
             sim, dt = CF.build(name, prec, stream=_STREAM[0].cuda_stream)
-            sim.set_feature(H.FEATURE_GRAPH, not plain)
-            sim.set_feature(H.FEATURE_GHOST_PUSH, not plain)
-            sim.set_feature(H.FEATURE_MERGE_LEVELS, not plain)
+            if plain:  # batched keeps the defaults (graphs for subcycled steps)
+                sim.set_feature(H.FEATURE_GRAPH, False)
+                sim.set_feature(H.FEATURE_GHOST_PUSH, False)
+                sim.set_feature(H.FEATURE_MERGE_LEVELS, False)
             t = 0.0
             for s in range(2):
                 sim.advance(t, dt)

@@ -509,7 +509,7 @@ int Ctx::autotune(int level, int reps, hipclaw_launch* best, float* ms)
     // on, the slot-addressed push kernel, whose output pool is the scratch
     // tiles (same-level neighbours are slots of the same level, so every
     // push lands in the scratch too).
-    const bool push = push_ghosts && merge_levels && !tsync && phys.mx >= 4 && phys.my >= 4;
+    const bool push = use_push() && merge_levels;
     MultiLevel ml{};
     ml.levels = d_levels;
     ml.slot0 = s0;
@@ -682,7 +682,7 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         HC_TRY(check_hip(hipEventRecord(e0, stream), "event"));
     }
     const hipclaw_launch* lc = launch.cols_per_lane > 0 ? &launch : nullptr;
-    const bool push = push_ghosts && !tsync && !src2 && phys.mx >= 4 && phys.my >= 4;
+    const bool push = use_push();
     ml.push = push ? d_push : nullptr;
     ml.par = par_dev;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
@@ -747,7 +747,14 @@ static unsigned level_bits(const bool* f)
 // by the pool parities at entry and the mesh.
 int Ctx::advance(double t, double dt, double* maxcfl)
 {
+    // fine steps of the finest level per coarse step (the step counters of
+    // advance_body); a coarse step of one fine step is a handful of launches,
+    // for which launching a graph costs more than it saves
+    const int flo = forest.global_minlevel(), fhi = forest.global_maxlevel();
+    const int fine_steps = dom.subcycle ? 1 << (fhi - std::min(dom.minlevel, flo))
+                                        : 1 << std::max(dom.maxlevel - dom.minlevel, fhi - flo);
     const bool graph_ok = use_graph && stream != nullptr && !prof && merge_levels &&
+                          (!graph_auto || fine_steps > 1) &&
                           !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV &&
                             !fused_swirl());
     if (!graph_ok) {
@@ -1198,7 +1205,7 @@ int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, v
     }
     h->c->stream = static_cast<hipStream_t>(stream);
     // whole coarse steps replay as graphs where the stream can be captured
-    h->c->use_graph = h->c->stream != nullptr;
+    h->c->use_graph = h->c->graph_auto = h->c->stream != nullptr;
     *out = h;
     return HIPCLAW_OK;
 }
@@ -1293,9 +1300,10 @@ int hipclaw_set_feature(hipclaw_ctx* ctx, int feature, int on)
     case HIPCLAW_FEATURE_MERGE_LEVELS: c.merge_levels = on != 0; break;
     case HIPCLAW_FEATURE_GHOST_PUSH:
         c.push_ghosts = on != 0;
+        c.push_auto = false;
         c.clear_pushed();
         break;
-    case HIPCLAW_FEATURE_GRAPH: c.use_graph = on != 0; break;
+    case HIPCLAW_FEATURE_GRAPH: c.use_graph = on != 0; c.graph_auto = false; break;
     case HIPCLAW_FEATURE_INIT_GHOSTCELL: c.init_ghostcell = on != 0; break;
     case HIPCLAW_FEATURE_FUSE_AUX: c.fuse_aux = on != 0; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown feature");

@@ -113,6 +113,14 @@ struct Ctx {
     // same-level ghost push from the step kernel (off with time sync, whose
     // corrections change edge cells between a step and the next ghost fill)
     bool push_ghosts = true;
+    // default: push only for patches of at least 32x32 cells; below that the
+    // edge staging costs the step kernel more than the copy launch it saves
+    bool push_auto = true;
+    bool use_push() const
+    {
+        return push_ghosts && !tsync && !src2 && phys.mx >= 4 && phys.my >= 4 &&
+               (!push_auto || (phys.mx >= 32 && phys.my >= 32));
+    }
     int* d_push = nullptr;          // 8 ints per slot
     bool pushed[MAXL] = {};         // level data unchanged since a pushing step
     void clear_pushed()
@@ -125,6 +133,7 @@ struct Ctx {
     // hipGraph of a whole coarse step, replayed when dt, the level pool
     // parities, the ghost-push state, the launch shape and the mesh match
     bool use_graph = false;
+    bool graph_auto = false;  // default: graphs only for coarse steps of several fine steps
     long long mesh_epoch = 0;
     struct GraphEntry {
         unsigned parity_in, pushed_in;

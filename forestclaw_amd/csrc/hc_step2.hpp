@@ -974,57 +974,65 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     }
 
     // ---- same-level ghost push --------------------------------------------
-    // After the sweep every lane of the wave (of the workgroup, with row
-    // bands) stores a share of the staged edge cells of the wave's patches
-    // into the ghost cells of their same-level neighbours across faces and
-    // corners.
+    // After the sweep the lanes of the wave (of the workgroup, with row
+    // bands) store the staged edge cells of the wave's patches into the
+    // ghost cells of their same-level neighbours across faces and corners,
+    // one staged value per lane and iteration over the patch's 4*meqn*(mx+my)
+    // values.  x-edge values go with the edge column fastest (lane pairs
+    // store 8 contiguous bytes), then the row; y-edge values with the column
+    // fastest (contiguous stores).
     if constexpr (PUSH) {
         __syncthreads();  // every thread of the block reaches this point
         const int tl = BAND ? (int)threadIdx.x : lane;    // pushing thread
         const int tn = BAND ? (int)blockDim.x : HC_WAVE;  // pushing threads
+        const int nxi = 4 * ME * A.my;                    // x-edge values per patch
+        const float rmy = 1.0f / (float)A.my, rmx = 1.0f / (float)A.mx;
+        // a / b for 0 <= a < 2^20 from a float reciprocal (one correction)
+        auto qdiv = [](int a, int b, float rb) {
+            int q = (int)((float)a * rb);
+            if (q * b > a) q--;
+            else if ((q + 1) * b <= a) q++;
+            return q;
+        };
         for (int sp = 0; sp < A.ppw; sp++) {
             const int pi = (BAND ? (int)blockIdx.x : gwave) * A.ppw + sp;
             if (pi >= A.npatch || pi >= A.ntiles) break;  // uniform across the wave
             const int st = A.slot0 + pi;
+            // the patch's neighbour table, loaded once (uniform across the wave)
             const int* tb = A.push + 8 * st;
+            int nt[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) nt[k] = tb[k];
             R* ob = static_cast<R*>(A.qout_l[A.levels[st]]);
             const R* sg = reinterpret_cast<const R*>(hc_smem) + (swave * A.ppw + sp) * pstage;
             auto put = [&](int nb, int cc, int rr, int m, R v) {
                 ob[(long long)nb * A.qstride + ((long long)m * NY + rr) * NX + cc] = v;
             };
-            const int nx_items = 4 * ME * A.my;
-            // x edges: a thread takes one row of a tn/2-row block and one of
-            // the two edge columns (thread pairs store 8 contiguous bytes)
-            const int cb = tl & 1, rl = tl >> 1;
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                const int nb = tb[side], nbl = tb[4 + side], nbh = tb[6 + side];
-                const int c = (side ? NX - 4 - A.mx : 2 + A.mx) + cb;
-#pragma unroll
-                for (int m = 0; m < ME; m++) {
-                    const R* sx = sg + ((side * 2 + cb) * ME + m) * A.my;
-                    for (int r = rl; r < A.my; r += tn / 2) {
-                        const int Rw = r + 2;
-                        const R v = sx[r];
-                        if (nb >= 0) put(nb, c, Rw, m, v);
-                        if (Rw <= 3 && nbl >= 0) put(nbl, c, Rw + A.my, m, v);
-                        if (Rw >= NY - 4 && nbh >= 0) put(nbh, c, Rw - A.my, m, v);
-                    }
-                }
-            }
-            // y edges: a thread takes one column of a ghost row
-#pragma unroll
-            for (int side = 0; side < 2; side++) {
-                const int nb = tb[2 + side];
-                if (nb < 0) continue;  // uniform across the wave
-#pragma unroll
-                for (int rr = 0; rr < 2; rr++) {
+            for (int e = tl; e < pstage; e += tn) {
+                if (e < nxi) {
+                    // x edges: e = ((side*ME + m)*my + r)*2 + cb
+                    const int cb = e & 1, h = e >> 1;
+                    const int g = qdiv(h, A.my, rmy);
+                    const int r = h - g * A.my;
+                    const int side = g / ME, m = g - side * ME;
+                    const R v = sg[((side * 2 + cb) * ME + m) * A.my + r];
+                    const int nb = side ? nt[1] : nt[0];
+                    const int nbl = side ? nt[5] : nt[4], nbh = side ? nt[7] : nt[6];
+                    const int c = (side ? NX - 4 - A.mx : 2 + A.mx) + cb;
+                    const int Rw = r + 2;
+                    if (nb >= 0) put(nb, c, Rw, m, v);
+                    if (Rw <= 3 && nbl >= 0) put(nbl, c, Rw + A.my, m, v);
+                    if (Rw >= NY - 4 && nbh >= 0) put(nbh, c, Rw - A.my, m, v);
+                } else {
+                    // y edges: e - nxi = ((side*2 + rr)*ME + m)*mx + i
+                    const int h = e - nxi;
+                    const int g = qdiv(h, A.mx, rmx);
+                    const int i = h - g * A.mx;
+                    const int sr = g / ME, m = g - sr * ME;
+                    const int side = sr >> 1, rr = sr & 1;
+                    const int nb = side ? nt[3] : nt[2];
                     const int Rt = side ? NY - 4 + rr - A.my : 2 + rr + A.my;
-#pragma unroll
-                    for (int m = 0; m < ME; m++) {
-                        const R* sy = sg + nx_items + ((side * 2 + rr) * ME + m) * A.mx;
-                        for (int i = tl; i < A.mx; i += tn) put(nb, i + 2, Rt, m, sy[i]);
-                    }
+                    if (nb >= 0) put(nb, i + 2, Rt, m, sg[h + nxi]);
                 }
             }
         }

@@ -292,20 +292,28 @@ int hipclaw_set_src2(hipclaw_ctx* ctx, int kind);
 int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel);
 /* Copy of all registers (num_patches * 8*meqn*(mx+my) doubles, slot order). */
 int hipclaw_get_registers(hipclaw_ctx* ctx, double* host);
-/* Execution features of hipclaw_advance (both on by default; results are
- * bitwise the same either way):
- *   MERGE_LEVELS  one patch-step launch for all levels due at a subcycled time
+/* Execution features of hipclaw_advance (results are bitwise the same
+ * either way):
+ *   MERGE_LEVELS  (default on) one patch-step launch for all levels due at a
+ *                 subcycled time, and one launch per ghost-fill phase and per
+ *                 time-sync phase for all levels
  *   GHOST_PUSH    the step writes updated edge cells into the ghost cells of
  *                 same-level neighbours, replacing the same-level copies of the
- *                 next ghost fill (not used while time sync is on)
- *   GRAPH         (on by default when the context's stream is not the null
- *                 stream) each coarse step is captured once as a hipGraph and
+ *                 next ghost fill (not used while time sync is on or with a
+ *                 source term).  Default: on for patches of at least 32x32
+ *                 cells, off below (the staging costs more than the copy it
+ *                 saves); setting the feature applies it to every patch size
+ *   GRAPH         each coarse step is captured once as a hipGraph and
  *                 replayed while the pool parities, the mesh and the features
  *                 are unchanged, for every t and dt: the step launches read dt
  *                 and the swirl time factor from a device parameter block
  *                 uploaded before each replay.  Not used while profiling,
  *                 with MERGE_LEVELS off, or with the swirl field stored in
- *                 aux (FUSE_AUX off). */
+ *                 aux (FUSE_AUX off).  Default when the context's stream is
+ *                 not the null stream: graphs for coarse steps of more than
+ *                 one fine step (subcycled or multi-rate schedules), eager
+ *                 launches otherwise; setting the feature on or off applies
+ *                 it to every coarse step. */
 #define HIPCLAW_FEATURE_MERGE_LEVELS 1
 #define HIPCLAW_FEATURE_GHOST_PUSH 2
 #define HIPCLAW_FEATURE_GRAPH 3

@@ -336,6 +336,7 @@ def test_graph_replay_matches_eager(dev, ts):
         sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
         sim.initialize("radial_dam", SMALL_DAM, nsub=4)
         sim.set_feature(H.FEATURE_GRAPH, graph)
+        sim.set_feature(H.FEATURE_GHOST_PUSH, True)  # default only from 32x32 patches
         sims.append(sim)
     side = torch.ones(1 << 20, device=dev)
     for s in range(7):
