@@ -101,6 +101,11 @@ template <class OP, int CAP = SEG_MAX> struct Segs {
 };
 // register resets of a time sync: three face lists per level
 using ResetSegs = Segs<int, 3 * MAXL>;
+// time interpolations run in the average launch of a ghost fill: the weight
+// of each list (one list per interpolated level)
+template <class R> struct TiWeights {
+    R a[SEG_MAX];
+};
 
 struct XferOp {         // regrid transfer into the new pools
     int kind;           // 0 keep (copy), 1 interpolate2fine, 2 average2coarse

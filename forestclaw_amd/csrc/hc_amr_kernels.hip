@@ -42,14 +42,62 @@ template <class R> __device__ inline R amr_slope(R sl, R sr)
 // (copy / average) or one coarse cell whose four children are written
 // (interpolation).  All meqn fields are handled by the thread.
 // ---------------------------------------------------------------------------
+__host__ __device__ inline int ti_ring_size(int mx, int my)
+{
+    return (mx < 2 * MBC || my < 2 * MBC) ? mx * my : 2 * MBC * mx + (my - 2 * MBC) * 2 * MBC;
+}
+
+// Time interpolation of one cell of the ring of interior cells within MBC of
+// the patch edge (the cells that ghost fills read), item k of the ring:
+// rows jj < MBC and jj >= my - MBC first, then the side columns.
 template <class R>
-__global__ void k_ghost(const Segs<GhostOp> SG, Pools<R> P, int mx, int my, int meqn)
+__device__ inline void ti_ring_cell(const TiOp& op, int k, const Pools<R>& P, int mx, int my, int meqn, R alpha)
+{
+    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
+    const int nrow = 2 * MBC * mx;
+    int ii, jj;
+    if (mx < 2 * MBC || my < 2 * MBC) {  // every cell is within MBC of an edge
+        jj = k / mx;
+        ii = k - jj * mx;
+    } else if (k < nrow) {
+        const int r = k / mx;
+        ii = k - r * mx;
+        jj = r < MBC ? r : my - 2 * MBC + r;
+    } else {
+        const int k2 = k - nrow;
+        const int r = k2 / (2 * MBC), cc = k2 - r * (2 * MBC);
+        jj = MBC + r;
+        ii = cc < MBC ? cc : mx - 2 * MBC + cc;
+    }
+    const long long base = (long long)op.slot * NX * NY * meqn;
+    const R* qc = P.q[op.level] + base;
+    const R* ql = P.qlast[op.level] + base;
+    R* qi = P.ti + base;
+    for (int m = 0; m < meqn; m++) {
+        const long long idx = ((long long)m * NY + jj + MBC) * NX + ii + MBC;
+        qi[idx] = ql[idx] + alpha * (qc[idx] - ql[idx]);
+    }
+}
+
+template <class R>
+__global__ void k_ghost(const Segs<GhostOp> SG, const Segs<TiOp> ST, const TiWeights<R> TW, Pools<R> P, int mx,
+                        int my, int meqn)
 {
     const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
     const long long T = (long long)NX * NY;
     const int items = MBC * (mx > my ? mx : my);
     const long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (gid >= (long long)SG.total() * items) return;
+    const long long ng = (long long)SG.total() * items;
+    if (gid >= ng) {
+        // time interpolations that run beside the ghost operations
+        const int ring = ti_ring_size(mx, my);
+        const long long t = gid - ng;
+        if (t >= (long long)ST.total() * ring) return;
+        const int o = (int)(t / ring);
+        const int s = ST.seg(o);
+        ti_ring_cell<R>(ST.ops[s][o - ST.start[s]], (int)(t - (long long)o * ring), P, mx, my, meqn, TW.a[s]);
+        return;
+    }
     const int o = (int)(gid / items);
     const int k = (int)(gid - (long long)o * items);
     const GhostOp op = SG.at(o);
@@ -651,11 +699,21 @@ __global__ void k_copy_tiles(const int* __restrict__ slots, int n, const R* __re
 static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <class R>
+void launch_ghost_ti(const Segs<GhostOp>& S, const Segs<TiOp>& T, const double* alpha, const Pools<R>& P,
+                     int mx, int my, int meqn, hipStream_t s)
+{
+    if (S.total() <= 0 && T.total() <= 0) return;
+    TiWeights<R> w{};
+    for (int k = 0; k < T.n; k++) w.a[k] = R(alpha[k]);
+    const long long n =
+        (long long)S.total() * MBC * (mx > my ? mx : my) + (long long)T.total() * ti_ring_size(mx, my);
+    hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, S, T, w, P, mx, my, meqn);
+}
+
+template <class R>
 void launch_ghost(const Segs<GhostOp>& S, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
 {
-    if (S.total() <= 0) return;
-    const long long n = (long long)S.total() * MBC * (mx > my ? mx : my);
-    hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, S, P, mx, my, meqn);
+    launch_ghost_ti<R>(S, Segs<TiOp>(), nullptr, P, mx, my, meqn, s);
 }
 
 template <class R>
@@ -751,6 +809,8 @@ void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long 
     template void launch_ghost<R>(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t); \
     template void launch_bc<R>(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);       \
     template void launch_ghost<R>(const Segs<GhostOp>&, const Pools<R>&, int, int, int, hipStream_t); \
+    template void launch_ghost_ti<R>(const Segs<GhostOp>&, const Segs<TiOp>&, const double*,             \
+                                     const Pools<R>&, int, int, int, hipStream_t);                     \
     template void launch_bc<R>(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);       \
     template void launch_timeinterp<R>(const TiOp*, int, const Pools<R>&, int, int, int, double,     \
                                        hipStream_t);                                              \

@@ -3,6 +3,7 @@
 // velocities, physical boundary conditions and the level CFL reduction.
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <mutex>
 #include <string>
 
@@ -199,7 +200,9 @@ __global__ void k_bc2(int npatch, int mx, int my, int meqn, R* q, const int* ids
 }
 
 // Maximum of n non-negative doubles (one workgroup; n is a patch count).
-__global__ void __launch_bounds__(1024) k_max_reduce(const double* x, int n, double* out)
+// The result is +0 or positive (NaN and -0 entries never win the comparisons);
+// as bits (OUT = unsigned long long) it orders as the doubles do.
+template <class OUT> __device__ inline void max_reduce_body(const double* x, int n, OUT* out)
 {
     __shared__ double s[1024];
     // eight independent loads in flight per thread: a level has up to ~10^5
@@ -228,7 +231,52 @@ __global__ void __launch_bounds__(1024) k_max_reduce(const double* x, int n, dou
         if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x + w] > s[threadIdx.x] ? s[threadIdx.x + w] : s[threadIdx.x];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (threadIdx.x == 0) {
+        if constexpr (sizeof(OUT) == 8 && !std::is_floating_point<OUT>::value)
+            *out = (OUT)__double_as_longlong(s[0]);
+        else
+            *out = s[0];
+    }
+}
+__global__ void __launch_bounds__(1024) k_max_reduce(const double* x, int n, double* out)
+{
+    max_reduce_body(x, n, out);
+}
+__global__ void __launch_bounds__(1024) k_max_reduce_bits(const double* x, int n, unsigned long long* out)
+{
+    max_reduce_body(x, n, out);
+}
+
+// Maximum of the n slots of a coarse step (bits of non-negative doubles)
+// into out as a double; every slot is cleared after it is read.
+__global__ void __launch_bounds__(1024) k_cfl_final(unsigned long long* acc, int n, double* out)
+{
+    __shared__ unsigned long long s[1024];
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const unsigned long long a = acc[i];
+        v = a > v ? a : v;
+        acc[i] = 0;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = __longlong_as_double((long long)s[0]);
+}
+
+int cfl_slot_reduce(const double* x, int n, unsigned long long* slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_max_reduce_bits, dim3(1), dim3(1024), 0, s, x, n, slot);
+    return check_hip(hipGetLastError(), "cfl slot reduce");
+}
+
+int cfl_final(unsigned long long* acc, int n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(1024), 0, s, acc, n, out);
+    return check_hip(hipGetLastError(), "cfl final");
 }
 
 }  // namespace hc
@@ -416,6 +464,7 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         a.swirl = ml->swirl;
         a.xylower = ml->xylower;
         a.par = ml->par;
+        a.cflmax = ml->cflmax;
         for (int L = 0; L < STEP_MAXL; L++) a.vtl[L] = ml->vtl[L];
         for (int L = 0; L < STEP_MAXL; L++) {
             a.dtl[L] = ml->dtl[L];

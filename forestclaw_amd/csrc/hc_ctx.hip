@@ -123,6 +123,8 @@ void Ctx::release_pools()
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
+    if (d_cflslot) (void)hipFree(d_cflslot);
+    d_cflslot = nullptr;
     double** dbufs[] = {&d_xylower, &d_dxdy, &d_cfl, &d_cflacc, &d_mass, &d_reg, &d_tagst};
     for (auto b : dbufs) {
         if (*b) (void)hipFree(*b);
@@ -259,6 +261,9 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
     HC_TRY(dmalloc((void**)&d_dxdy, sizeof(double) * 2 * nn));
     HC_TRY(dmalloc((void**)&d_cfl, sizeof(double) * nn));
     HC_TRY(dmalloc((void**)&d_cflacc, sizeof(double) * 4096));
+    HC_TRY(dmalloc((void**)&d_cflslot, sizeof(unsigned long long) * 4096));
+    HC_TRY(check_hip(hipMemset(d_cflslot, 0, sizeof(unsigned long long) * 4096), "cfl slots"));
+    cflslot_dirty = false;
     HC_TRY(dmalloc((void**)&d_levels, sizeof(int) * nn));
     HC_TRY(dmalloc((void**)&d_iota, sizeof(int) * nn));
     HC_TRY(dmalloc((void**)&d_tags, sizeof(int) * nn));
@@ -338,7 +343,10 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
 // ---------------------------------------------------------------------------
 int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
 {
-    if (par_dry) return HIPCLAW_OK;
+    if (par_dry) {
+        ti_pending.clear();
+        return HIPCLAW_OK;
+    }
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
     const int mincoarse = minlevel, maxcoarse = maxlevel - 1;
     const int tl = minlevel - 1;
@@ -363,9 +371,19 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
             if (use_ti) bcc.add(plan[tl].bc_ti.p, plan[tl].bc_ti.n);
             for (int L = maxlevel; L >= minlevel; L--) bca.add(plan[L].bc.p, plan[L].bc.n);
             if (use_ti) bca.add(plan[tl].bc_ti.p, plan[tl].bc_ti.n);
-            if (!avg.ok || !copy.ok || !bcc.ok || !bca.ok)
+            // the pending time interpolations write the interior ring of the
+            // interpolated pool, which no average reads or writes
+            Segs<TiOp> tis;
+            double alpha[SEG_MAX];
+            for (const auto& ti : ti_pending) {
+                if (plan[ti.first].ti.n <= 0) continue;
+                if (tis.n < SEG_MAX) alpha[tis.n] = ti.second;
+                tis.add(plan[ti.first].ti.p, plan[ti.first].ti.n);
+            }
+            ti_pending.clear();
+            if (!avg.ok || !copy.ok || !bcc.ok || !bca.ok || !tis.ok)
                 return set_error(HIPCLAW_ERR_UNSUPPORTED, "ghost update: too many operation lists");
-            launch_ghost<R>(avg, P, mx, my, meqn, stream);
+            launch_ghost_ti<R>(avg, tis, alpha, P, mx, my, meqn, stream);
             launch_ghost<R>(copy, P, mx, my, meqn, stream);
             launch_bc<R>(bcc, P, mx, my, meqn, stream);
             for (int L = maxcoarse; L >= mincoarse; L--)
@@ -471,7 +489,7 @@ int Ctx::update_level(int level, double t, double dt)
         ev_cells.push_back((long long)np * phys.mx * phys.my);
     }
     if (cflacc_count < 4095) {
-        HC_TRY(hipclaw_max_reduce(d_cfl + s0, np, d_cflacc + cflacc_count, stream));
+        HC_TRY(cfl_slot_reduce(d_cfl + s0, np, d_cflslot + cflacc_count, stream));
         cflacc_count++;
     }
     parity[level] ^= 1;
@@ -627,6 +645,15 @@ int Ctx::time_sync(int minlevel, int maxlevel)
     return check_hip(hipGetLastError(), "time sync");
 }
 
+// After a coarse step that did not finish: clear every Courant-number slot
+// (the final reduction of the slots, into an unused element).
+int Ctx::clear_cfl_slots()
+{
+    HC_TRY(cfl_final(d_cflslot, 4095, d_cflacc + 4094, stream));
+    cflslot_dirty = false;
+    return HIPCLAW_OK;
+}
+
 int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
 {
     // graph mode: this launch's dt and vt per level into the parameter block
@@ -685,6 +712,10 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
     const bool push = use_push();
     ml.push = push ? d_push : nullptr;
     ml.par = par_dev;
+    // small launches take their Courant-number maximum by atomics in the
+    // step kernel (one launch fewer per fine step); large ones by a reduction
+    const bool cfl_atomic = cflacc_count < 4095 && np < CFL_ATOMIC_MAX;
+    ml.cflmax = cfl_atomic ? d_cflslot + cflacc_count : nullptr;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
     if (src2) {
@@ -704,7 +735,7 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
         ev_cells.push_back((long long)np * phys.mx * phys.my);
     }
     if (cflacc_count < 4095) {
-        HC_TRY(hipclaw_max_reduce(d_cfl + s0, np, d_cflacc + cflacc_count, stream));
+        if (!cfl_atomic) HC_TRY(cfl_slot_reduce(d_cfl + s0, np, d_cflslot + cflacc_count, stream));
         cflacc_count++;
     }
     for (int L = lo; L <= hi; L++) parity[L] ^= 1;
@@ -757,9 +788,11 @@ int Ctx::advance(double t, double dt, double* maxcfl)
                           (!graph_auto || fine_steps > 1) &&
                           !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV &&
                             !fused_swirl());
+    if (cflslot_dirty) HC_TRY(clear_cfl_slots());
+    cflslot_dirty = true;
     if (!graph_ok) {
         HC_TRY(advance_body(t, dt));
-        if (maxcfl) HC_TRY(hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream));
+        HC_TRY(cfl_final(d_cflslot, cflacc_count, d_cflacc + 4095, stream));
     } else {
         HC_TRY(ensure_par());
         // host parameters of this coarse step into a free ring slot
@@ -807,7 +840,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
             rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
             if (!rc) {
                 rc = advance_body(t, dt);
-                if (!rc) rc = hipclaw_max_reduce(d_cflacc, cflacc_count, d_cflacc + 4095, stream);
+                if (!rc) rc = cfl_final(d_cflslot, cflacc_count, d_cflacc + 4095, stream);
                 if (!rc && par_n != nlaunch) rc = set_error(HIPCLAW_ERR_STATE, "graph schedule mismatch");
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(stream, &g);
@@ -847,6 +880,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
         }
         HC_TRY(check_hip(hipGraphLaunch(e->exec, stream), "graph launch"));
     }
+    cflslot_dirty = false;
     if (maxcfl) {
         HC_TRY(check_hip(hipMemcpyAsync(maxcfl, d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, stream),
                          "cfl D2H"));
@@ -883,6 +917,7 @@ int Ctx::advance_body(double t, double dt)
         }
     }
     cflacc_count = 0;
+    ti_pending.clear();
     // recursive level advance (advance_level).  The recursion is walked on the
     // host first: the levels it updates at one fine step are independent of
     // each other (each reads its own pool and writes its other one), so they
@@ -913,7 +948,10 @@ int Ctx::advance_body(double t, double dt)
         tis.clear();
         advance_level(advance_level, hi, nf);
         HC_TRY(update_levels(ulo, hi, tl, dtl));
-        for (const auto& ti : tis) HC_TRY(timeinterp(ti.first, ti.second));
+        if (merge_levels)  // run by the average launch of the next ghost fill
+            ti_pending.insert(ti_pending.end(), tis.begin(), tis.end());
+        else
+            for (const auto& ti : tis) HC_TRY(timeinterp(ti.first, ti.second));
         if (c[hi].last_step < c[hi].total_steps) {
             if (sub) {
                 int tl = hi - 1;
@@ -1655,15 +1693,20 @@ int hipclaw_update_level(hipclaw_ctx* ctx, int level, double t, double dt, doubl
     CTX_READY(ctx);
     Ctx& c = *ctx->c;
     if (level < 0 || level >= MAXL) return set_error(HIPCLAW_ERR_ARG, "level out of range");
+    if (c.cflslot_dirty) HC_TRY(c.clear_cfl_slots());
     c.cflacc_count = 0;
+    c.cflslot_dirty = true;
     int rc = c.update_level(level, t, dt);
     if (rc) return rc;
+    if (c.cflacc_count > 0) HC_TRY(cfl_final(c.d_cflslot, c.cflacc_count, c.d_cflacc + 4095, c.stream));
+    c.cflslot_dirty = false;
     if (cfl) {
         if (c.cflacc_count == 0) {
             *cfl = 0.0;
             return HIPCLAW_OK;
         }
-        rc = check_hip(hipMemcpyAsync(cfl, c.d_cflacc, sizeof(double), hipMemcpyDeviceToHost, c.stream), "cfl");
+        rc = check_hip(hipMemcpyAsync(cfl, c.d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                       "cfl");
         if (rc) return rc;
         return check_hip(hipStreamSynchronize(c.stream), "update_level");
     }

@@ -39,6 +39,10 @@ struct LevelPlan {
     int s0 = 0, s1 = 0;                  // slot range of the level
 };
 
+// step launches of fewer patches than this reduce their Courant numbers by
+// atomics in the step kernel (at most this many atomics on one address)
+constexpr int CFL_ATOMIC_MAX = 2048;
+
 struct Ctx {
     hipclaw_physics phys{};
     hipclaw_domain dom{};
@@ -54,7 +58,17 @@ struct Ctx {
     double* d_xylower = nullptr;   // 2 per slot
     double* d_dxdy = nullptr;      // 2 per slot
     double* d_cfl = nullptr;       // per slot
-    double* d_cflacc = nullptr;    // running max of level CFLs (device scalar array)
+    double* d_cflacc = nullptr;    // [4095]: the last coarse step's maximum Courant number
+    // Courant-number slots of a coarse step: one per step launch, the bits of
+    // a non-negative double (small launches: atomic max from the step kernel,
+    // so zero before it; larger ones: stored by a reduction); cleared by the
+    // reduction that reads them (cfl_final)
+    unsigned long long* d_cflslot = nullptr;
+    bool cflslot_dirty = false;    // slots possibly not cleared (an interrupted step)
+    // time interpolations (level, weight) of a coarse step waiting for the
+    // next ghost fill, whose average launch runs them (merged levels)
+    std::vector<std::pair<int, double>> ti_pending;
+    int clear_cfl_slots();
     int* d_levels = nullptr;       // level per slot
     int* d_iota = nullptr;         // 0..cap-1
     int* d_tags = nullptr;         // per slot

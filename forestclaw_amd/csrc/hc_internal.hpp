@@ -76,7 +76,14 @@ struct MultiLevel {
     // optional device parameter block: dt of level L at par[L], vt at
     // par[STEP_MAXL + L] (graph replays; see StepArgs::par)
     const double* par;
+    // optional atomic launch maximum of the patch Courant numbers (StepArgs::cflmax)
+    unsigned long long* cflmax;
 };
+// Courant-number slots of a coarse step (hc_api.hip): the maximum of n
+// patch values stored into one slot as double bits; the maximum of n slots
+// into out, clearing the slots.
+int cfl_slot_reduce(const double* x, int n, unsigned long long* slot, hipStream_t s);
+int cfl_final(unsigned long long* acc, int n, double* out, hipStream_t s);
 int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void* q_out,
                  const void* aux, const double* dxdy, const int* patch_ids, int ntiles, double dt,
                  double* cfl_patch, double* reg, const hipclaw_launch* launch, void* stream,

@@ -72,6 +72,10 @@ struct StepArgs {
     // read from the device at par[L] and par[STEP_MAXL + L] (written before
     // each replay) instead of dtl / vtl, so one graph serves every dt.
     const double* par;
+    // optional launch maximum of the patch Courant numbers, kept as the bits
+    // of a non-negative double (integer order = double order) by atomic max
+    // of every positive patch value; the caller clears it beforehand
+    unsigned long long* cflmax;
     // Row bands (nband > 1, q_out != q_in): a workgroup of
     // nband waves takes the same ppw patches, wave b the interior rows
     // [2 + b*band_rows, 2 + (b+1)*band_rows) (clipped to my), sweeping from
@@ -1057,9 +1061,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             double m = cfl_band[0][slot];
             for (int b = 1; b < A.nband; b++) m = dmax(m, cfl_band[b][slot]);
             cflp[tix] = m;
+            if (A.cflmax && m > 0.0) atomicMax(A.cflmax, (unsigned long long)__double_as_longlong(m));
         }
     } else {
-        if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
+        if (pact && lp == A.lpp - 1 && cflp) {
+            cflp[tix] = cd;
+            if (A.cflmax && cd > 0.0) atomicMax(A.cflmax, (unsigned long long)__double_as_longlong(cd));
+        }
     }
 }
 
