@@ -98,17 +98,17 @@ __device__ inline long long reg_off(int mx, int my, int meqn, int face, int bloc
     return off + (long long)block * meqn * (face < 2 ? my : mx);
 }
 
-// r += s * v and r -= s * v in FP64 without contraction, as
+// r + s * v and r - s * v in FP64 without contraction, as
 // clawpack46_time_sync.f:37-156 and :163-226 accumulate.
-__device__ inline void reg_add(double* r, double s, double v)
+__device__ inline double reg_plus(double r, double s, double v)
 {
 #pragma clang fp contract(off)
-    *r = *r + s * v;
+    return r + s * v;
 }
-__device__ inline void reg_sub(double* r, double s, double v)
+__device__ inline double reg_minus(double r, double s, double v)
 {
 #pragma clang fp contract(off)
-    *r = *r - s * v;
+    return r - s * v;
 }
 
 // 0 + x and 0 - x: the reference accumulates fluxes into zeroed arrays, and
@@ -450,33 +450,60 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             else cdc[k][0] = R(0);
         }
 
-        // ---- time-sync edge fluxes of the data before the update ----------
+        // ---- time-sync registers of this row ------------------------------
+        // Every register element takes one addend per step (FP64, old + s*v or
+        // old - s*v, uncontracted, as clawpack46_time_sync.f accumulates).  The
+        // elements this row updates are loaded here, ahead of the row's
+        // arithmetic, and written where their addends are made:
+        //   0  x-face edge flux of row J           lanes 1, 2, NX-3, NX-2
+        //   1  x-face fluctuation of row J-2       the same lanes
+        //   2  y-face edge flux of rows 1, 2, NY-3, NY-2
+        //   3  y-face fluctuation of rows 2, NY-3  (row J-2)
+        //   4  y-face fluctuation of ghost rows 1, NY-2 (rows J = 3, NY-1)
+        // Slots 0 and 2 take their addends from the row's data (the edge
+        // fluxes before the update) and are written at the end of the row.
+        int tso[CPL][5];
+        double tsold[CPL][5][ME];
+        double tsv0[CPL][ME], tsv2[CPL][ME];
+        double* rg = nullptr;
+        double dtel_x = 0.0, dtel_y = 0.0;
         if constexpr (SYNC) {
-            double* rg = A.reg + (long long)tix * A.regstride;
-            const double dtel_y = dtp * dy, dtel_x = dtp * dx;
+            rg = A.reg + (long long)tix * A.regstride;
+            dtel_y = dtp * dy;
+            dtel_x = dtp * dx;
+            const int Rw2 = J - 2;
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
                 const int c = col[k];
-                double qd[ME], fl[ME];
+                const bool xe = cv[k] && (c == 1 || c == 2 || c == NX - 3 || c == NX - 2);
+                const bool yi = cv[k] && c >= 2 && c <= NX - 3;
+                const int xf = c <= 2 ? 0 : 1;
+                tso[k][0] = (xe && J >= ra && J < rb)
+                                ? (int)reg_off(A.mx, A.my, ME, xf, (c == 2 || c == NX - 3) ? 2 : 3) + (J - 2)
+                                : -1;
+                tso[k][1] = (xe && Rw2 >= ra && Rw2 < rb)
+                                ? (int)reg_off(A.mx, A.my, ME, xf, (c == 1 || c == NX - 3) ? 1 : 0) + (Rw2 - 2)
+                                : -1;
+                tso[k][2] = (yi && (((J == 1 || J == 2) && first_band) || ((J == NY - 3 || J == NY - 2) && last_band)))
+                                ? (int)reg_off(A.mx, A.my, ME, J <= 2 ? 2 : 3, (J == 2 || J == NY - 3) ? 2 : 3) + (c - 2)
+                                : -1;
+                tso[k][3] = (yi && Rw2 >= ra && Rw2 < rb && (Rw2 == 2 || Rw2 == NY - 3))
+                                ? (int)reg_off(A.mx, A.my, ME, Rw2 == 2 ? 2 : 3, Rw2 == 2 ? 0 : 1) + (c - 2)
+                                : -1;
+                tso[k][4] = (yi && ((J == 3 && first_band) || (J == NY - 1 && last_band)))
+                                ? (int)reg_off(A.mx, A.my, ME, J == 3 ? 2 : 3, J == 3 ? 1 : 0) + (c - 2)
+                                : -1;
+#pragma unroll
+                for (int sl = 0; sl < 5; sl++) {
+                    const int st = sl < 2 ? A.my : A.mx;  // element stride of the field m
+#pragma unroll
+                    for (int m = 0; m < ME; m++) tsold[k][sl][m] = tso[k][sl] >= 0 ? rg[tso[k][sl] + m * st] : 0.0;
+                }
+                double qd[ME];
 #pragma unroll
                 for (int m = 0; m < ME; m++) qd[m] = (double)qc[k][m];
-                if (cv[k] && J >= ra && J < rb && (c == 1 || c == 2 || c == NX - 3 || c == NX - 2)) {
-                    const int face = c <= 2 ? 0 : 1;
-                    const int blk = (c == 2 || c == NX - 3) ? 2 : 3;
-                    S::template cons_flux<0>(qd, pp.grav, fl);
-                    double* r = rg + reg_off(A.mx, A.my, ME, face, blk) + (J - 2);
-#pragma unroll
-                    for (int m = 0; m < ME; m++) reg_add(r + m * A.my, dtel_y, fl[m]);
-                }
-                if (cv[k] && c >= 2 && c <= NX - 3 &&
-                    (((J == 1 || J == 2) && first_band) || ((J == NY - 3 || J == NY - 2) && last_band))) {
-                    const int face = J <= 2 ? 2 : 3;
-                    const int blk = (J == 2 || J == NY - 3) ? 2 : 3;
-                    S::template cons_flux<1>(qd, pp.grav, fl);
-                    double* r = rg + reg_off(A.mx, A.my, ME, face, blk) + (c - 2);
-#pragma unroll
-                    for (int m = 0; m < ME; m++) reg_add(r + m * A.mx, dtel_x, fl[m]);
-                }
+                if (tso[k][0] >= 0) S::template cons_flux<0>(qd, pp.grav, tsv0[k]);
+                if (tso[k][2] >= 0) S::template cons_flux<1>(qd, pp.grav, tsv2[k]);
             }
         }
 
@@ -869,54 +896,49 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     (void)e2l;
                     if constexpr (SYNC) {
                         // fluctuations on the patch edges (accumulate_waves)
-                        double* rg = A.reg + (long long)tix * A.regstride;
                         const int c = col[k];
-                        const double dtel_y = dtp * dy, dtel_x = dtp * dx;
-                        if (cv[k] && (c == 1 || c == NX - 3)) {
-                            const R fmr = (fmx2[k][m] + e2[k][m]) + e1r;
-                            reg_add(rg + reg_off(A.mx, A.my, ME, c == 1 ? 0 : 1, 1) + m * A.my + (Rw - 2),
-                                    dtel_y, (double)fmr);
+                        if (tso[k][1] >= 0) {
+                            double* r = rg + tso[k][1] + m * A.my;
+                            if (c == 1 || c == NX - 3)
+                                *r = reg_plus(tsold[k][1][m], dtel_y, (double)((fmx2[k][m] + e2[k][m]) + e1r));
+                            else
+                                *r = reg_minus(tsold[k][1][m], dtel_y, (double)((fpx2[k][m] + e2l) + e1[k][m]));
                         }
-                        if (cv[k] && (c == 2 || c == NX - 2)) {
-                            const R fpc = (fpx2[k][m] + e2l) + e1[k][m];
-                            reg_sub(rg + reg_off(A.mx, A.my, ME, c == 2 ? 0 : 1, 0) + m * A.my + (Rw - 2),
-                                    dtel_y, (double)fpc);
-                        }
-                        if (cv[k] && c >= 2 && c <= NX - 3 && Rw == 2) {
-                            const R gpc = G2[k][m] + fdp_prev[k][m];
-                            reg_sub(rg + reg_off(A.mx, A.my, ME, 2, 0) + m * A.mx + (c - 2), dtel_x,
-                                    (double)gpc);
-                        }
-                        if (cv[k] && c >= 2 && c <= NX - 3 && Rw == NY - 3) {
-                            const R gmu = G1[k][m] + fdm[k][m];
-                            reg_add(rg + reg_off(A.mx, A.my, ME, 3, 1) + m * A.mx + (c - 2), dtel_x,
-                                    (double)gmu);
+                        if (tso[k][3] >= 0) {
+                            double* r = rg + tso[k][3] + m * A.mx;
+                            if (Rw == 2)
+                                *r = reg_minus(tsold[k][3][m], dtel_x, (double)(G2[k][m] + fdp_prev[k][m]));
+                            else
+                                *r = reg_plus(tsold[k][3][m], dtel_x, (double)(G1[k][m] + fdm[k][m]));
                         }
                     }
                 }
             }
         }
 
-        // ---- time-sync fluctuations on the two ghost-row edge faces ---------
+        // ---- time-sync fluctuations on the two ghost-row edge faces, and
+        // the edge fluxes of the row's data (slots 0 and 2) ------------------
         if constexpr (SYNC) {
-            double* rg = A.reg + (long long)tix * A.regstride;
-            const double dtel_x = dtp * dx;
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
-                const int c = col[k];
-                if (!(cv[k] && c >= 2 && c <= NX - 3)) continue;
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
-                    if (J == 3 && first_band)  // gm on the bottom edge face (top face of ghost row 1)
-                        reg_add(rg + reg_off(A.mx, A.my, ME, 2, 1) + m * A.mx + (c - 2), dtel_x,
-                                (double)(G1[k][m] + fdm[k][m]));
-                    if (J == NY - 1 && last_band)  // gp on the top edge face (bottom face of ghost row NY-2)
-                        reg_sub(rg + reg_off(A.mx, A.my, ME, 3, 0) + m * A.mx + (c - 2), dtel_x,
-                                (double)(G1[k][m] + fdp[k][m]));
+                    if (tso[k][4] >= 0) {
+                        // gm on the bottom edge face (top face of ghost row 1),
+                        // gp on the top edge face (bottom face of ghost row NY-2)
+                        double* r = rg + tso[k][4] + m * A.mx;
+                        if (J == 3)
+                            *r = reg_plus(tsold[k][4][m], dtel_x, (double)(G1[k][m] + fdm[k][m]));
+                        else
+                            *r = reg_minus(tsold[k][4][m], dtel_x, (double)(G1[k][m] + fdp[k][m]));
+                    }
+                    if (tso[k][0] >= 0)
+                        rg[tso[k][0] + m * A.my] = reg_plus(tsold[k][0][m], dtel_y, tsv0[k][m]);
+                    if (tso[k][2] >= 0)
+                        rg[tso[k][2] + m * A.mx] = reg_plus(tsold[k][2][m], dtel_x, tsv2[k][m]);
                 }
             }
         }
-
 
         // ---- this row's values enter the rings (renames when unrolled) ----
 #pragma unroll

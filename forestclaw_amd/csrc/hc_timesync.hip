@@ -59,9 +59,11 @@ template <class R> struct QList {
     R* q[SEG_MAX];
 };
 
-// One workgroup (one wave) per patch.  LDS: add[f][g][m][k], f < 4, g < 2.
+// One workgroup of TS_THREADS per patch (the addends of all four faces in
+// one pass, then the edge cells).  LDS: add[f][g][m][k], f < 4, g < 2.
+constexpr int TS_THREADS = 256;
 template <class R>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(TS_THREADS)
 k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ reg, long long regstride,
              const double* __restrict__ dxdy, int mx, int my, int meqn)
 {
@@ -78,21 +80,25 @@ k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ r
     auto A = [&](int f, int g, int m, int k) -> double& {
         return add[(((long long)f * 2 + g) * meqn + m) * maxlen + k];
     };
-    // phase 1: addends of every face
-    for (int f = 0; f < 4; f++) {
+    // phase 1: addends of every face, items (f, m, k) of the four faces in turn
+    const int nx_items = meqn * my, ntot = 2 * meqn * (mx + my);
+    for (int t = threadIdx.x; t < ntot; t += blockDim.x) {
+        int f, u;
+        if (t < 2 * nx_items) { f = t < nx_items ? 0 : 1; u = t - f * nx_items; }
+        else { const int v = t - 2 * nx_items; f = v < meqn * mx ? 2 : 3; u = v - (f - 2) * meqn * mx; }
+        // the face's entries by selects (f differs across the lanes)
+        const int kind = f == 0 ? o.kind[0] : f == 1 ? o.kind[1] : f == 2 ? o.kind[2] : o.kind[3];
+        if (kind == TS_NONE) continue;
+        const int n0 = f == 0 ? o.nbr[0][0] : f == 1 ? o.nbr[1][0] : f == 2 ? o.nbr[2][0] : o.nbr[3][0];
+        const int n1 = f == 0 ? o.nbr[0][1] : f == 1 ? o.nbr[1][1] : f == 2 ? o.nbr[2][1] : o.nbr[3][1];
         const int len = f < 2 ? my : mx;
-        if (o.kind[f] == TS_NONE) continue;
-        for (int t = threadIdx.x; t < meqn * len; t += blockDim.x) {
-            const int m = t / len, k = t - m * len;
-            if (o.kind[f] == TS_SAME) {
-                const double* rn = reg + (long long)o.nbr[f][0] * regstride;
-                A(f, 0, m, k) = 0.5 * ts_delta_same(rt, rn, mx, my, meqn, f, m, k) / area;
-            } else {
-                for (int g = 0; g < 2; g++) {
-                    const double* rf = reg + (long long)o.nbr[f][g] * regstride;
-                    A(f, g, m, k) = ts_delta_f2c(rt, rf, mx, my, meqn, f, g, m, k) / area;
-                }
-            }
+        const int m = u / len, k = u - m * len;
+        if (kind == TS_SAME) {
+            const double* rn = reg + (long long)n0 * regstride;
+            A(f, 0, m, k) = 0.5 * ts_delta_same(rt, rn, mx, my, meqn, f, m, k) / area;
+        } else {
+            A(f, 0, m, k) = ts_delta_f2c(rt, reg + (long long)n0 * regstride, mx, my, meqn, f, 0, m, k) / area;
+            A(f, 1, m, k) = ts_delta_f2c(rt, reg + (long long)n1 * regstride, mx, my, meqn, f, 1, m, k) / area;
         }
     }
     __syncthreads();
@@ -146,8 +152,8 @@ void launch_ts_correct(const Segs<TsOp>& S, R* const* qs, const double* reg, lon
     QList<R> ql{};
     for (int k = 0; k < S.n; k++) ql.q[k] = qs[k];
     const size_t lds = sizeof(double) * 8 * meqn * (mx > my ? mx : my);
-    hipLaunchKernelGGL(k_ts_correct<R>, dim3(S.total()), dim3(64), lds, s, S, ql, reg, regstride, dxdy, mx,
-                       my, meqn);
+    hipLaunchKernelGGL(k_ts_correct<R>, dim3(S.total()), dim3(TS_THREADS), lds, s, S, ql, reg, regstride, dxdy,
+                       mx, my, meqn);
 }
 
 template <class R>
