@@ -161,7 +161,7 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
 #define HC_STEP_MIN_WAVES 2
 #endif
 template <class R, int CPL, bool SYNC>
-constexpr int STEP_UNR = (SYNC || (sizeof(R) == 8 && CPL == 2)) ? 1 : HC_STEP_UNROLL;
+constexpr int STEP_UNR = ((SYNC && sizeof(R) == 8) || (sizeof(R) == 8 && CPL == 2)) ? 1 : HC_STEP_UNROLL;
 template <class R, int CPL, bool SYNC>
 constexpr int STEP_WPE = (!SYNC && (sizeof(R) == 4 || CPL == 1)) ? HC_STEP_MIN_WAVES : 1;
 
