@@ -281,13 +281,16 @@ BF16_PEAK = 2.5e15
 def small_configs(args):
     """The other configurations (forestclaw_amd.configs C1 swirl, C2 radial
     dam, C3 shock-bubble) on their meshes after the initial refinement, no
-    regrid: FP32 batched (the library defaults: merged launches, ghost push,
-    graph replays of subcycled coarse steps) against FP64 plain (one launch per level step, eager), each
-    timed over its stated run length after two warm-up steps."""
+    regrid, fixed dt: FP32 batched (the library defaults: merged launches,
+    ghost push, graph replays; the steps go to the device in calls of up to
+    64, as the fixed-dt run loop takes them, Hipclaw46.run) against FP64
+    plain (one call and one launch per level step, eager), each timed over
+    its stated run length after untimed warm-up calls of the same sizes."""
     import torch
     from forestclaw_amd import configs as CF
     from forestclaw_amd import hipclaw as H
     lengths = {"C1": 200, "C2": 100, "C3": 30}
+    chunk = 64
     if not _STREAM:
         _STREAM.append(torch.cuda.Stream())
     out = {}
@@ -295,25 +298,34 @@ def small_configs(args):
         res = {}
         for label, prec, plain in (("fp32_batched", H.FP32, False), ("fp64_plain", H.FP64, True)):
             sim, dt = CF.build(name, prec, stream=_STREAM[0].cuda_stream)
-            if plain:  # batched keeps the defaults (graphs for subcycled steps)
+            if plain:  # batched keeps the defaults
                 sim.set_feature(H.FEATURE_GRAPH, False)
                 sim.set_feature(H.FEATURE_GHOST_PUSH, False)
                 sim.set_feature(H.FEATURE_MERGE_LEVELS, False)
+            calls = [1] * nsteps if plain else \
+                [chunk] * (nsteps // chunk) + ([nsteps % chunk] if nsteps % chunk else [])
             t = 0.0
-            for s in range(2):
-                sim.advance(t, dt)
-                t += dt
+
+            def go(k, t):
+                if plain:
+                    sim.advance(t, dt)
+                else:
+                    sim.advance_steps(t, dt, k)
+                for _ in range(k):
+                    t += dt
+                return t
+            for k in sorted(set(calls)) if not plain else [1, 1]:
+                t = go(k, t)
             c0 = sim.cell_updates()
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for s in range(nsteps):
-                sim.advance(t, dt)
-                t += dt
+            for k in calls:
+                t = go(k, t)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             res[label] = {"cell_updates_per_s": (sim.cell_updates() - c0) / (t1 - t0),
                           "ms_per_step": 1e3 * (t1 - t0) / nsteps, "patches": sim.num_patches(),
-                          "graph_replays": sim.graph_stats()[1]}
+                          "steps_per_call": max(calls), "graph_replays": sim.graph_stats()[1]}
             sim.close()
         res["speedup"] = res["fp32_batched"]["cell_updates_per_s"] / res["fp64_plain"]["cell_updates_per_s"]
         res["steps"] = nsteps
@@ -436,6 +448,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="leave out the C1-C3 timings (key \"configs\" of the default line)")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of hipGraph replay")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain device copies of 1 GiB (float32), the reference "
@@ -528,6 +542,10 @@ def main():
         log("CPU baseline")
         cpu, port = cpu_baseline(sim64, args.cpu_seconds)
     hp = host_path(sim64, args.host_path, dt) if args.host_path > 0 else None
+    small = None
+    if not args.no_configs:
+        log("configurations C1-C3")
+        small = small_configs(args)
 
     w = 4
     T, N, meqn = 36 * 36, 32 * 32, 3
@@ -579,6 +597,8 @@ def main():
     }
     if port is not None:
         res["cpu_port_openmp"] = port
+    if small is not None:
+        res["configs"] = small
     if hp is not None:
         res["host_path_pcie"] = hp
     if args.cnn > 0:

@@ -424,6 +424,9 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
     if (!patch_ids && ntiles < npatch)
         return set_error(HIPCLAW_ERR_ARG, "ntiles < npatch without a patch list");
     if (npatch == 0) return HIPCLAW_OK;
+    // lanes of the kernels that own no patch read tile 0 (their row loads are
+    // unconditional), so it must exist
+    if (ntiles < 1) return set_error(HIPCLAW_ERR_ARG, "ntiles must be >= 1");
     if (!q_in || !q_out || !dxdy) return set_error(HIPCLAW_ERR_ARG, "NULL buffer");
     if (phys->maux > 0 && !aux) return set_error(HIPCLAW_ERR_ARG, "aux is NULL but maux > 0");
     if (!(dt >= 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be >= 0");
@@ -466,9 +469,12 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         a.par = ml->par;
         a.cflmax = ml->cflmax;
         for (int L = 0; L < STEP_MAXL; L++) a.vtl[L] = ml->vtl[L];
+        // levels the launch does not cover get q_in, the pool every slot of
+        // the launch lives in: the kernels' lanes without a patch read from
+        // the tile of slot slot0 through the pointer of level 0
         for (int L = 0; L < STEP_MAXL; L++) {
             a.dtl[L] = ml->dtl[L];
-            a.qin_l[L] = ml->qin_l[L];
+            a.qin_l[L] = ml->qin_l[L] ? ml->qin_l[L] : q_in;
             a.qout_l[L] = ml->qout_l[L];
         }
     }

@@ -768,34 +768,51 @@ static unsigned level_bits(const bool* f)
     return b;
 }
 
-// Coarse step with the Courant-number reduction; eager or as a hipGraph.
-// A graph records the device work of one coarse step; what depends on (t, dt),
-// each step launch's dt and swirl vt per level, comes from the parameter
-// block d_par, which a dry pass of the schedule fills on the host (the same
-// arithmetic as the eager path) and one copy uploads before each launch.  The
-// schedule itself (which levels step when, the time-interpolation weights)
-// depends only on the step counters, so a graph serves every dt; it is keyed
-// by the pool parities at entry and the mesh.
+// Coarse steps with the Courant-number reduction; eager or as a hipGraph.
+// A graph records the device work of nstep coarse steps; what depends on
+// (t, dt), each step launch's dt and swirl vt per level, comes from the
+// parameter block d_par, which a dry pass of the schedule fills on the host
+// (the same arithmetic as the eager path) and one copy uploads before each
+// launch.  The schedule itself (which levels step when, the
+// time-interpolation weights) depends only on the step counters, so a graph
+// serves every dt; it is keyed by the pool parities at entry, the mesh and
+// the number of steps.  Step k's maximum Courant number goes to
+// d_cflacc[4095] (one step) or d_cflacc[k] (several), read once at the end.
 int Ctx::advance(double t, double dt, double* maxcfl)
 {
+    return advance_n(t, dt, 1, maxcfl);
+}
+
+int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl)
+{
+    if (nstep < 1 || nstep > ADVANCE_MAXSTEPS) return set_error(HIPCLAW_ERR_ARG, "advance: nstep out of range");
     // fine steps of the finest level per coarse step (the step counters of
-    // advance_body); a coarse step of one fine step is a handful of launches,
-    // for which launching a graph costs more than it saves
+    // advance_body); a single coarse step of one fine step is a handful of
+    // launches, for which launching a graph costs more than it saves
     const int flo = forest.global_minlevel(), fhi = forest.global_maxlevel();
     const int fine_steps = dom.subcycle ? 1 << (fhi - std::min(dom.minlevel, flo))
                                         : 1 << std::max(dom.maxlevel - dom.minlevel, fhi - flo);
     const bool graph_ok = use_graph && stream != nullptr && !prof && merge_levels &&
-                          (!graph_auto || fine_steps > 1) &&
+                          (!graph_auto || fine_steps * nstep > 1) &&
                           !(swirl_tperiod > 0 && solver_family(phys.solver) == HIPCLAW_SOLVER_ADV &&
                             !fused_swirl());
+    double* const cfl_dev = nstep == 1 ? d_cflacc + 4095 : d_cflacc;
+    auto steps = [&]() -> int {
+        double tk = t;
+        for (int k = 0; k < nstep; k++) {
+            HC_TRY(advance_body(tk, dt));
+            if (!par_dry) HC_TRY(cfl_final(d_cflslot, cflacc_count, cfl_dev + k, stream));
+            tk += dt;
+        }
+        return HIPCLAW_OK;
+    };
     if (cflslot_dirty) HC_TRY(clear_cfl_slots());
     cflslot_dirty = true;
     if (!graph_ok) {
-        HC_TRY(advance_body(t, dt));
-        HC_TRY(cfl_final(d_cflslot, cflacc_count, d_cflacc + 4095, stream));
+        HC_TRY(steps());
     } else {
         HC_TRY(ensure_par());
-        // host parameters of this coarse step into a free ring slot
+        // host parameters of these coarse steps into a free ring slot
         const int slot = par_slot;
         par_slot = (par_slot + 1) % PAR_RING;
         if (par_ev_used[slot]) HC_TRY(check_hip(hipEventSynchronize(par_ev[slot]), "parameter slot"));
@@ -803,7 +820,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
         par_n = 0;
         par_dry = true;
         const int cnt0 = cflacc_count;
-        int rc = advance_body(t, dt);
+        int rc = steps();
         par_dry = false;
         cflacc_count = cnt0;
         const int nlaunch = par_n;
@@ -820,13 +837,13 @@ int Ctx::advance(double t, double dt, double* maxcfl)
             return rc;
         }
         par_ev_used[slot] = true;
-        // Key: pool parities, mesh.  The ghost-push flags at entry do not
-        // change the recorded work: the first fine step updates every level,
-        // which sets them, before any ghost fill reads them.
+        // Key: pool parities, mesh, steps.  The ghost-push flags at entry do
+        // not change the recorded work: the first fine step updates every
+        // level, which sets them, before any ghost fill reads them.
         const unsigned pin = level_bits(parity), qin = level_bits(pushed);
         GraphEntry* e = nullptr;
         for (auto& g : graphs)
-            if (g.parity_in == pin && g.epoch == mesh_epoch) e = &g;
+            if (g.parity_in == pin && g.epoch == mesh_epoch && g.nstep == nstep) e = &g;
         if (!e) {
             // capture: the body runs its host-side bookkeeping as usual while the
             // device work is recorded (its kernels read d_par); restore the
@@ -839,8 +856,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
             par_n = 0;
             rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
             if (!rc) {
-                rc = advance_body(t, dt);
-                if (!rc) rc = cfl_final(d_cflslot, cflacc_count, d_cflacc + 4095, stream);
+                rc = steps();
                 if (!rc && par_n != nlaunch) rc = set_error(HIPCLAW_ERR_STATE, "graph schedule mismatch");
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(stream, &g);
@@ -855,7 +871,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
                         (void)hipGraphExecDestroy(graphs.front().exec);
                         graphs.erase(graphs.begin());
                     }
-                    graphs.push_back({pin, qin, mesh_epoch, ex, level_bits(parity), level_bits(pushed),
+                    graphs.push_back({pin, qin, mesh_epoch, nstep, ex, level_bits(parity), level_bits(pushed),
                                       cell_updates - cells0, cflacc_count});
                     e = &graphs.back();
                     graph_captures++;
@@ -882,7 +898,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
     }
     cflslot_dirty = false;
     if (maxcfl) {
-        HC_TRY(check_hip(hipMemcpyAsync(maxcfl, d_cflacc + 4095, sizeof(double), hipMemcpyDeviceToHost, stream),
+        HC_TRY(check_hip(hipMemcpyAsync(maxcfl, cfl_dev, sizeof(double) * nstep, hipMemcpyDeviceToHost, stream),
                          "cfl D2H"));
         HC_TRY(check_hip(hipStreamSynchronize(stream), "advance"));
     }
@@ -1725,6 +1741,15 @@ int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl)
     CTX_READY(ctx);
     if (!(dt > 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be > 0");
     return ctx->c->advance(t, dt, maxcfl);
+}
+
+int hipclaw_advance_steps(hipclaw_ctx* ctx, double t, double dt, int nsteps, double* maxcfl)
+{
+    CTX_READY(ctx);
+    if (!(dt > 0)) return set_error(HIPCLAW_ERR_ARG, "dt must be > 0");
+    if (nsteps < 1 || nsteps > Ctx::ADVANCE_MAXSTEPS)
+        return set_error(HIPCLAW_ERR_ARG, "nsteps must be 1..256");
+    return ctx->c->advance_n(t, dt, nsteps, maxcfl);
 }
 
 int hipclaw_set_tagging(hipclaw_ctx* ctx, int ivar, double refine_threshold, double coarsen_threshold)

@@ -143,6 +143,10 @@ struct Ctx {
     }
     int timeinterp(int level, double alpha);
     int advance(double t, double dt, double* maxcfl);
+    // nstep coarse steps of one dt (t, t + dt, ... accumulated as the run
+    // loop does); maxcfl[k] = the Courant number of step k
+    int advance_n(double t, double dt, int nstep, double* maxcfl);
+    static constexpr int ADVANCE_MAXSTEPS = 256;
     int advance_body(double t, double dt);
     // hipGraph of a whole coarse step, replayed when dt, the level pool
     // parities, the ghost-push state, the launch shape and the mesh match
@@ -152,6 +156,7 @@ struct Ctx {
     struct GraphEntry {
         unsigned parity_in, pushed_in;
         long long epoch;
+        int nstep;
         hipGraphExec_t exec;
         unsigned parity_out, pushed_out;
         long long cells;

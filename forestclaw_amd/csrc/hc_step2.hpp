@@ -162,6 +162,11 @@ template <int CPL, class T> __device__ inline T from_right(const T (&v)[CPL], in
 #endif
 template <class R, int CPL, bool SYNC>
 constexpr int STEP_UNR = ((SYNC && sizeof(R) == 8) || (sizeof(R) == 8 && CPL == 2)) ? 1 : HC_STEP_UNROLL;
+// HC_STEP_PREFETCH: the unrolled kernels load tile row J+1 while row J is
+// computed (one row of q in flight per lane) instead of at the start of row J.
+#ifndef HC_STEP_PREFETCH
+#define HC_STEP_PREFETCH 1
+#endif
 template <class R, int CPL, bool SYNC>
 constexpr int STEP_WPE = (!SYNC && (sizeof(R) == 4 || CPL == 1)) ? HC_STEP_MIN_WAVES : 1;
 
@@ -173,6 +178,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 {
     using S = SolverT<R>;
     constexpr int ME = S::ME, MW = S::MW;
+    constexpr bool PF = UNR == 4 && HC_STEP_PREFETCH;
     using Face = typename S::Face;
     using Trans = typename S::Trans;
     // FP32 sums over a wave's components skip those that are zero by
@@ -400,12 +406,39 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         // row from the per-lane tile base instead of one pointer induction
         // variable per unrolled row and field kept live across the loop
         const int Jo = __builtin_amdgcn_readfirstlane(J);
+        if constexpr (PF) {
+            // row J was loaded during row J-1 (or before the loop); row J+1
+            // goes into the slot of row J-3, which no row from J on reads, so
+            // its load latency is hidden behind this row's arithmetic
+            // (loaded unconditionally and zeroed by a select: without
+            // branches around the loads the wait for row J counts exactly the
+            // loads issued after it).  Every address is valid: a lane without
+            // a patch reads tile 0 of the launch (patch 0: slot slot0 of a
+            // merged launch, whose q pool pointer step2_launch sets for every
+            // level, or tile 0 of a batch, which it requires to exist); rows
+            // past the sweep read row Jlast (<= NY-1) of the tile; lanes past
+            // the tile's columns read column 0.
+            constexpr int SN = (P + 1) & 3;
+            const bool nin = J + 1 <= Jlast;
+            const int Jn = nin ? Jo + 1 : Jlast;
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int cc = cv[k] ? col[k] : 0;
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    const R v = qp[((long long)m * NY + Jn) * NX + cc];
+                    qr[SN][k][m] = (cv[k] && nin) ? v : R(0);
+                }
+            }
+        }
         R vcur[CPL], vnext[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
+            if constexpr (!PF) {
 #pragma unroll
-            for (int m = 0; m < ME; m++)
-                qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
+                for (int m = 0; m < ME; m++)
+                    qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
+            }
             if constexpr (S::MAUX > 0) {
                 if (A.swirl && sizeof(R) == 4) {
                     const float b0 = rowin ? swyf[Jo] : 0.0f, db = rowin ? swyf[NY + Jo] : 0.0f;
@@ -961,6 +994,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     };
 
     if constexpr (UNR == 4) {
+        if constexpr (PF) {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+#pragma unroll
+                for (int m = 0; m < ME; m++)
+                    qr[0][k][m] = cv[k] ? qp[((long long)m * NY + J0) * NX + col[k]] : R(0);
+            }
+        }
         for (int J = J0; J <= Jlast; J += 4) {
             row(std::integral_constant<int, 0>{}, J);
             row(std::integral_constant<int, 1>{}, J + 1);

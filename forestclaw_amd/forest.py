@@ -284,6 +284,14 @@ class Hipclaw46:
         self.count_amr_advance += 1
         return c.value
 
+    def advance_steps(self, t: float, dt: float, nsteps: int) -> np.ndarray:
+        """nsteps fixed-dt coarse steps from t (one graph launch when graphs
+        are on); the maximum Courant number of each, read back once."""
+        c = np.zeros(nsteps)
+        _check(_lib().hipclaw_advance_steps(self.h, t, dt, nsteps, c.ctypes.data_as(_DP)))
+        self.count_amr_advance += nsteps
+        return c
+
     def regrid(self, init_flag: int = 0) -> bool:
         ch = ctypes.c_int()
         _check(_lib().hipclaw_regrid(self.h, init_flag, ctypes.byref(ch)))
@@ -319,11 +327,14 @@ class Hipclaw46:
         return ms.value, nl.value, nc.value
 
     def run(self, tfinal: float, nout: int = 1, dt0: float | None = None, max_steps: int = 10**9,
-            on_step=None):
+            on_step=None, batch: int = 64):
         """fclaw2d_run outstyle 1 (src/fclaw2d_run.c:119-297): variable dt from the
         CFL number unless use_fixed_dt; reject and retake a step whose maxcfl
         exceeds max_cfl (the solution is restored from the device copy that
         save_step made before the step, src/fclaw2d_run.c:155-158, 220-236).
+        With use_fixed_dt no step depends on the previous one's Courant number,
+        so the steps up to the next output time or regrid (at most batch, and
+        one at a time when on_step is given) go to the device as one call.
         self.retakes counts rejected steps; self.dt_history records every
         attempted (dt, Courant number)."""
         o = self.opt
@@ -331,9 +342,27 @@ class Hipclaw46:
         t0 = self.t
         dt_outer = (tfinal - t0) / nout
         steps = 0
+        regrid_on = o.regrid_interval > 0 and o.maxlevel > o.minlevel
         for n in range(nout):
             tend = t0 + (n + 1) * dt_outer
             while self.t < tend and steps < max_steps:
+                if o.use_fixed_dt and on_step is None and batch > 1:
+                    # the steps the loop below would take before leaving
+                    # the interval, reaching max_steps or a regrid
+                    k, tt = 0, self.t
+                    while k < min(batch, max_steps - steps) and tt < tend:
+                        k += 1
+                        tt += dt_min
+                        if regrid_on and (self.n_inner + k) % o.regrid_interval == 0:
+                            break
+                    cfls = self.advance_steps(self.t, dt_min, k)
+                    self.dt_history.extend((dt_min, float(c)) for c in cfls)
+                    self.n_inner += k
+                    steps += k
+                    self.t = tt
+                    if regrid_on and self.n_inner % o.regrid_interval == 0:
+                        self.regrid()
+                    continue
                 if not o.use_fixed_dt:
                     self.save_step()
                 dt_step = dt_min
@@ -358,7 +387,6 @@ class Hipclaw46:
                     dt_min = dt_min * o.desired_cfl / cfl
                 if on_step:
                     on_step(self, cfl, dt_step)
-                if o.regrid_interval > 0 and self.n_inner % o.regrid_interval == 0 and \
-                        o.maxlevel > o.minlevel:
+                if regrid_on and self.n_inner % o.regrid_interval == 0:
                     self.regrid()
         return dt_min

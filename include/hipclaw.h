@@ -399,6 +399,14 @@ int hipclaw_timeinterp(hipclaw_ctx* ctx, int level, double alpha);
  * of size dt with subcycling, intermediate and final ghost updates; returns
  * the maximum Courant number over all level updates. */
 int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl);
+/* nsteps (1..256) coarse steps of one dt, as the fixed-dt run loop takes them
+ * between outputs and regrids (src/fclaw2d_run.c:150-245 with use_fixed_dt:
+ * no save, no retake): step k starts at t + dt + ... + dt (k additions, as
+ * the loop accumulates t_curr) and maxcfl[k] (NULL: not returned) is its
+ * maximum Courant number, all read back once at the end.  Bitwise the same
+ * as nsteps calls of hipclaw_advance; the steps are one hipGraph launch when
+ * graphs are on. */
+int hipclaw_advance_steps(hipclaw_ctx* ctx, double t, double dt, int nsteps, double* maxcfl);
 
 /* Tagging variable (0-based field ivar, the reference's threshold-variable
  * minus 1) and thresholds of the regrid tag pass. */

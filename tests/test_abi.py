@@ -88,3 +88,12 @@ def test_launch_shape_defaults_and_tuned_shapes():
     assert H.launch_shape(p8, 10) == (1, 4, 4)
     with pytest.raises(H.HipclawError):
         H.launch_shape(p64, 10, (1, 1, -2))
+
+
+def test_patch_list_over_no_tiles_rejected():
+    """A launch reads tile 0 from lanes that own no patch, so a patch list
+    over buffers of no tiles is rejected before any device call."""
+    p = H.physics(H.SOLVER_SWE_ROE, 32, 32)
+    ids = (ctypes.c_int * 2)(0, 1)
+    with pytest.raises(H.HipclawError, match="ntiles"):
+        H.step2_batch(p, 2, 1, 1, None, 1, 0.1, patch_ids=ctypes.addressof(ids), ntiles=0, stream=0)

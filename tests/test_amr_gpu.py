@@ -476,3 +476,64 @@ def test_rejected_step_is_retaken_from_the_device_copy(dev, precision):
     assert b.dt_history == a.dt_history[1:]
     assert np.array_equal(a.get_q(), b.get_q())
     assert a.t == b.t
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+@pytest.mark.parametrize("case", ["swirl_uniform", "swe_time_sync"])
+def test_advance_steps_equal_single_steps(dev, precision, case):
+    """hipclaw_advance_steps (the fixed-dt steps between outputs in one call,
+    one graph launch when graphs are on) gives the bits and the per-step
+    Courant numbers of one hipclaw_advance per step, for the uniform swirl
+    mesh (one fine step per coarse step, time-dependent velocities) and a
+    subcycled SWE mesh with time sync; the run loop with use_fixed_dt takes
+    its steps in batches and ends where step-by-step calls end."""
+    import torch
+    stream = torch.cuda.Stream()
+    sims = []
+    for mode in ("single", "eager", "graph"):
+        if case == "swirl_uniform":
+            opt = F.Options(minlevel=3, maxlevel=3, mx=8, my=8, solver=H.SOLVER_ADV, precision=precision)
+            sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+            sim.set_swirl(8.0)
+            sim.initialize("swirl", (0.5,))
+        else:
+            opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_SWE_ROE, precision=precision,
+                            refine_threshold=0.5, coarsen_threshold=0.05, time_sync=True)
+            sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+            sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        sim.set_feature(H.FEATURE_GRAPH, mode == "graph")
+        sims.append(sim)
+    if case == "swe_time_sync":
+        assert len(set(sims[0].patch_info()["level"])) >= 2
+    dt = 0.004 if case != "swirl_uniform" else 0.9 / 64
+    ts = [0.0]  # step start times, accumulated as the run loop does
+    for _ in range(15):
+        ts.append(ts[-1] + dt)
+    c_single = [sims[0].advance(ts[k], dt) for k in range(7)]
+    for sim in sims[1:]:
+        c = np.concatenate([sim.advance_steps(ts[0], dt, 4), sim.advance_steps(ts[4], dt, 3)])
+        assert np.array_equal(c, np.array(c_single)), case
+        assert np.array_equal(sim.get_q(), sims[0].get_q()), case
+    caps, reps = sims[2].graph_stats()
+    assert caps == 2 and reps == 0, (caps, reps)
+    # two more calls of 4 steps: the second starts in the pool-parity state
+    # of the first and replays its graph
+    c_single = [sims[0].advance(ts[k], dt) for k in range(7, 15)]
+    for sim in sims[1:]:
+        c = np.concatenate([sim.advance_steps(ts[7], dt, 4), sim.advance_steps(ts[11], dt, 4)])
+        assert np.array_equal(c, np.array(c_single)), case
+        assert np.array_equal(sim.get_q(), sims[0].get_q()), case
+    caps, reps = sims[2].graph_stats()
+    assert caps == 3 and reps == 1, (caps, reps)
+    # the fixed-dt run loop: batched and one step at a time end equal
+    runs = []
+    for batch in (1, 5):
+        opt = F.Options(minlevel=3, maxlevel=3, mx=8, my=8, solver=H.SOLVER_ADV, precision=precision,
+                        use_fixed_dt=True)
+        sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+        sim.set_swirl(8.0)
+        sim.initialize("swirl", (0.5,))
+        sim.run(13 * 0.01, nout=2, dt0=0.01, batch=batch)
+        runs.append((sim.get_q(), sim.t, list(sim.dt_history), sim.n_inner))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1:] == runs[1][1:]
