@@ -10,11 +10,13 @@ A step is one coarse step (fclaw2d_advance_all_levels): every level update
 at its subcycled times, time interpolation and all ghost updates.  Regrid is
 not part of a step.  The headline value is the batched FP32 path (one step
 launch per subcycled time for all due levels, same-level ghost cells pushed
-by the step kernel, coarse steps replayed from a hipGraph).  The FP64 path
-runs the same steps from the same data twice: batched like the FP32 run, and
-plain (one launch per level step, ghost copies by the ghost kernel, eager
-launches).  Both are bitwise equal to the CPU oracle in the parity tests, so
-the FP64 run is also the full-size oracle that gives the FP32 error.  The CPU baseline times the reference's own Fortran patch step
+by the step kernel, the fixed-dt coarse steps in one hipclaw_advance_steps
+call replayed from a hipGraph).  The FP64 path runs the same steps from the
+same data twice: batched like the FP32 run, and plain (one call and one
+launch per level step, ghost copies by the ghost kernel, eager launches).
+Both are bitwise equal to the CPU oracle in the parity tests, so the FP64
+run is also the full-size oracle that gives the FP32 error.  The CPU
+baseline times the reference's own Fortran patch step
 (built into oracle/_ref) on a bounded sample of the same patches, one
 process per host core, with the oracle's C restatement under OpenMP beside it.
 
@@ -95,27 +97,49 @@ def run_steps(sim, nsteps, dt, t0=0.0):
 
 
 def timed(sim, args, dt, graph, plain=False):
-    """Warm-up steps, then args.steps timed coarse steps (graph replays when
-    graph is set), then the same number of eager steps with HIP events around
-    every patch-step launch for the kernel time.  plain: the unbatched path
-    (one step launch per level and level step, same-level ghost cells copied
-    by the ghost kernel, eager launches)."""
+    """Warm-up steps, then args.steps timed coarse steps, then the same number
+    of eager steps with HIP events around every patch-step launch for the
+    kernel time.  Batched: the fixed-dt steps go to the device as
+    hipclaw_advance_steps calls of up to 256 steps (what the fixed-dt run
+    loop does, Hipclaw46.run), replayed from a hipGraph when graph is set; the
+    graph of the timed call is captured after the warm-up, untimed
+    (hipclaw_prepare_steps).  plain: the unbatched path (one call and one
+    step launch per level and level step, same-level ghost cells copied by
+    the ghost kernel, eager launches, the Courant number read back after
+    every coarse step)."""
     import torch
     from forestclaw_amd import hipclaw as H
     sim.set_feature(H.FEATURE_GRAPH, graph and not plain)
     sim.set_feature(H.FEATURE_GHOST_PUSH, not (args.no_push or plain))
     sim.set_feature(H.FEATURE_MERGE_LEVELS, not (args.no_merge or plain))
-    run_steps(sim, args.warmup, dt)
+    chunk = min(args.steps, 256)
+    calls = [chunk] * (args.steps // chunk) + ([args.steps % chunk] if args.steps % chunk else [])
+    t = 0.0
+    if plain:
+        t = run_steps(sim, args.warmup, dt)
+    elif args.warmup > 0:
+        sim.advance_steps(t, dt, args.warmup)
+        for _ in range(args.warmup):
+            t += dt
+    if not plain:
+        sim.prepare_steps(chunk)
     c0 = sim.cell_updates()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    t = run_steps(sim, args.steps, dt, t0=args.warmup * dt)
+    if plain:
+        t = run_steps(sim, args.steps, dt, t0=t)
+    else:
+        for k in calls:
+            sim.advance_steps(t, dt, k)
+            for _ in range(k):
+                t += dt
+        torch.cuda.synchronize()
     t1 = time.perf_counter()
     cells = sim.cell_updates() - c0
     caps, reps = sim.graph_stats()
     # kernel timing pass (eager, events on the context's stream)
     sim.profile(True)
-    run_steps(sim, args.steps, dt, t0=t)
+    t = run_steps(sim, args.steps, dt, t0=t)
     kms, launches, kcells = sim.profile_read()
     sim.profile(False)
     return dict(seconds=t1 - t0, cells=cells, kernel_ms=kms, launches=launches, kcells=kcells,

@@ -712,9 +712,9 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
     const bool push = use_push();
     ml.push = push ? d_push : nullptr;
     ml.par = par_dev;
-    // small launches take their Courant-number maximum by atomics in the
-    // step kernel (one launch fewer per fine step); large ones by a reduction
-    const bool cfl_atomic = cflacc_count < 4095 && np < CFL_ATOMIC_MAX;
+    // the launch's Courant-number maximum is taken by atomics in the step
+    // kernel (one per wave): no reduction launch per fine step
+    const bool cfl_atomic = cflacc_count < 4095;
     ml.cflmax = cfl_atomic ? d_cflslot + cflacc_count : nullptr;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
@@ -783,7 +783,7 @@ int Ctx::advance(double t, double dt, double* maxcfl)
     return advance_n(t, dt, 1, maxcfl);
 }
 
-int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl)
+int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
 {
     if (nstep < 1 || nstep > ADVANCE_MAXSTEPS) return set_error(HIPCLAW_ERR_ARG, "advance: nstep out of range");
     // fine steps of the finest level per coarse step (the step counters of
@@ -806,6 +806,7 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl)
         }
         return HIPCLAW_OK;
     };
+    if (prepare && !graph_ok) return HIPCLAW_OK;  // nothing to capture
     if (cflslot_dirty) HC_TRY(clear_cfl_slots());
     cflslot_dirty = true;
     if (!graph_ok) {
@@ -828,15 +829,17 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl)
             par_host = nullptr;
             return rc;
         }
-        rc = check_hip(hipMemcpyAsync(d_par, par_host, sizeof(double) * (size_t)nlaunch * PAR_STRIDE,
-                                      hipMemcpyHostToDevice, stream),
-                       "parameter upload");
-        if (!rc) rc = check_hip(hipEventRecord(par_ev[slot], stream), "parameter event");
-        if (rc) {
-            par_host = nullptr;
-            return rc;
+        if (!prepare) {
+            rc = check_hip(hipMemcpyAsync(d_par, par_host, sizeof(double) * (size_t)nlaunch * PAR_STRIDE,
+                                          hipMemcpyHostToDevice, stream),
+                           "parameter upload");
+            if (!rc) rc = check_hip(hipEventRecord(par_ev[slot], stream), "parameter event");
+            if (rc) {
+                par_host = nullptr;
+                return rc;
+            }
+            par_ev_used[slot] = true;
         }
-        par_ev_used[slot] = true;
         // Key: pool parities, mesh, steps.  The ghost-push flags at entry do
         // not change the recorded work: the first fine step updates every
         // level, which sets them, before any ghost fill reads them.
@@ -878,12 +881,19 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl)
                 }
             }
             par_host = nullptr;
-            if (rc) {
+            if (rc || prepare) {
+                // failed capture, or nothing to run: back to the state at entry
                 std::memcpy(parity, par0, sizeof(par0));
                 std::memcpy(pushed, push0, sizeof(push0));
                 cell_updates = cells0;
+                cflacc_count = cnt0;
+                cflslot_dirty = false;
                 return rc;
             }
+        } else if (prepare) {
+            par_host = nullptr;
+            cflslot_dirty = false;
+            return HIPCLAW_OK;  // already captured
         } else {
             par_host = nullptr;
             graph_replays++;
@@ -1750,6 +1760,15 @@ int hipclaw_advance_steps(hipclaw_ctx* ctx, double t, double dt, int nsteps, dou
     if (nsteps < 1 || nsteps > Ctx::ADVANCE_MAXSTEPS)
         return set_error(HIPCLAW_ERR_ARG, "nsteps must be 1..256");
     return ctx->c->advance_n(t, dt, nsteps, maxcfl);
+}
+
+int hipclaw_prepare_steps(hipclaw_ctx* ctx, int nsteps)
+{
+    CTX_READY(ctx);
+    if (nsteps < 1 || nsteps > Ctx::ADVANCE_MAXSTEPS)
+        return set_error(HIPCLAW_ERR_ARG, "nsteps must be 1..256");
+    // the schedule does not depend on (t, dt): any positive dt records it
+    return ctx->c->advance_n(0.0, 1.0, nsteps, nullptr, true);
 }
 
 int hipclaw_set_tagging(hipclaw_ctx* ctx, int ivar, double refine_threshold, double coarsen_threshold)

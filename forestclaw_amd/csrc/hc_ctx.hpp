@@ -39,10 +39,6 @@ struct LevelPlan {
     int s0 = 0, s1 = 0;                  // slot range of the level
 };
 
-// step launches of fewer patches than this reduce their Courant numbers by
-// atomics in the step kernel (at most this many atomics on one address)
-constexpr int CFL_ATOMIC_MAX = 2048;
-
 struct Ctx {
     hipclaw_physics phys{};
     hipclaw_domain dom{};
@@ -145,7 +141,9 @@ struct Ctx {
     int advance(double t, double dt, double* maxcfl);
     // nstep coarse steps of one dt (t, t + dt, ... accumulated as the run
     // loop does); maxcfl[k] = the Courant number of step k
-    int advance_n(double t, double dt, int nstep, double* maxcfl);
+    // prepare: capture the graph these steps would replay (graph mode only)
+    // and leave the state as it was, running nothing
+    int advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare = false);
     static constexpr int ADVANCE_MAXSTEPS = 256;
     int advance_body(double t, double dt);
     // hipGraph of a whole coarse step, replayed when dt, the level pool

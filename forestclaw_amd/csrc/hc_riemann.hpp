@@ -62,6 +62,8 @@ template <class R> struct Adv {
     // component m of wave w is zero by construction (none here)
     template <int DIR> __device__ static constexpr bool wz(int, int) { return false; }
     static constexpr bool RPT2_SCALED = false;
+    static constexpr bool SMAX = false;
+    __device__ static inline R smax(const Face&, const Trans&) { return R(0); }
     template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     template <int DIR>
@@ -143,7 +145,8 @@ template <class R, bool C5> struct SweT {
     // square root per state gives sqrt(h) = h r, hu/sqrt(h) = hu r,
     // u = hu r^2 and sqrt(g h) = sqrt(g) sqrt(h); one of g hbar gives both
     // a = g hbar t and 1/(2a) = t/2; the shear wave's two zero components are
-    // left out of the sums.
+    // left out of the sums; the transonic entropy-fix fractions divide by a
+    // positive difference with the hardware reciprocal.
     template <int DIR>
     __device__ static inline void rpn2_f32(const R* qa, const R* qb, const PhysParams& pp, Face& f,
                                            Trans& roe)
@@ -189,7 +192,7 @@ template <class R, bool C5> struct SweT {
             const R s1 = hu1 * (r1 * r1) - sg * (h1 * r1);
             R sfract;
             if (sa < R(0) && s1 > R(0))
-                sfract = sa * ((s1 - s0) / (s1 - sa));
+                sfract = sa * ((s1 - s0) * __builtin_amdgcn_rcpf(s1 - sa));  // s1 - sa > 0
             else if (s0 < R(0))
                 sfract = s0;
             else
@@ -202,7 +205,7 @@ template <class R, bool C5> struct SweT {
             const R r3 = __builtin_amdgcn_rsqf(h3);
             const R s3 = hu3 * (r3 * r3) + sg * (h3 * r3);
             if (s3 < R(0) && sb > R(0))
-                sfract = s3 * ((sb - s2) / (sb - s3));
+                sfract = s3 * ((sb - s2) * __builtin_amdgcn_rcpf(sb - s3));  // sb - s3 > 0
             else if (s2 < R(0))
                 sfract = s2;
             else
@@ -221,6 +224,11 @@ template <class R, bool C5> struct SweT {
     // FP32 split scaled by sc (the step's dt/(2 dx)): sc enters the three
     // wave coefficients instead of the six outputs
     static constexpr bool RPT2_SCALED = sizeof(R) == 4;
+    // FP32: the largest |s| of a face, |u| + a (u the normal Roe velocity):
+    // |fl(u - a)|, |u| and |fl(u + a)| are at most fl(|u| + a), and one of them
+    // equals it, so dt/dx (|u| + a) is the maximum of dt/dx |s_w| bit for bit
+    static constexpr bool SMAX = sizeof(R) == 4;
+    __device__ static inline R smax(const Face&, const Trans& roe) { return M<R>::abs_(roe.u) + roe.a; }
     template <int DIR>
     __device__ static inline void rpt2_scaled(const R* asdq, const Trans& roe, R sc, R* bm, R* bp)
     {
@@ -404,6 +412,8 @@ template <class R, bool C5> struct Euler5T {
         return w == 4 ? m != 4 : (m == 4 || (w == 1 && (m == 0 || m == (DIR == 0 ? 1 : 2))));
     }
     static constexpr bool RPT2_SCALED = false;
+    static constexpr bool SMAX = false;
+    __device__ static inline R smax(const Face&, const Trans&) { return R(0); }
     template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     template <int DIR>
@@ -576,6 +586,8 @@ template <class R> struct EulerHlle {
     __device__ static inline void cell(const R*, const PhysParams&, R*) {}
     template <int DIR> __device__ static constexpr bool wz(int, int) { return false; }
     static constexpr bool RPT2_SCALED = false;
+    static constexpr bool SMAX = false;
+    __device__ static inline R smax(const Face&, const Trans&) { return R(0); }
     template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
 
     // normal flux, normal velocity and sound speed of one state

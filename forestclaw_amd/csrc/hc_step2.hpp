@@ -74,7 +74,8 @@ struct StepArgs {
     const double* par;
     // optional launch maximum of the patch Courant numbers, kept as the bits
     // of a non-negative double (integer order = double order) by atomic max
-    // of every positive patch value; the caller clears it beforehand
+    // of every positive value (one per wave, or per patch with row bands);
+    // the caller clears it beforehand
     unsigned long long* cflmax;
     // Row bands (nband > 1, q_out != q_in): a workgroup of
     // nband waves takes the same ppw patches, wave b the interior rows
@@ -578,6 +579,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 // Courant number on faces i = 1 .. mx+1 of every swept row
                 // (FP32: max(dt/dx s, -dt/dx s) as dt/dx |s|, dt/dx > 0)
                 if (cv[k] && col[k] >= 2 && col[k] <= NX - 2 && J <= Jlast) {
+                    if constexpr (S::SMAX) cfl[k] = dmax(cfl[k], dtdx * S::smax(fx[k], tx[k]));
+                    else
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         if constexpr (sizeof(R) == 4) {
@@ -757,6 +760,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 else cdm[0] = R(0);
                 S::template rpn2<1>(qm1[k], qc[k], cdm, cdc[k], vcur[k], pp, fyn[k], tyn[k]);
                 if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2 && J > J0 && J <= Jlast) {
+                    if constexpr (S::SMAX) cfl[k] = dmax(cfl[k], dtdy * S::smax(fyn[k], tyn[k]));
+                    else
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         if constexpr (sizeof(R) == 4) {
@@ -1127,9 +1132,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             if (A.cflmax && m > 0.0) atomicMax(A.cflmax, (unsigned long long)__double_as_longlong(m));
         }
     } else {
-        if (pact && lp == A.lpp - 1 && cflp) {
-            cflp[tix] = cd;
-            if (A.cflmax && cd > 0.0) atomicMax(A.cflmax, (unsigned long long)__double_as_longlong(cd));
+        if (pact && lp == A.lpp - 1 && cflp) cflp[tix] = cd;
+        if (A.cflmax) {
+            // one atomic per wave, of the maximum over its patches
+            double wm = (pact && lp == A.lpp - 1) ? cd : 0.0;
+#pragma unroll
+            for (int d = 1; d < HC_WAVE; d <<= 1) wm = dmax(wm, __shfl_xor(wm, d, HC_WAVE));
+            if (lane == 0 && wm > 0.0) atomicMax(A.cflmax, (unsigned long long)__double_as_longlong(wm));
         }
     }
 }

@@ -292,6 +292,11 @@ class Hipclaw46:
         self.count_amr_advance += nsteps
         return c
 
+    def prepare_steps(self, nsteps: int):
+        """Capture the graph of advance_steps(nsteps) from the current state
+        without running it."""
+        _check(_lib().hipclaw_prepare_steps(self.h, nsteps))
+
     def regrid(self, init_flag: int = 0) -> bool:
         ch = ctypes.c_int()
         _check(_lib().hipclaw_regrid(self.h, init_flag, ctypes.byref(ch)))

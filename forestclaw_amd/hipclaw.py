@@ -148,6 +148,7 @@ _SIGS.update({
     "hipclaw_timeinterp": ([_VP, ctypes.c_int, ctypes.c_double], ctypes.c_int),
     "hipclaw_advance": ([_VP, ctypes.c_double, ctypes.c_double, _DP], ctypes.c_int),
     "hipclaw_advance_steps": ([_VP, ctypes.c_double, ctypes.c_double, ctypes.c_int, _DP], ctypes.c_int),
+    "hipclaw_prepare_steps": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_set_tagging": ([_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double], ctypes.c_int),
     "hipclaw_regrid": ([_VP, ctypes.c_int, _IP], ctypes.c_int),
     "hipclaw_initialize": ([_VP, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int], ctypes.c_int),

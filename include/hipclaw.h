@@ -407,6 +407,12 @@ int hipclaw_advance(hipclaw_ctx* ctx, double t, double dt, double* maxcfl);
  * as nsteps calls of hipclaw_advance; the steps are one hipGraph launch when
  * graphs are on. */
 int hipclaw_advance_steps(hipclaw_ctx* ctx, double t, double dt, int nsteps, double* maxcfl);
+/* Captures the hipGraph that hipclaw_advance_steps(nsteps) would replay from
+ * the current state (mesh, pool parities), without running anything and
+ * leaving the solution and counters unchanged; no effect when graphs are off.
+ * No counterpart in the reference: it moves graph capture out of the first
+ * call (e.g. ahead of a latency-bound serving loop). */
+int hipclaw_prepare_steps(hipclaw_ctx* ctx, int nsteps);
 
 /* Tagging variable (0-based field ivar, the reference's threshold-variable
  * minus 1) and thresholds of the regrid tag pass. */

@@ -537,3 +537,33 @@ def test_advance_steps_equal_single_steps(dev, precision, case):
         runs.append((sim.get_q(), sim.t, list(sim.dt_history), sim.n_inner))
     assert np.array_equal(runs[0][0], runs[1][0])
     assert runs[0][1:] == runs[1][1:]
+
+
+def test_prepare_steps_captures_without_running(dev):
+    """hipclaw_prepare_steps records the graph of the next advance_steps call
+    and leaves solution, counters and pool state as they were; the call that
+    follows replays it and ends bitwise where an eager run ends."""
+    import torch
+    stream = torch.cuda.Stream()
+    sims = []
+    for graph in (False, True):
+        opt = F.Options(minlevel=2, maxlevel=4, mx=8, my=8, solver=H.SOLVER_SWE_ROE,
+                        refine_threshold=0.5, coarsen_threshold=0.05, time_sync=True)
+        sim = F.Hipclaw46(opt, stream=stream.cuda_stream)
+        sim.initialize("radial_dam", SMALL_DAM, nsub=4)
+        sim.set_feature(H.FEATURE_GRAPH, graph)
+        sims.append(sim)
+    q0 = sims[1].get_q()
+    sims[1].prepare_steps(3)
+    sims[1].prepare_steps(3)  # already captured: nothing new
+    assert sims[1].graph_stats() == (1, 0)
+    assert sims[1].cell_updates() == 0
+    assert np.array_equal(sims[1].get_q(), q0)
+    sims[0].prepare_steps(3)  # graphs off: no effect
+    assert sims[0].graph_stats() == (0, 0)
+    dt = 0.004
+    c = [sim.advance_steps(0.0, dt, 3) for sim in sims]
+    assert np.array_equal(c[0], c[1])
+    assert np.array_equal(sims[0].get_q(), sims[1].get_q())
+    assert sims[1].graph_stats() == (1, 1)
+    assert sims[0].cell_updates() == sims[1].cell_updates() > 0
