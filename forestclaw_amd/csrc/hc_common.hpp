@@ -46,6 +46,25 @@ template <class R> __device__ inline R dmax(R a, R b) { return a > b ? a : b; }
 // of a zero).  FP64 keeps the reference's comparisons.
 template <> __device__ inline float dmin<float>(float a, float b) { return __builtin_fminf(a, b); }
 template <> __device__ inline float dmax<float>(float a, float b) { return __builtin_fmaxf(a, b); }
+// min(x, 0), max(x, 0) and the maximum of two non-negative values.  FP32: the
+// integer min / max of the bit patterns (a negative float, -0 included, is a
+// negative integer; non-negative floats order like their bits), one
+// instruction each with no operand quieting: IEEE-mode v_min_f32 / v_max_f32
+// need operands known to be canonical, and values made in another basic
+// block get an extra v_max_f32 x, x first.  FP64: the comparisons above.
+template <class R> __device__ inline R zmin(R x) { return dmin(x, R(0)); }
+template <class R> __device__ inline R zmax(R x) { return dmax(x, R(0)); }
+template <class R> __device__ inline R pmax(R a, R b) { return dmax(a, b); }
+// max(0, x) with the zero first, as the limiters write it (FP64 keeps the
+// comparison order, which decides the sign of a zero result)
+template <class R> __device__ inline R zmax0(R x) { return dmax(R(0), x); }
+template <> __device__ inline float zmin<float>(float x) { return __int_as_float(min(__float_as_int(x), 0)); }
+template <> __device__ inline float zmax<float>(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+template <> __device__ inline float zmax0<float>(float x) { return zmax<float>(x); }
+template <> __device__ inline float pmax<float>(float a, float b)
+{
+    return __int_as_float(max(__float_as_int(a), __float_as_int(b)));
+}
 
 // Whole-wave neighbour exchange (lane l receives lane l-1 / l+1) with the
 // GFX9 DPP wave_shr:1 / wave_shl:1 row controls: one VALU move, no LDS round

@@ -73,8 +73,8 @@ template <class R> struct Adv {
         R delta = qb[0] - qa[0];
         f.w[0][0] = delta;
         f.s[0] = sface;
-        f.am[0] = R(0) + dmin(f.s[0], R(0)) * f.w[0][0];
-        f.ap[0] = R(0) + dmax(f.s[0], R(0)) * f.w[0][0];
+        f.am[0] = R(0) + zmin(f.s[0]) * f.w[0][0];
+        f.ap[0] = R(0) + zmax(f.s[0]) * f.w[0][0];
     }
 
     template <int DIR>
@@ -83,8 +83,8 @@ template <class R> struct Adv {
     {
         R dn = imp == 1 ? t.m_dn : t.p_dn;
         R up = imp == 1 ? t.m_up : t.p_up;
-        bm[0] = dmin(dn, R(0)) * asdq[0];
-        bp[0] = dmax(up, R(0)) * asdq[0];
+        bm[0] = zmin(dn) * asdq[0];
+        bp[0] = zmax(up) * asdq[0];
     }
 };
 
@@ -172,8 +172,8 @@ template <class R, bool C5> struct SweT {
         f.w[1][0] = R(0); f.w[1][mu] = R(0); f.w[1][mv] = a2; f.s[1] = u;
         f.w[2][0] = a3; f.w[2][mu] = a3 * s2; f.w[2][mv] = a3 * v; f.s[2] = s2;
         if (!pp.efix) {
-            const R m0 = dmin(s0, R(0)), m2 = dmin(s2, R(0)), m1 = dmin(u, R(0));
-            const R p0 = dmax(s0, R(0)), p2 = dmax(s2, R(0)), p1 = dmax(u, R(0));
+            const R m0 = zmin(s0), m2 = zmin(s2), m1 = zmin(u);
+            const R p0 = zmax(s0), p2 = zmax(s2), p1 = zmax(u);
             for (int m = 0; m < 3; m++) {
                 f.am[m] = m0 * f.w[0][m] + m2 * f.w[2][m];
                 f.ap[m] = p0 * f.w[0][m] + p2 * f.w[2][m];
@@ -246,8 +246,8 @@ template <class R, bool C5> struct SweT {
         const R a1 = (sp * asdq[0] - asdq[mv]) * i2a;
         const R a3 = (asdq[mv] - sm * asdq[0]) * i2a;
         const R a2 = (asdq[mu] - u * asdq[0]) * sc;
-        const R m1 = dmin(sm, R(0)) * a1, m2 = dmin(v, R(0)) * a2, m3 = dmin(sp, R(0)) * a3;
-        const R p1 = dmax(sm, R(0)) * a1, p2 = dmax(v, R(0)) * a2, p3 = dmax(sp, R(0)) * a3;
+        const R m1 = zmin(sm) * a1, m2 = zmin(v) * a2, m3 = zmin(sp) * a3;
+        const R p1 = zmax(sm) * a1, p2 = zmax(v) * a2, p3 = zmax(sp) * a3;
         bm[0] = m1 + m3; bm[mu] = u * (m1 + m3) + m2; bm[mv] = sm * m1 + sp * m3;
         bp[0] = p1 + p3; bp[mu] = u * (p1 + p3) + p2; bp[mv] = sm * p1 + sp * p3;
     }
@@ -305,8 +305,8 @@ template <class R, bool C5> struct SweT {
             for (int m = 0; m < 3; m++) {
                 R am = R(0), ap = R(0);
                 for (int w = 0; w < 3; w++) {
-                    am = am + dmin(f.s[w], R(0)) * f.w[w][m];
-                    ap = ap + dmax(f.s[w], R(0)) * f.w[w][m];
+                    am = am + zmin(f.s[w]) * f.w[w][m];
+                    ap = ap + zmax(f.s[w]) * f.w[w][m];
                 }
                 f.am[m] = am; f.ap[m] = ap;
             }
@@ -385,8 +385,8 @@ template <class R, bool C5> struct SweT {
         for (int m = 0; m < 3; m++) {
             R b0 = R(0), b1 = R(0);
             for (int w = 0; w < 3; w++) {
-                b0 = b0 + dmin(sb[w], R(0)) * wb[w][m];
-                b1 = b1 + dmax(sb[w], R(0)) * wb[w][m];
+                b0 = b0 + zmin(sb[w]) * wb[w][m];
+                b1 = b1 + zmax(sb[w]) * wb[w][m];
             }
             bm[m] = b0; bp[m] = b1;
         }
@@ -645,8 +645,8 @@ template <class R> struct EulerHlle {
             const R w2 = dq - w1;
             f.w[0][m] = w1;
             f.w[1][m] = w2;
-            f.am[m] = dmin(s1, R(0)) * w1 + dmin(s2, R(0)) * w2;
-            f.ap[m] = dmax(s1, R(0)) * w1 + dmax(s2, R(0)) * w2;
+            f.am[m] = zmin(s1) * w1 + zmin(s2) * w2;
+            f.ap[m] = zmax(s1) * w1 + zmax(s2) * w2;
         }
     }
 
@@ -687,8 +687,8 @@ template <class R> struct EulerHlle {
             const R d = asdq[m];
             const R v1 = MM::divp(t2 * d - bq, t2 - t1);
             const R v2 = d - v1;
-            bm[m] = dmin(t1, R(0)) * v1 + dmin(t2, R(0)) * v2;
-            bp[m] = dmax(t1, R(0)) * v1 + dmax(t2, R(0)) * v2;
+            bm[m] = zmin(t1) * v1 + zmin(t2) * v2;
+            bp[m] = zmax(t1) * v1 + zmax(t2) * v2;
         }
     }
 };
@@ -703,17 +703,17 @@ template <class R> using Euler5C5 = Euler5T<R, true>;
 template <class R> __device__ inline R limiter_phi(int mthlim, R r)
 {
     switch (mthlim) {
-    case 1: return dmax(R(0), dmin(R(1), r));
-    case 2: return dmax(dmax(R(0), dmin(R(1), R(2) * r)), dmin(R(2), r));
+    case 1: return zmax0(dmin(R(1), r));
+    case 2: return dmax(zmax0(dmin(R(1), R(2) * r)), dmin(R(2), r));
     case 3: return M<R>::divp(r + M<R>::abs_(r), R(1) + M<R>::abs_(r));
     case 4: {
         R c = (R(1) + r) / R(2);
-        return dmax(R(0), dmin(dmin(c, R(2)), R(2) * r));
+        return zmax0(dmin(dmin(c, R(2)), R(2) * r));
     }
     case 5: return r;
     case 6: {
         const R th = R((double)1.3f);  // single-precision literal in the reference
-        return dmax(R(0), dmin(dmin(th * r, (R(1) + r) / R(2)), th));
+        return zmax0(dmin(dmin(th * r, (R(1) + r) / R(2)), th));
     }
     default: return R(1);
     }

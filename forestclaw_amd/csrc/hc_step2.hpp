@@ -579,12 +579,12 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 // Courant number on faces i = 1 .. mx+1 of every swept row
                 // (FP32: max(dt/dx s, -dt/dx s) as dt/dx |s|, dt/dx > 0)
                 if (cv[k] && col[k] >= 2 && col[k] <= NX - 2 && J <= Jlast) {
-                    if constexpr (S::SMAX) cfl[k] = dmax(cfl[k], dtdx * S::smax(fx[k], tx[k]));
+                    if constexpr (S::SMAX) cfl[k] = pmax(cfl[k], dtdx * S::smax(fx[k], tx[k]));
                     else
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         if constexpr (sizeof(R) == 4) {
-                            cfl[k] = dmax(cfl[k], dtdx * M<R>::abs_(fx[k].s[w]));
+                            cfl[k] = pmax(cfl[k], dtdx * M<R>::abs_(fx[k].s[w]));
                         } else {
                             R a = dtdx * fx[k].s[w];
                             R b = -dtdx * fx[k].s[w];
@@ -760,12 +760,12 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 else cdm[0] = R(0);
                 S::template rpn2<1>(qm1[k], qc[k], cdm, cdc[k], vcur[k], pp, fyn[k], tyn[k]);
                 if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2 && J > J0 && J <= Jlast) {
-                    if constexpr (S::SMAX) cfl[k] = dmax(cfl[k], dtdy * S::smax(fyn[k], tyn[k]));
+                    if constexpr (S::SMAX) cfl[k] = pmax(cfl[k], dtdy * S::smax(fyn[k], tyn[k]));
                     else
 #pragma unroll
                     for (int w = 0; w < MW; w++) {
                         if constexpr (sizeof(R) == 4) {
-                            cfl[k] = dmax(cfl[k], dtdy * M<R>::abs_(fyn[k].s[w]));
+                            cfl[k] = pmax(cfl[k], dtdy * M<R>::abs_(fyn[k].s[w]));
                         } else {
                             R a = dtdy * fyn[k].s[w];
                             R b = -dtdy * fyn[k].s[w];
