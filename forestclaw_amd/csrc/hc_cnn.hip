@@ -684,6 +684,66 @@ int hipclaw_cnn_forward(hipclaw_cnn* c, const void* tiles, const int* idx, int n
     return check_hip(hipGetLastError(), "cnn head");
 }
 
+int hipclaw_cnn_tag_patches(hipclaw_cnn* c, int n, const double* const* host_q, const double* dx,
+                            int meqn, double threshold, int* tags)
+{
+    if (!c) return set_error(HIPCLAW_ERR_ARG, "cnn is NULL");
+    if (n < 0 || (n > 0 && (!host_q || !dx || !tags))) return set_error(HIPCLAW_ERR_ARG, "NULL argument");
+    if (meqn < 1 || meqn > 16) return set_error(HIPCLAW_ERR_ARG, "meqn must be 1..16");
+    if (!(threshold > 0)) return set_error(HIPCLAW_ERR_ARG, "threshold must be > 0");
+    if (n == 0) return HIPCLAW_OK;
+    int rc = ensure_cap(c, n);
+    if (rc) return rc;
+    // stage the host tiles (32x32 patches, mbc 2, q(i, j, m)) into one device
+    // buffer in slot order, then the same tile-making kernel and network
+    // forward pass as the device-resident regrid
+    const size_t tile = (size_t)NPIX * meqn;
+    std::vector<double> hq(tile * (size_t)n), hdx(2 * (size_t)n);
+    std::vector<int> hslot(n), hlev(n, 0);
+    for (int p = 0; p < n; p++) {
+        if (!host_q[p]) return set_error(HIPCLAW_ERR_ARG, "NULL patch pointer");
+        std::memcpy(hq.data() + tile * p, host_q[p], sizeof(double) * tile);
+        hdx[2 * p] = hdx[2 * p + 1] = dx[p];
+        hslot[p] = p;
+    }
+    double *dq = nullptr, *ddx = nullptr;
+    int *dslot = nullptr, *dlev = nullptr;
+    float* dlog = nullptr;
+    void* dt = nullptr;
+    rc = check_hip(hipMalloc((void**)&dq, sizeof(double) * hq.size()), "tag q");
+    if (!rc) rc = check_hip(hipMalloc((void**)&ddx, sizeof(double) * hdx.size()), "tag dx");
+    if (!rc) rc = check_hip(hipMalloc((void**)&dslot, sizeof(int) * 2 * (size_t)n), "tag slots");
+    if (!rc) rc = check_hip(hipMalloc((void**)&dlog, sizeof(float) * 2 * (size_t)n), "tag logits");
+    if (!rc) rc = check_hip(hipMalloc(&dt, sizeof(bf16) * (size_t)NPIX * C0 * n), "tag tiles");
+    if (!rc) {
+        dlev = dslot + n;
+        rc = check_hip(hipMemcpyAsync(dq, hq.data(), sizeof(double) * hq.size(), hipMemcpyHostToDevice, c->stream),
+                       "tag q");
+        if (!rc) rc = check_hip(hipMemcpyAsync(ddx, hdx.data(), sizeof(double) * hdx.size(), hipMemcpyHostToDevice,
+                                               c->stream), "tag dx");
+        if (!rc) rc = check_hip(hipMemcpyAsync(dslot, hslot.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream),
+                                "tag slots");
+        if (!rc) rc = check_hip(hipMemcpyAsync(dlev, hlev.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream),
+                                "tag levels");
+    }
+    if (!rc) {
+        hc::Pools<double> P{};
+        P.q[0] = dq;
+        rc = hc::launch_make_tiles<double>(dslot, dlev, n, P, ddx, 32, 32, meqn, threshold, dt, c->stream);
+    }
+    if (!rc) rc = hipclaw_cnn_forward(c, dt, nullptr, n, dlog);
+    std::vector<float> lg(2 * (size_t)n);
+    if (!rc) rc = check_hip(hipMemcpyAsync(lg.data(), dlog, sizeof(float) * lg.size(), hipMemcpyDeviceToHost,
+                                           c->stream), "tag logits");
+    if (!rc) rc = check_hip(hipStreamSynchronize(c->stream), "tag sync");
+    if (!rc)
+        for (int p = 0; p < n; p++) tags[p] = (double)lg[2 * p + 1] - (double)lg[2 * p] >= 0.0 ? 1 : 0;
+    void* bufs[] = {dq, ddx, dslot, dlog, dt};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
+}
+
 int hipclaw_cnn_gradients(hipclaw_cnn* c, const void* tiles, const int* labels, const int* idx, int n,
                           float* grad_host, float* loss_host)
 {

@@ -88,6 +88,8 @@ _SIGS = {
     "hipclaw_cnn_set_params": ([_VP, _VP], ctypes.c_int),
     "hipclaw_cnn_get_params": ([_VP, _VP], ctypes.c_int),
     "hipclaw_cnn_forward": ([_VP, _VP, _VP, ctypes.c_int, _VP], ctypes.c_int),
+    "hipclaw_cnn_tag_patches": ([_VP, ctypes.c_int, _VP, _DP, ctypes.c_int, ctypes.c_double, _VP],
+                                ctypes.c_int),
     "hipclaw_cnn_gradients": ([_VP, _VP, _VP, _VP, ctypes.c_int, _VP, _VP], ctypes.c_int),
     "hipclaw_cnn_train_step": ([_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_float, _VP],
                                ctypes.c_int),
@@ -313,6 +315,20 @@ class Cnn:
         _check(lib().hipclaw_cnn_gradients(self.h, _ptr(tiles), _ptr(labels), _ptr(idx), n,
                                            g.ctypes.data, loss.ctypes.data))
         return float(loss[0]), g
+
+    def tag_patches(self, patches, dx, threshold):
+        """Refinement tags of host patches (a list of FP64 (meqn, 36, 36)
+        arrays, 32x32 patches) from the network; returns an int array."""
+        import numpy as np
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in patches]
+        n = len(arrs)
+        ptrs = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in arrs])
+        dxa = np.ascontiguousarray(dx, dtype=np.float64)
+        tags = np.zeros(max(n, 1), dtype=np.int32)
+        meqn = arrs[0].shape[0] if n else 1
+        _check(lib().hipclaw_cnn_tag_patches(self.h, n, ctypes.addressof(ptrs), dxa.ctypes.data_as(_DP),
+                                             meqn, float(threshold), tags.ctypes.data))
+        return tags[:n]
 
     def train_step(self, tiles, labels, idx, n, lr, loss_dev=None):
         _check(lib().hipclaw_cnn_train_step(self.h, _ptr(tiles), _ptr(labels), _ptr(idx), n,

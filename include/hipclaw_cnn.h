@@ -54,6 +54,18 @@ int hipclaw_make_tiles(hipclaw_ctx* ctx, void* tiles_dev, int* tags_dev);
  * except that patches with |p - 0.5| < margin take the plain tag.  net = NULL
  * restores plain tagging.  Coarsening keeps the plain test. */
 int hipclaw_set_tagging_cnn(hipclaw_ctx* ctx, hipclaw_cnn* net, float margin);
+/* Refinement tags of n host-resident patches from the network, for a
+ * ForestClaw that keeps its patches on the host (patch_vt->tag4refinement,
+ * src/patches/clawpatch/fclaw2d_clawpatch.cpp:1373, called per patch from
+ * cb_fclaw2d_regrid_tag4refinement, src/fclaw2d_regrid.c:45-72): the caller
+ * gathers the patches of a regrid pass once and answers each per-patch call
+ * from tags[].  host_q[p]: the patch's q(1-mbc:mx+mbc, 1-mbc:my+mbc, meqn)
+ * array, FP64, 32x32 patches, mbc 2; dx[p] its cell width; threshold the
+ * plain criterion's refine threshold (the tile normalisation of
+ * hipclaw_make_tiles).  tags[p] = 1: refine (p(refine) >= 0.5).  Staging
+ * buffers are allocated per call. */
+int hipclaw_cnn_tag_patches(hipclaw_cnn* net, int n, const double* const* host_q, const double* dx, int meqn,
+                            double threshold, int* tags);
 /* Statistics of the last regrid's tag pass: patches, refine tags of the
  * network, of the plain test, misses (plain refine, network not), false
  * refines, fallbacks. */

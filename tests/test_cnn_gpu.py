@@ -2,9 +2,10 @@
 
 * forward logits and every parameter gradient (conv weights / biases via the
   MFMA weight- and data-gradient kernels, head) against the CPU oracle with
-  the same bfloat16 rounding points (oracle/cnn.py): logits within 2% of
-  their scale, each gradient tensor within 5% relative L2 and cosine >= 0.998
-  (bf16 storage, FP32 vs FP64 accumulation);
+  the same bfloat16 rounding points (oracle/cnn.py): logits within 1e-4 of
+  their scale, the loss within 1e-4 relative, each gradient tensor within
+  1e-4 relative L2 (measured: 1e-5 for the logits, at most 7e-6 for the
+  gradients; what remains is FP32 against FP64 accumulation);
 * the batch index list gathers tiles exactly;
 * Adam training on the GPU separates tiles with a sharp front from smooth
   ones (loss < 0.3, accuracy > 95%).
@@ -55,11 +56,11 @@ def test_forward_and_gradients_match_oracle(dev):
     got = lg.cpu().numpy()
     scale = np.abs(ref).max()
     print(f"\nlogits max abs err {np.abs(got - ref).max():.3e} (scale {scale:.3e})")
-    assert np.abs(got - ref).max() <= 2e-2 * scale + 1e-6
+    assert np.abs(got - ref).max() <= 1e-4 * scale
 
     loss, g = net.gradients(td, _dev(lab, torch.int32))
     rl, rg = N.loss_and_grad(p, tq, lab)
-    assert loss == pytest.approx(rl, rel=2e-2)
+    assert loss == pytest.approx(rl, rel=1e-4)
     off, _ = N.layout()
     for name, (o, shape) in off.items():
         a = g[o:o + int(np.prod(shape))].astype(np.float64)
@@ -67,7 +68,7 @@ def test_forward_and_gradients_match_oracle(dev):
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
         cos = a @ b / max(np.linalg.norm(a) * np.linalg.norm(b), 1e-30)
         print(f"{name}: rel L2 {rel:.3e} cos {cos:.6f}")
-        assert rel <= 5e-2 and cos >= 0.998, name
+        assert rel <= 1e-4 and cos >= 1 - 1e-8, name
 
 
 def test_index_gather(dev):
@@ -103,3 +104,29 @@ def test_training_separates_fronts(dev):
     acc = float(((lg[:, 1] > lg[:, 0]).int().cpu().numpy() == lab).mean())
     print(f"\nloss {hist[0]:.3f} -> {np.mean(hist[-5:]):.3f}, accuracy {acc:.3f}")
     assert np.mean(hist[-5:]) < 0.3 and acc > 0.95
+
+
+def test_tag_patches_of_host_arrays_match_the_regrid_pass(dev):
+    """hipclaw_cnn_tag_patches (the batched tag4refinement hook for a host-
+    resident forest) gives, from host copies of a context's patches, the
+    refine decisions of the context's own pass: its tiles and the network's
+    logits (p(refine) >= 0.5)."""
+    import torch
+    from forestclaw_amd import learned_tagging as LT
+    sim = LT.make_sim(2, 4, precision=H.FP64)
+    sim.initialize("radial_dam", (0.3, -0.2, 0.6, 2.5, 1.0), nsub=4)
+    net = H.Cnn(seed=5)
+    n = sim.num_patches()
+    tiles = torch.empty((n, 36, 36, 8), dtype=torch.bfloat16, device=dev)
+    sim.make_tiles(tiles)
+    lg = torch.empty((n, 2), dtype=torch.float32, device=dev)
+    net.forward(tiles, lg)
+    torch.cuda.synchronize()
+    lgd = lg.cpu().numpy().astype(np.float64)
+    want = (lgd[:, 1] - lgd[:, 0] >= 0).astype(np.int32)
+    q = sim.get_q()
+    info = sim.patch_info()
+    got = net.tag_patches(list(q), info["dx"], LT.THRESHOLD)
+    assert len(set(info["level"])) >= 2 and 0 < want.sum() < n
+    assert np.array_equal(got, want)
+    assert len(net.tag_patches([], [], LT.THRESHOLD)) == 0

@@ -56,3 +56,42 @@ def test_learned_tags_imitate_and_gain(dev):
               f"(regrid 3) {err['cnn3']:.3e}")
         assert err["cnn0"] <= 1.05 * err["plain1"]
         assert err["cnn3"] < err["plain3"]
+
+
+RTOL = 3e-4  # Richardson estimate of h: ~1e-2 at the dam's front, <= 1e-4 away from it
+
+
+def test_richardson_labels(dev):
+    """Labels from the method's own Richardson error estimates as well as the
+    plain tags (tag OR estimate > RTOL): the estimate flags patches the
+    gradient test does not and the other way round, the network learns the
+    combined label (held-out miss and false-refine rates <= 1%), and AMR
+    with it stays within 5% of the plain-AMR solution error."""
+    st = {}
+    data = LT.generate(nruns=16, levels=(3, 4, 5), samples=12, seed=1, richardson_tol=RTOL, stats=st)
+    net = H.Cnn(seed=3)
+    hist = LT.train(net, data, epochs=5, batch=1024, lr=2e-3)
+    tst = {}
+    test = LT.generate(nruns=3, levels=(3, 4, 5), samples=12, seed=99, neg_keep=1.0,
+                       richardson_tol=RTOL, stats=tst)
+    lg = LT.predict(net, test)
+    pred = (lg[:, 1] > lg[:, 0])
+    lab = test.labels == 1
+    miss = float((~pred & lab).sum()) / max(float(lab.sum()), 1.0)
+    false = float((pred & ~lab).sum()) / max(float((~lab).sum()), 1.0)
+    print(f"\nRichardson labels: training label counts {st}; loss {hist[0]:.3f} -> {hist[-1]:.4f}; "
+          f"held-out {tst}: miss rate {miss:.4f}, false-refine rate {false:.5f}")
+    assert st["estimate_only"] > 0 and st["tag_only"] > 0
+    assert miss <= 0.01 and false <= 0.01
+    rng = np.random.default_rng(7)
+    for c in range(2):
+        case = LT.dam_case(rng)
+        href = LT.fine_reference(case, NC)
+        err = {}
+        for name, n in (("plain", None), ("richardson", net)):
+            sim, nreg, _ = LT.run_amr(case, NC, 1, net=n)
+            err[name] = LT.amr_error(sim, href)
+            sim.close()
+        print(f"case {c}: rel L1 error plain {err['plain']:.3e}, learned with Richardson labels "
+              f"{err['richardson']:.3e}")
+        assert err["richardson"] <= 1.05 * err["plain"]
