@@ -3,7 +3,8 @@
 // src/patches/clawpatch/fclaw2d_clawpatch_output_ascii.c, with the Fortran
 // writers of src/patches/clawpatch/fort_4.6/fclaw2d_clawpatch46_output_ascii.f),
 // so that VisClaw / Matlab readers and frame diffs work on this solver's
-// output.  Host code only: the context's solution is downloaded once per frame.
+// output, and the frame diff of two runs (hipclaw_compare_frames).  Host code
+// only: the context's solution is downloaded once per frame.
 //
 // Record layouts (Fortran edit descriptors of the reference):
 //   fort.tXXXX  time E30.20, then meqn, ngrids, num_aux, num_dim (I5) with labels
@@ -154,6 +155,190 @@ int hipclaw_ascii_patch(const char* dir, int iframe, int mx, int my, int meqn, i
     std::string out;
     patch_text(out, mx, my, meqn, mbc, xlower, ylower, dx, dy, q, patch_num, level, blockno, mpirank);
     return write_file(frame_name(dir, 'q', iframe), out, "a");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Frame diff (scripts/compare_files/compare_files.f90): two runs' frames of
+// the same mesh, cell by cell.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Lines {
+    std::vector<std::string> v;
+    size_t k = 0;
+    bool load(const std::string& path)
+    {
+        FILE* f = std::fopen(path.c_str(), "r");
+        if (!f) return false;
+        std::string cur;
+        int c;
+        while ((c = std::fgetc(f)) != EOF) {
+            if (c == '\n') {
+                v.push_back(cur);
+                cur.clear();
+            } else {
+                cur += (char)c;
+            }
+        }
+        if (!cur.empty()) v.push_back(cur);
+        std::fclose(f);
+        return true;
+    }
+    // the next record with a value (list-directed input skips blank records)
+    const char* next()
+    {
+        while (k < v.size()) {
+            const std::string& l = v[k++];
+            if (l.find_first_not_of(" \t\r") != std::string::npos) return l.c_str();
+        }
+        return nullptr;
+    }
+};
+
+// one value of an E-edit field: [-]0.dddE[+-]ee or [-]0.ddd[+-]eee
+bool read_real(const char*& p, double& x)
+{
+    while (*p == ' ' || *p == '\t' || *p == ',') p++;
+    if (!*p) return false;
+    char* e = nullptr;
+    x = std::strtod(p, &e);
+    if (e == p) return false;
+    if ((*e == '+' || *e == '-') && e[1] >= '0' && e[1] <= '9') {  // three-digit exponent, no E
+        char* e2 = nullptr;
+        const long ex = std::strtol(e, &e2, 10);
+        x *= std::pow(10.0, (double)ex);
+        e = e2;
+    }
+    p = e;
+    return true;
+}
+
+bool first_int(Lines& L, int& v)
+{
+    const char* l = L.next();
+    if (!l) return false;
+    char* e = nullptr;
+    v = (int)std::strtol(l, &e, 10);
+    return e != l;
+}
+
+bool first_real(Lines& L, double& x)
+{
+    const char* l = L.next();
+    return l && read_real(l, x);
+}
+
+}  // namespace
+
+extern "C" {
+
+int hipclaw_compare_frames(const char* dir1, const char* dir2, const char* dir3, int iframe, int nmax,
+                           double* global_max, int* grid_max)
+{
+    if (iframe < 0 || iframe > 9999) return set_error(HIPCLAW_ERR_ARG, "frame number 0..9999");
+    Lines t1, t2, q1, q2;
+    const std::string ft1 = frame_name(dir1, 't', iframe), ft2 = frame_name(dir2, 't', iframe);
+    const std::string fq1 = frame_name(dir1, 'q', iframe), fq2 = frame_name(dir2, 'q', iframe);
+    if (!t1.load(ft1)) return set_error(HIPCLAW_ERR_ARG, ("could not open file " + ft1).c_str());
+    if (!t2.load(ft2)) return set_error(HIPCLAW_ERR_ARG, ("could not open file " + ft2).c_str());
+    double tm1 = 0, tm2 = 0;
+    int meqn1 = 0, meqn2 = 0, ng1 = 0, ng2 = 0;
+    if (!first_real(t1, tm1) || !first_int(t1, meqn1) || !first_int(t1, ng1) || !first_real(t2, tm2) ||
+        !first_int(t2, meqn2) || !first_int(t2, ng2))
+        return set_error(HIPCLAW_ERR_ARG, "unreadable fort.t file");
+    if (meqn1 != meqn2) return set_error(HIPCLAW_ERR_ARG, "meqn1 .ne. meqn2");
+    if (std::fabs(tm1 - tm2) > 1e-5) return set_error(HIPCLAW_ERR_ARG, "t1 and t2 are not close enough to compare files");
+    if (ng1 != ng2) return set_error(HIPCLAW_ERR_ARG, "ngrids1 .ne. ngrids2");
+    const int meqn = meqn1, ngrids = ng1;
+    if (meqn < 1 || meqn > 10) return set_error(HIPCLAW_ERR_ARG, "meqn must be 1..10 (format 10E26.16)");
+    if (!q1.load(fq1)) return set_error(HIPCLAW_ERR_ARG, ("could not open file " + fq1).c_str());
+    if (!q2.load(fq2)) return set_error(HIPCLAW_ERR_ARG, ("could not open file " + fq2).c_str());
+
+    // fort.tXXXX of the difference (format e18.8 / i5 records, two empty records)
+    std::string tt;
+    put_e(tt, tm1, 18, 8);
+    tt += "    time\n";
+    put_i5(tt, meqn, "                 meqn");
+    put_i5(tt, ngrids, "                 ngrids");
+    put_i5(tt, 0, "                 maux");
+    tt += "\n\n";
+    std::vector<double> gmax(meqn, 0.0), gridmax((size_t)std::max(ngrids, 0) * meqn, 0.0);
+    std::vector<int> gwhere(meqn, 0);
+    std::string qt;
+    for (int n = 0; n < ngrids; n++) {
+        int hd1[6], hd2[6];
+        double geo1[4], geo2[4];
+        for (int k = 0; k < 6; k++)
+            if (!first_int(q1, hd1[k]) || !first_int(q2, hd2[k]))
+                return set_error(HIPCLAW_ERR_ARG, "unreadable grid header");
+        for (int k = 0; k < 4; k++)
+            if (!first_real(q1, geo1[k]) || !first_real(q2, geo2[k]))
+                return set_error(HIPCLAW_ERR_ARG, "unreadable grid header");
+        if (hd1[0] != hd2[0]) return set_error(HIPCLAW_ERR_ARG, "ngrid1 .ne. ngrid2");
+        if (hd1[4] != hd2[4]) return set_error(HIPCLAW_ERR_ARG, "mx1 .ne. mx2");
+        if (geo1[0] != geo2[0]) return set_error(HIPCLAW_ERR_ARG, "xlow1 .ne. xlow2");
+        if (geo1[1] != geo2[1]) return set_error(HIPCLAW_ERR_ARG, "ylow1 .ne. ylow2");
+        const int ngrid = hd1[0], level = hd1[1], blockno = hd1[2], rank = std::max(hd1[3], hd2[3]);
+        const int mx = hd1[4], my = hd1[5];
+        if (mx < 1 || my < 1 || ngrid < 0 || ngrid >= ngrids) return set_error(HIPCLAW_ERR_ARG, "bad grid header");
+        std::vector<double> qc((size_t)meqn * mx * my);
+        for (int m = 0; m < meqn; m++) gridmax[(size_t)ngrid * meqn + m] = 0.0;
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                double a[10], b[10];
+                for (int pass = 0; pass < 2; pass++) {
+                    Lines& L = pass ? q2 : q1;
+                    double* dst = pass ? b : a;
+                    int got = 0;
+                    while (got < meqn) {
+                        const char* l = L.next();
+                        if (!l) return set_error(HIPCLAW_ERR_ARG, "fort.q file ends early");
+                        while (got < meqn && read_real(l, dst[got])) got++;
+                    }
+                }
+                for (int m = 0; m < meqn; m++) {
+                    const double d = a[m] - b[m];
+                    qc[((size_t)m * my + j) * mx + i] = d;
+                    if (std::fabs(d) > gmax[m]) {
+                        gmax[m] = std::fabs(d);
+                        gwhere[m] = ngrid;
+                    }
+                    double& gm = gridmax[(size_t)ngrid * meqn + m];
+                    gm = std::max(gm, std::fabs(d));
+                }
+            }
+        patch_text(qt, mx, my, meqn, 0, geo1[0], geo1[1], geo1[2], geo1[3], qc.data(), ngrid, level, blockno, rank);
+    }
+    // compare_summary.XXXX
+    std::string sm;
+    for (int m = 0; m < meqn; m++) {
+        put_e(sm, gmax[m], 30, 16);
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "     occurs in field%5d on grid %5d\n", m + 1, gwhere[m]);
+        sm += buf;
+    }
+    sm += "  \n";
+    for (int n = 0; n < ngrids; n++) {
+        char buf[16];
+        std::snprintf(buf, sizeof buf, "%5d", n);
+        sm += buf;
+        for (int m = 0; m < meqn; m++) put_e(sm, gridmax[(size_t)n * meqn + m], 16, 4);
+        sm += "\n";
+    }
+    char sname[32];
+    std::snprintf(sname, sizeof sname, "compare_summary.%04d", iframe);
+    const std::string d3 = dir3 && *dir3 ? std::string(dir3) : std::string(".");
+    int rc = write_file(frame_name(dir3, 't', iframe), tt, "w");
+    if (!rc) rc = write_file(frame_name(dir3, 'q', iframe), qt, "w");
+    if (!rc) rc = write_file(d3 + "/" + sname, sm, "w");
+    if (rc) return rc;
+    for (int m = 0; m < meqn && m < nmax; m++) {
+        if (global_max) global_max[m] = gmax[m];
+        if (grid_max) grid_max[m] = gwhere[m];
+    }
+    return HIPCLAW_OK;
 }
 
 }  // extern "C"

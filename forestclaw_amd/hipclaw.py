@@ -163,6 +163,8 @@ _SIGS.update({
     "hipclaw_ascii_patch": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                              _DP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "hipclaw_compare_frames": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                _DP, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "hipclaw_cell_updates": ([_VP, ctypes.POINTER(ctypes.c_longlong)], ctypes.c_int),
     "hipclaw_profile": ([_VP, ctypes.c_int], ctypes.c_int),
     "hipclaw_profile_read": ([_VP, _DP, ctypes.POINTER(ctypes.c_longlong),

@@ -464,6 +464,18 @@ int hipclaw_ascii_header(const char* dir, int iframe, double time, int meqn, int
 int hipclaw_ascii_patch(const char* dir, int iframe, int mx, int my, int meqn, int mbc, double xlower,
                         double ylower, double dx, double dy, const double* q, int patch_num, int level,
                         int blockno, int mpirank);
+/* Frame diff of two runs on the same mesh (scripts/compare_files/
+ * compare_files.f90): reads <dir1>/fort.{t,q}XXXX and <dir2>/fort.{t,q}XXXX,
+ * fails like the reference tool when meqn, the number of grids, a grid's
+ * number, mx, xlow or ylow differ or the times differ by more than 1e-5, and
+ * writes into dir3 the difference frame q1 - q2 (fort.qXXXX in the writers'
+ * format, values below 1e-99 as zero, and fort.tXXXX) and
+ * compare_summary.XXXX (per field the largest |q1 - q2| and the grid where it
+ * first occurs; per grid the largest |q1 - q2| of every field).
+ * global_max / grid_max (nmax entries each, either may be NULL) receive the
+ * per-field maxima and grids.  meqn <= 10, as the reference's format. */
+int hipclaw_compare_frames(const char* dir1, const char* dir2, const char* dir3, int iframe, int nmax,
+                           double* global_max, int* grid_max);
 /* FP64 conservation sums sum(q dx dy) per field (meqn doubles). */
 int hipclaw_mass(hipclaw_ctx* ctx, double* sums);
 /* Cell updates performed so far (count_single_step * mx * my). */
