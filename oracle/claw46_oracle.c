@@ -556,9 +556,60 @@ static void rpt2_hlle(const orc_cfg* c, int ixy, int imp, const double* asdq, sl
     }
 }
 
+/* Conservative transport with f-waves (applications/clawpack/transport/2d/
+ * all/clawpack46_rpn2cons_fw_manifold.f:23-46): edge length g of cell k,
+ * normal velocities on the two sides of face k scaled by it, the f-wave
+ * (flux difference) and the average speed; everything goes one way. */
+static void rpn2_fw(const orc_cfg* c, int ixy, slice_t* sl)
+{
+    (void)c;
+    const int idir = ixy - 1;
+    for (int k = 1; k < sl->N; k++) {
+        const double g = A2(5 + idir, k);
+        const double urrot = g * A2(1 + 2 * idir, k);
+        const double ulrot = g * A2(2 + 2 * idir, k - 1);
+        const double qrr = Q1(0, k), qll = Q1(0, k - 1);
+        const double uhat = (ulrot + urrot) / 2.0;
+        if (uhat >= 0) {
+            AM(0, k) = 0.0;
+            AP(0, k) = urrot * qrr - ulrot * qll;
+        } else {
+            AM(0, k) = urrot * qrr - ulrot * qll;
+            AP(0, k) = 0.0;
+        }
+        WV(0, 0, k) = urrot * qrr - ulrot * qll;
+        SP(0, k) = uhat;
+    }
+}
+
+/* clawpack46_rpt2cons_manifold.f:25-63: the split of face k's fluctuation
+ * with the averages of the transverse normal velocities on the face's two
+ * cells in the neighbouring slices (aux2: this slice, aux3: the next), scaled
+ * by the transverse edge length of cell i1 = k-1 (imp 1) or k (imp 2). */
+static void rpt2_fw(const orc_cfg* c, int ixy, int imp, const double* asdq, slice_t* sl)
+{
+    (void)c;
+    const int idir = 2 - ixy;
+    const int N = sl->N;
+    for (int k = 1; k < N; k++) {
+        const int i1 = k - 2 + imp;
+        double g = A2(5 + idir, i1);
+        double vrrot = g * A2(1 + 2 * idir, k);
+        double vlrot = g * A2(2 + 2 * idir, k - 1);
+        double vhat = (vrrot + vlrot) / 2.0;
+        BM(0, k) = dmin(vhat, 0.0) * asdq[k];
+        g = A3(5 + idir, i1);
+        vrrot = g * A3(1 + 2 * idir, k);
+        vlrot = g * A3(2 + 2 * idir, k - 1);
+        vhat = (vrrot + vlrot) / 2.0;
+        BP(0, k) = dmax(vhat, 0.0) * asdq[k];
+    }
+}
+
 static void rpn2(const orc_cfg* c, int ixy, slice_t* sl)
 {
     switch (c->solver) {
+    case ORC_SOLVER_TRANSPORT_FW: rpn2_fw(c, ixy, sl); break;
     case ORC_SOLVER_ADV: rpn2_adv(c, ixy, sl); break;
     case ORC_SOLVER_SWE_ROE: rpn2_swe(c, ixy, sl); break;
     case ORC_SOLVER_EULER_HLLE: rpn2_hlle(c, ixy, sl); break;
@@ -569,6 +620,7 @@ static void rpn2(const orc_cfg* c, int ixy, slice_t* sl)
 static void rpt2(const orc_cfg* c, int ixy, int imp, const double* asdq, slice_t* sl)
 {
     switch (c->solver) {
+    case ORC_SOLVER_TRANSPORT_FW: rpt2_fw(c, ixy, imp, asdq, sl); break;
     case ORC_SOLVER_ADV: rpt2_adv(c, ixy, imp, asdq, sl); break;
     case ORC_SOLVER_SWE_ROE: rpt2_swe(c, ixy, imp, asdq, sl); break;
     case ORC_SOLVER_EULER_HLLE: rpt2_hlle(c, ixy, imp, asdq, sl); break;
@@ -643,7 +695,14 @@ static double flux1d(const orc_cfg* c, int ixy, double dtdx, slice_t* sl)
 
     rpn2(c, ixy, sl);
 
-    for (int k = 1; k <= N - 2; k++)
+    /* clawpack46_flux2fw.f works on faces i = 1..mx+1 only (slice cells
+     * mbc..n+mbc), which is all the interior update reads; clawpack46_flux2.f
+     * on every face */
+    const int fw = c->solver == ORC_SOLVER_TRANSPORT_FW;
+    const int k0 = fw ? sl->mbc : 1;
+    const int kg = fw ? sl->n + sl->mbc : N - 2;  /* last face of the Godunov fluxes */
+    const int kc = fw ? sl->n + sl->mbc : N - 1;  /* last face of the corrections, splits */
+    for (int k = k0; k <= kg; k++)
         for (int m = 0; m < meqn; m++) {
             FP(m, k) = FP(m, k) - AP(m, k);
             FM(m, k) = FM(m, k) + AM(m, k);
@@ -660,13 +719,16 @@ static double flux1d(const orc_cfg* c, int ixy, double dtdx, slice_t* sl)
 
     if (c->order2 == 2) {
         if (limit) limiter(c, sl);
-        for (int k = 1; k <= N - 1; k++) {
+        for (int k = k0; k <= kc; k++) {
             double dtdxave = 0.5 * (dtdx + dtdx);
             for (int m = 0; m < meqn; m++) {
                 double cq = 0.0;
-                for (int w = 0; w < mw; w++)
-                    cq = cq + fabs(SP(w, k)) * (1.0 - fabs(SP(w, k)) * dtdxave) *
-                                  WV(w, m, k);
+                for (int w = 0; w < mw; w++) {
+                    if (fw) /* dsign(1, s) (1 - |s| dtdx) fwave, clawpack46_flux2fw.f:176-177 */
+                        cq = cq + copysign(1.0, SP(w, k)) * (1.0 - fabs(SP(w, k)) * dtdxave) * WV(w, m, k);
+                    else
+                        cq = cq + fabs(SP(w, k)) * (1.0 - fabs(SP(w, k)) * dtdxave) * WV(w, m, k);
+                }
                 CQ(m, k) = cq;
                 FM(m, k) = FM(m, k) + 0.5 * cq;
                 FP(m, k) = FP(m, k) + 0.5 * cq;
@@ -676,7 +738,7 @@ static double flux1d(const orc_cfg* c, int ixy, double dtdx, slice_t* sl)
 
     if (c->transverse > 0) {
         if (c->order2 > 1 && c->transverse == 2) {
-            for (int k = 1; k <= N - 2; k++)
+            for (int k = k0; k <= (fw ? kc : N - 2); k++)
                 for (int m = 0; m < meqn; m++) {
                     AM(m, k) = AM(m, k) + CQ(m, k);
                     AP(m, k) = AP(m, k) - CQ(m, k);
@@ -684,7 +746,7 @@ static double flux1d(const orc_cfg* c, int ixy, double dtdx, slice_t* sl)
         }
         rpt2(c, ixy, 1, sl->amdq, sl);
         for (int m = 0; m < meqn; m++)
-            for (int k = 1; k <= N - 1; k++) {
+            for (int k = k0; k <= kc; k++) {
                 double gu = 0.5 * dtdx * BM(m, k);
                 GM(0, m, k - 1) = GM(0, m, k - 1) - gu;
                 GP(0, m, k - 1) = GP(0, m, k - 1) - gu;
@@ -694,7 +756,7 @@ static double flux1d(const orc_cfg* c, int ixy, double dtdx, slice_t* sl)
             }
         rpt2(c, ixy, 2, sl->apdq, sl);
         for (int m = 0; m < meqn; m++)
-            for (int k = 1; k <= N - 1; k++) {
+            for (int k = k0; k <= kc; k++) {
                 double gu = 0.5 * dtdx * BM(m, k);
                 GM(0, m, k) = GM(0, m, k) - gu;
                 GP(0, m, k) = GP(0, m, k) - gu;
@@ -719,6 +781,7 @@ static int check_cfg(const orc_cfg* c)
         if (c->mthlim[w] < 0 || c->mthlim[w] > 6) return 1;
     switch (c->solver) {
     case ORC_SOLVER_ADV: if (c->maux < 2) return 1; break;
+    case ORC_SOLVER_TRANSPORT_FW: if (c->maux < 7 || c->meqn != 1 || c->mwaves != 1) return 1; break;
     case ORC_SOLVER_SWE_ROE: if (c->meqn != 3 || c->mwaves != 3) return 1; break;
     case ORC_SOLVER_EULER5_ROE: if (c->meqn != 5 || c->mwaves != 5) return 1; break;
     case ORC_SOLVER_EULER_HLLE: if (c->meqn != 5 || c->mwaves != 2 || c->claw5) return 1; break;

@@ -21,8 +21,15 @@ enum {
     ORC_SOLVER_ADV = 0,        /* clawpack46_rpn2adv / rpt2adv            */
     ORC_SOLVER_SWE_ROE = 1,    /* shallow/2d/rp clawpack46_rpn2 / rpt2    */
     ORC_SOLVER_EULER5_ROE = 2, /* euler/2d/rp clawpack46_rpn2/rpt2_euler5 */
-    ORC_SOLVER_EULER_HLLE = 3  /* Euler + tracer, HLLE normal solver (2 waves) and
+    ORC_SOLVER_EULER_HLLE = 3, /* Euler + tracer, HLLE normal solver (2 waves) and
                                   the HLL form of the Roe transverse split        */
+    ORC_SOLVER_TRANSPORT_FW = 4 /* conservative transport, f-waves: the transport
+                                   applications' clawpack46_rpn2cons_fw_manifold /
+                                   clawpack46_rpt2cons_manifold with the f-wave
+                                   flux function clawpack46_flux2fw; aux per cell
+                                   (1-based) 1 capacity, 2-5 normal velocities on
+                                   the left, right, bottom and top edges, 6-7 the
+                                   x- and y-edge lengths over dy and dx           */
 };
 
 typedef struct {

@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE, SOLVER_EULER_HLLE = 0, 1, 2, 3
+SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE, SOLVER_EULER_HLLE, SOLVER_TRANSPORT_FW = 0, 1, 2, 3, 4
 
 
 class OrcCfg(ctypes.Structure):
@@ -72,10 +72,11 @@ def _dp(a: np.ndarray):
 def make_cfg(solver: int, mx: int, my: int, mbc: int = 2, *, maux: int | None = None,
              order=(2, 2), mthlim=None, grav: float = 1.0, gamma: float = 1.4,
              efix: int = 1, claw5: bool = False) -> OrcCfg:
-    meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5, SOLVER_EULER_HLLE: 5}[solver]
+    meqn = {SOLVER_ADV: 1, SOLVER_SWE_ROE: 3, SOLVER_EULER5_ROE: 5, SOLVER_EULER_HLLE: 5,
+            SOLVER_TRANSPORT_FW: 1}[solver]
     mwaves = 2 if solver == SOLVER_EULER_HLLE else meqn
     if maux is None:
-        maux = 2 if solver == SOLVER_ADV else 0
+        maux = {SOLVER_ADV: 2, SOLVER_TRANSPORT_FW: 7}.get(solver, 0)
     c = OrcCfg()
     c.solver, c.mx, c.my, c.mbc = solver, mx, my, mbc
     c.meqn, c.maux, c.mwaves = meqn, maux, mwaves

@@ -102,3 +102,35 @@ def adv_state(rng, mx, my, mbc, xlower=0.0, ylower=0.0):
 def adv_dt(aux, dx, dy, cfl=0.9):
     return cfl * min(dx / max(np.abs(aux[0]).max(), 1e-12),
                      dy / max(np.abs(aux[1]).max(), 1e-12))
+
+
+def fw_state(rng, mx, my, mbc, *, per_edge=False, lengths=False):
+    """Conservative transport with f-waves on a Cartesian grid: a density
+    with a jump and a smooth part, aux (7, NY, NX) as the transport
+    applications' setaux + b4step2 give it on an unmapped grid: capacity 1,
+    the normal velocities of a cell's left/right/bottom/top edges (its
+    rotating velocity (u, v), or independent values per edge with
+    per_edge), edge lengths 1 (or perturbed with lengths)."""
+    dx, dy = 1.0 / mx, 1.0 / my
+    X, Y = _grid(mx, my, mbc, 0.0, 0.0, dx, dy)
+    q = 0.5 + 0.3 * np.sin(2 * math.pi * (X + rng.uniform())) * np.cos(2 * math.pi * Y)
+    q = np.where(X < rng.uniform(0.3, 0.7), q + 1.0, q)[None]
+    u = np.sin(math.pi * X) ** 2 * np.sin(2 * math.pi * Y) + 0.2 * rng.standard_normal(X.shape)
+    v = -np.sin(math.pi * Y) ** 2 * np.sin(2 * math.pi * X) + 0.2 * rng.standard_normal(X.shape)
+    aux = np.zeros((7,) + X.shape)
+    aux[0] = 1.0
+    aux[1] = aux[2] = u
+    aux[3] = aux[4] = v
+    if per_edge:
+        aux[2] = u + 0.1 * rng.standard_normal(X.shape)
+        aux[4] = v + 0.1 * rng.standard_normal(X.shape)
+    aux[5] = aux[6] = 1.0
+    if lengths:
+        aux[5] = rng.uniform(0.9, 1.1, X.shape)
+        aux[6] = rng.uniform(0.9, 1.1, X.shape)
+    return q, aux, dx, dy
+
+
+def fw_dt(aux, dx, dy, cfl=0.9):
+    smax = max(np.abs(aux[1:3]).max() * aux[5].max() / dx, np.abs(aux[3:5]).max() * aux[6].max() / dy)
+    return cfl / smax

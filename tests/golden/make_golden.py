@@ -95,6 +95,24 @@ def main():
     qi, ierr = R.timeinterp(qc, ql, 0.375, mx=mx, my=my, mbc=mbc)
     assert ierr == 0
     save("timeinterp", qcurr=qc, qlast=ql, qinterp=qi, alpha=0.375, mx=mx, my=my, mbc=mbc)
+    fw_golden()
+
+
+def fw_golden():
+    """f-wave step fixtures (python tests/golden/make_golden.py fw)."""
+    rng = np.random.default_rng(SEED + 11)
+    # ---- conservative transport with f-waves (flux2fw) ------------------
+    for k, ((mx, my), order, lim, opts) in enumerate([((16, 16), (2, 2), 4, {}),
+                                                      ((16, 12), (2, 2), 1, {"per_edge": True}),
+                                                      ((12, 16), (2, 1), 3, {"lengths": True}),
+                                                      ((16, 16), (1, 2), 4, {"per_edge": True, "lengths": True})]):
+        mbc = 2
+        q, aux, dx, dy = C.fw_state(rng, mx, my, mbc, **opts)
+        dt = C.fw_dt(aux, dx, dy)
+        qo, cfl = R.step2("fw", q, aux, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy, dt=dt, order=order,
+                          mthlim=[lim])
+        save(f"fw_step_{k}", q=q, aux=aux, qout=qo, cfl=cfl, mx=mx, my=my, mbc=mbc, dx=dx, dy=dy,
+             dt=dt, order=np.array(order), mthlim=np.array([lim]))
 
 
 def ascii_golden():
@@ -191,5 +209,7 @@ if __name__ == "__main__":
         src2_golden()
     elif sys.argv[1:] == ["claw5"]:
         claw5_golden()
+    elif sys.argv[1:] == ["fw"]:
+        fw_golden()
     else:
         main()

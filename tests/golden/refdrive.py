@@ -37,6 +37,10 @@ RP = {  # physics -> (library, rpn2 symbol, rpt2 symbol)
     "swe": ("libref_swe.so", "clawpack46_rpn2_", "clawpack46_rpt2_"),
     "euler": ("libref_euler.so", "clawpack46_rpn2_euler5_", "clawpack46_rpt2_euler5_"),
 }
+# the f-wave step: clawpack46_flux2fw with the transport applications'
+# f-wave pair (applications/clawpack/transport/2d/all/clawpack46_rpn2cons_fw_manifold.f,
+# clawpack46_rpt2cons_manifold.f), passed as rpn2fw / rpt2fw with use_fwaves = 1
+RP_FW = ("libref_fw.so", "clawpack46_rpn2cons_fw_manifold_", "clawpack46_rpt2cons_manifold_")
 
 
 def available() -> bool:
@@ -72,8 +76,10 @@ def set_params(physics: str, *, grav=1.0, gamma=1.4, tperiod=4.0):
 def step2(physics: str, q: np.ndarray, aux: np.ndarray | None, *, mx, my, mbc, dx, dy,
           dt, order=(2, 2), mthlim=None, xlower=0.0, ylower=0.0, t=0.0):
     """Run CLAWPACK46_STEP2_WRAP on a copy of q (tile layout (meqn, NY, NX)).
-    Returns (q_after, cfl)."""
-    libname, rpn2_sym, rpt2_sym = RP[physics]
+    physics "fw": the f-wave step (use_fwaves = 1, flux2fw).  Returns
+    (q_after, cfl)."""
+    fw = physics == "fw"
+    libname, rpn2_sym, rpt2_sym = RP_FW if fw else RP[physics]
     L = _lib(libname)
     meqn = q.shape[0]
     mwaves = meqn
@@ -93,7 +99,7 @@ def step2(physics: str, q: np.ndarray, aux: np.ndarray | None, *, mx, my, mbc, d
     ierr = _ci(0)
     rpn2 = ctypes.cast(getattr(L, rpn2_sym), ctypes.c_void_p)
     rpt2 = ctypes.cast(getattr(L, rpt2_sym), ctypes.c_void_p)
-    flux2 = ctypes.cast(L.clawpack46_flux2_, ctypes.c_void_p)
+    flux2 = ctypes.cast(L.clawpack46_flux2fw_ if fw else L.clawpack46_flux2_, ctypes.c_void_p)
     L.clawpack46_set_block_(ctypes.byref(_ci(0)))
     L.clawpack46_step2_wrap_(
         ctypes.byref(_ci(maxm)), ctypes.byref(_ci(meqn)), ctypes.byref(_ci(maux)),
@@ -103,7 +109,7 @@ def step2(physics: str, q: np.ndarray, aux: np.ndarray | None, *, mx, my, mbc, d
         ctypes.byref(_cd(dt)), ctypes.byref(cfl), _p(work), ctypes.byref(_ci(mwork)),
         ctypes.byref(_cd(xlower)), ctypes.byref(_cd(ylower)), ctypes.byref(_ci(0)),
         ctypes.byref(_cd(t)), _p(fp), _p(fm), _p(gp), _p(gm), rpn2, rpt2, rpn2, rpt2,
-        flux2, bcc, ctypes.byref(ierr), ctypes.byref(_ci(0)))
+        flux2, bcc, ctypes.byref(ierr), ctypes.byref(_ci(1 if fw else 0)))
     L.clawpack46_unset_block_()
     if ierr.value != 0:
         raise RuntimeError(f"step2_wrap ierror={ierr.value}")

@@ -32,8 +32,8 @@ def _golden(prefix):
 
 
 def _cfg_for(name, g, **over):
-    solver = {"swe": O.SOLVER_SWE_ROE, "euler": O.SOLVER_EULER5_ROE, "adv": O.SOLVER_ADV}[
-        name.split("_")[0]]
+    solver = {"swe": O.SOLVER_SWE_ROE, "euler": O.SOLVER_EULER5_ROE, "adv": O.SOLVER_ADV,
+              "fw": O.SOLVER_TRANSPORT_FW}[name.split("_")[0]]
     kw = dict(order=tuple(int(x) for x in g["order"]), mthlim=[int(x) for x in g["mthlim"]])
     if solver == O.SOLVER_SWE_ROE:
         kw["grav"] = float(g["grav"])
@@ -43,12 +43,13 @@ def _cfg_for(name, g, **over):
     return O.make_cfg(solver, int(g["mx"]), int(g["my"]), int(g["mbc"]), **kw)
 
 
-STEP_CASES = _golden("swe_step") + _golden("euler_step") + _golden("adv_step")
+STEP_CASES = (_golden("swe_step") + _golden("euler_step") + _golden("adv_step")
+              + _golden("fw_step"))
 
 
 def test_golden_fixtures_present():
     assert len(_golden("swe_step")) >= 6 and len(_golden("euler_step")) >= 2
-    assert len(_golden("adv_step")) >= 3
+    assert len(_golden("adv_step")) >= 3 and len(_golden("fw_step")) >= 4
 
 
 @pytest.mark.parametrize("name", STEP_CASES)
@@ -218,4 +219,117 @@ def test_oracle_matches_live_reference_random():
                                 order=order, mthlim=lim)
         qo, co = O.step2(O.make_cfg(O.SOLVER_SWE_ROE, mx, my, 2, order=order, mthlim=lim), q,
                          None, dx, dy, dt)
+        assert np.array_equal(qr, qo) and cr == co
+
+
+# ---- conservative transport with f-waves ---------------------------------
+
+def _fw(q, aux, dx, dy, dt, order=(2, 2), lim=4):
+    ny, nx = q.shape[1:]
+    cfg = O.make_cfg(O.SOLVER_TRANSPORT_FW, nx - 4, ny - 4, 2, order=order, mthlim=[lim])
+    return O.step2(cfg, q, aux, dx, dy, dt)
+
+
+@pytest.mark.parametrize("u,v", [(0.7, 0.4), (-0.6, 0.5), (0.45, -0.8), (-0.3, -0.9)])
+def test_fw_constant_velocity_equals_wave_propagation(u, v):
+    """With one velocity everywhere the f-wave is s times the jump, so the
+    f-wave step equals the wave-propagation advection step (pinned above
+    against the reference and the textbook) to rounding, for every sign of
+    (u, v): a correction weighted by |s| instead of sign(s), or a transverse
+    velocity taken from the wrong side, fails this."""
+    rng = np.random.default_rng(int(100 * (u + 2 * v) + 500))
+    q = rng.uniform(0, 1, (1, 20, 20))
+    aux = np.zeros((7, 20, 20))
+    aux[0] = 1.0
+    aux[1] = aux[2] = u
+    aux[3] = aux[4] = v
+    aux[5] = aux[6] = 1.0
+    dt = 0.8 * 0.05 / max(abs(u), abs(v))
+    for order in [(2, 2), (2, 1), (1, 2)]:
+        qf, cf = _fw(q, aux, 0.05, 0.05, dt, order)
+        qa, ca = O.step2(O.make_cfg(O.SOLVER_ADV, 16, 16, 2, order=order), q,
+                         np.stack([np.full((20, 20), u), np.full((20, 20), v)]), 0.05, 0.05, dt)
+        np.testing.assert_allclose(qf[:, 2:-2, 2:-2], qa[:, 2:-2, 2:-2], rtol=0, atol=1e-14)
+        assert cf == pytest.approx(ca, rel=1e-15)
+
+
+def test_fw_1d_reduction_matches_bale_leveque_fwave_formula():
+    """v = 0, q and u constant in y, u > 0 varying cell to cell: the step is the
+    1-D f-wave method of Bale, LeVeque, Mitran and Rossmanith (SIAM J. Sci.
+    Comput. 24, 2002), eqs. 2.10-2.14, with the MC limiter, written out here:
+    Z_{i-1/2} = u_i Q_i - u_{i-1} Q_{i-1}, s = (u_{i-1} + u_i)/2,
+    Q_i -= dt/dx (Z_{i-1/2} + F_{i+1/2} - F_{i-1/2}),
+    F = 1/2 sgn(s) (1 - dt/dx |s|) phi(r) Z."""
+    mx, mbc, dx = 24, 2, 0.1
+    N = mx + 2 * mbc
+    rng = np.random.default_rng(8)
+    uc = rng.uniform(0.3, 0.9, N)
+    Q = np.concatenate([np.zeros(8), np.ones(10), rng.uniform(0, 1, N - 18)])
+    dt = 0.8 * dx / uc.max()
+    q = np.tile(Q, (N, 1))[None]
+    aux = np.zeros((7, N, N))
+    aux[0] = 1.0
+    aux[1] = aux[2] = np.tile(uc, (N, 1))
+    aux[5] = aux[6] = 1.0
+    qn, cfl = _fw(q, aux, dx, dx, dt)
+    Z = uc[1:] * Q[1:] - uc[:-1] * Q[:-1]   # Z[k] at face between cells k and k+1
+    s = 0.5 * (uc[1:] + uc[:-1])
+    assert cfl == pytest.approx(dt / dx * s[1:-1].max(), rel=1e-15)
+
+    def phi(r):
+        return max(0.0, min((1 + r) / 2, 2.0, 2 * r))
+
+    def F(k):  # face between cells k and k+1, upwind neighbour face k-1 (s > 0)
+        w = Z[k]
+        r = Z[k - 1] / w if w != 0 else 0.0
+        return 0.5 * (1 - dt / dx * s[k]) * (phi(r) if w != 0 else 1.0) * w
+
+    ref = Q.copy()
+    for i in range(mbc, N - mbc):
+        ref[i] = Q[i] - dt / dx * (Z[i - 1] + F(i) - F(i - 1))
+    np.testing.assert_allclose(qn[0, 5, mbc:N - mbc], ref[mbc:N - mbc], rtol=0, atol=1e-14)
+
+
+def test_fw_conserves_with_cell_velocities():
+    """With one velocity per cell (uL = uR, vB = vT) and unit edge lengths the
+    f-waves telescope: the interior total of a compactly supported q is
+    conserved to rounding, for any velocity field."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import cases as C
+    rng = np.random.default_rng(21)
+    q, aux, dx, dy = C.fw_state(rng, 16, 16, 2)
+    x = np.arange(20) - 2 + 0.5
+    X, Y = np.meshgrid(x, x)
+    q[0] = np.where((X - 8) ** 2 + (Y - 8) ** 2 < 16, q[0], 0.0)
+    qn, _ = _fw(q, aux, dx, dy, C.fw_dt(aux, dx, dy))
+    assert abs(qn[0, 2:-2, 2:-2].sum() - q[0, 2:-2, 2:-2].sum()) < 1e-13
+
+
+@pytest.mark.parametrize("perm,which", [([0, 2, 1, 3, 4, 5, 6], "fw_step_1"),
+                                        ([0, 1, 2, 4, 3, 5, 6], "fw_step_3"),
+                                        ([0, 1, 2, 3, 4, 6, 5], "fw_step_2")])
+def test_fw_planted_swapped_aux_is_detected(perm, which):
+    """Left/right (bottom/top) velocities and x/y edge lengths swapped in the
+    input change the result, so the fixtures tell those components apart."""
+    g = _load(which)
+    qn, _ = O.step2(_cfg_for(which, g), g["q"], g["aux"][perm], float(g["dx"]), float(g["dy"]),
+                    float(g["dt"]))
+    assert np.abs(qn - g["qout"]).max() > 1e-6
+
+
+@pytest.mark.skipif(not refdrive.available(), reason="oracle/_ref not built (needs /root/reference)")
+def test_fw_oracle_matches_live_reference_random():
+    import cases as C
+    rng = np.random.default_rng(13)
+    for k in range(4):
+        mx, my = int(rng.integers(6, 20)), int(rng.integers(6, 20))
+        q, aux, dx, dy = C.fw_state(rng, mx, my, 2, per_edge=k % 2 == 1, lengths=k >= 2)
+        dt = C.fw_dt(aux, dx, dy)
+        lim = [int(rng.integers(0, 7))]
+        order = [(2, 2), (2, 1), (1, 1), (2, 0)][k]
+        qr, cr = refdrive.step2("fw", q, aux, mx=mx, my=my, mbc=2, dx=dx, dy=dy, dt=dt,
+                                order=order, mthlim=lim)
+        qo, co = O.step2(O.make_cfg(O.SOLVER_TRANSPORT_FW, mx, my, 2, order=order, mthlim=lim),
+                         q, aux, dx, dy, dt)
         assert np.array_equal(qr, qo) and cr == co
