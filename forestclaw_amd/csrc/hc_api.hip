@@ -93,6 +93,12 @@ int validate_physics(const hipclaw_physics* p)
         if (!(p->gamma > 1)) return set_error(HIPCLAW_ERR_ARG, "gamma must be > 1");
         if (is_claw5(p->solver)) return set_error(HIPCLAW_ERR_UNSUPPORTED, "HLLE has no Clawpack 5 variant");
         break;
+    case HIPCLAW_SOLVER_TRANSPORT_FW:
+        if (p->meqn != 1 || p->mwaves != 1 || p->maux != 7)
+            return set_error(HIPCLAW_ERR_ARG, "f-wave transport needs meqn=mwaves=1, maux=7");
+        if (is_claw5(p->solver))
+            return set_error(HIPCLAW_ERR_UNSUPPORTED, "f-wave transport has no Clawpack 5 variant");
+        break;
     default:
         return set_error(HIPCLAW_ERR_ARG, "unknown solver");
     }
@@ -309,6 +315,7 @@ int hipclaw_physics_default(int solver, int mx, int my, hipclaw_physics* out)
     case HIPCLAW_SOLVER_SWE_ROE: out->meqn = out->mwaves = 3; break;
     case HIPCLAW_SOLVER_EULER5_ROE: out->meqn = out->mwaves = 5; break;
     case HIPCLAW_SOLVER_EULER_HLLE: out->meqn = 5; out->mwaves = 2; break;
+    case HIPCLAW_SOLVER_TRANSPORT_FW: out->meqn = out->mwaves = 1; out->maux = 7; break;
     default: return set_error(HIPCLAW_ERR_ARG, "unknown solver");
     }
     for (int w = 0; w < 5; w++) out->mthlim[w] = w < out->mwaves ? 4 : 0;

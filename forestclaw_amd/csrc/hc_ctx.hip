@@ -1252,6 +1252,9 @@ int hipclaw_ctx_create(const hipclaw_physics* phys, const hipclaw_domain* dom, v
         dom->maxlevel >= MAXL - 1 || !(dom->bx > dom->ax) || !(dom->by > dom->ay))
         return set_error(HIPCLAW_ERR_ARG, "bad domain");
     if (phys->mx % 2 || phys->my % 2) return set_error(HIPCLAW_ERR_ARG, "mx, my must be even for AMR");
+    if (solver_family(phys->solver) == HIPCLAW_SOLVER_TRANSPORT_FW)
+        return set_error(HIPCLAW_ERR_UNSUPPORTED,
+                         "f-wave transport steps patch batches (hipclaw_step2_batch); the AMR context has no fill of its edge data");
     for (int f = 0; f < 4; f++)
         if (phys->mthbc[f] != 1 && phys->mthbc[f] != 3)
             return set_error(HIPCLAW_ERR_UNSUPPORTED, "mthbc must be 1 (extrapolation) or 3 (wall)");

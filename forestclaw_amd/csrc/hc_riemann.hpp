@@ -8,6 +8,8 @@
 //           clawpack46_rpt2adv.f:18-25
 //   SWE     applications/clawpack/shallow/2d/rp/clawpack46_rpn2.f:83-245,
 //           clawpack46_rpt2.f:34-72 (Roe averages shared with rpn2 as /comroe/ does)
+//   FwCons  applications/clawpack/transport/2d/all/clawpack46_rpn2cons_fw_manifold.f:23-46,
+//           clawpack46_rpt2cons_manifold.f:25-63 (with flux2fw's dsign(1, s) weight)
 //   Euler5  applications/clawpack/euler/2d/rp/clawpack46_rpn2_euler5.f:62-278,
 //           clawpack46_rpt2_euler5.f:48-118 (rpt2 recomputes the same Roe
 //           averages from the same two cells; they are reused here)
@@ -28,7 +30,7 @@
 
 namespace hc {
 
-enum { SOLVER_ADV = 0, SOLVER_SWE = 1, SOLVER_EULER5 = 2, SOLVER_EULER_HLLE = 3 };
+enum { SOLVER_ADV = 0, SOLVER_SWE = 1, SOLVER_EULER5 = 2, SOLVER_EULER_HLLE = 3, SOLVER_TRANSPORT_FW = 4 };
 
 template <class R, int ME_, int MW_> struct FaceT {
     static constexpr int ME = ME_, MW = MW_;
@@ -52,6 +54,7 @@ struct PhysParams {
 template <class R> struct Adv {
     static constexpr int ME = 1, MW = 1, MAUX = 2;
     static constexpr bool HAS_CONS = false;
+    static constexpr bool FW = false;  // f-wave edge-velocity form (FwCons)
     using Face = FaceT<R, 1, 1>;
     // transverse speeds for the amdq split (cell on the low side) and the
     // apdq split (cell on the high side): {down, up}
@@ -89,11 +92,75 @@ template <class R> struct Adv {
 };
 
 // ---------------------------------------------------------------------------
+// Conservative transport q_t + (u q)_x + (v q)_y = 0 in f-wave form, with the
+// edge data of the transport applications (aux components, 0-based: 1 / 2
+// normal velocity on a cell's left / right edge, 3 / 4 on its bottom / top
+// edge, 5 / 6 the x- / y-edge length ratio; component 0, the capacity, is not
+// read).  The single wave is the flux jump (the f-wave) and its speed the mean
+// of the two edge velocities; the kernel forms the velocities of a face from
+// the two cells' edge data (rpn2f / mean) and takes the second-order weight
+// sign(s) instead of |s| (fw_weight).
+// ---------------------------------------------------------------------------
+template <class R> struct FwCons {
+    static constexpr int ME = 1, MW = 1, MAUX = 7;
+    static constexpr bool HAS_CONS = false;
+    static constexpr bool FW = true;
+    using Face = FaceT<R, 1, 1>;
+    // mean transverse velocities for the amdq split (edge length of the
+    // low-side cell) and the apdq split (of the high-side cell): {down, up}
+    struct Trans { R m_dn, m_up, p_dn, p_up; };
+
+    static constexpr int NC = 0;
+    __device__ static inline void cell(const R*, const PhysParams&, R*) {}
+    template <int DIR> __device__ static constexpr bool wz(int, int) { return false; }
+    static constexpr bool RPT2_SCALED = false;
+    static constexpr bool SMAX = false;
+    __device__ static inline R smax(const Face&, const Trans&) { return R(0); }
+    template <int DIR> __device__ static void rpt2_scaled(const R*, const Trans&, R, R*, R*) {}
+
+    // face between qa (low side) and qb: g the edge length of the high-side
+    // cell, uhi its low-edge velocity, ulo the low-side cell's high-edge
+    // velocity (rpn2cons_fw_manifold.f:29-44)
+    __device__ static inline void rpn2f(R qa, R qb, R g, R uhi, R ulo, Face& f)
+    {
+        const R ur = g * uhi;
+        const R ul = g * ulo;
+        const R fw = ur * qb - ul * qa;
+        const R uhat = (ul + ur) * R(0.5);
+        f.w[0][0] = fw;
+        f.s[0] = uhat;
+        f.am[0] = uhat >= R(0) ? R(0) : fw;
+        f.ap[0] = uhat >= R(0) ? fw : R(0);
+    }
+
+    // transverse mean velocity on one side (rpt2cons_manifold.f:33-36,45-48)
+    __device__ static inline R mean(R g, R vhi, R vlo) { return (g * vhi + g * vlo) * R(0.5); }
+
+    // second-order weight of a wave of speed s: dsign(1, s) (flux2fw.f:176)
+    __device__ static inline R fw_weight(R s)
+    {
+        if constexpr (sizeof(R) == 8) return __builtin_copysign(1.0, s);
+        else return __builtin_copysignf(1.0f, s);
+    }
+
+    template <int DIR>
+    __device__ static inline void rpt2(int imp, const R* asdq, const Trans& t,
+                                       const PhysParams&, R* bm, R* bp)
+    {
+        R dn = imp == 1 ? t.m_dn : t.p_dn;
+        R up = imp == 1 ? t.m_up : t.p_up;
+        bm[0] = zmin(dn) * asdq[0];
+        bp[0] = zmax(up) * asdq[0];
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Shallow water, Roe solver with Harten-Hyman entropy fix.
 // ---------------------------------------------------------------------------
 template <class R, bool C5> struct SweT {
     static constexpr int ME = 3, MW = 3, MAUX = 0;
     static constexpr bool HAS_CONS = true;
+    static constexpr bool FW = false;  // f-wave edge-velocity form (FwCons)
     using Face = FaceT<R, 3, 3>;
     struct Trans { R u, v, a; };
 
@@ -399,6 +466,7 @@ template <class R, bool C5> struct SweT {
 template <class R, bool C5> struct Euler5T {
     static constexpr int ME = 5, MW = 5, MAUX = 0;
     static constexpr bool HAS_CONS = false;
+    static constexpr bool FW = false;  // f-wave edge-velocity form (FwCons)
     using Face = FaceT<R, 5, 5>;
     struct Trans { R u, v, enth, a, g1a2, euv, u2v2; };
 
@@ -579,6 +647,7 @@ template <class R, bool C5> struct Euler5T {
 template <class R> struct EulerHlle {
     static constexpr int ME = 5, MW = 2, MAUX = 0;
     static constexpr bool HAS_CONS = false;
+    static constexpr bool FW = false;  // f-wave edge-velocity form (FwCons)
     using Face = FaceT<R, 5, 2>;
     using Trans = typename Euler5T<R, false>::Trans;
 

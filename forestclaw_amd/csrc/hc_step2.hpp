@@ -306,7 +306,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     float swa0[CPL], swda[CPL];
     const float* swyf = nullptr;
     float swcu = 0.0f, swcv = 0.0f;
-    if constexpr (S::MAUX > 0) {
+    if constexpr (S::MAUX > 0 && !S::FW) {
         if (A.swirl) {
             const double x0 = pact ? A.xylower[2 * tix] : 0.0;
             const double y0 = pact ? A.xylower[2 * tix + 1] : 0.0;
@@ -440,7 +440,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 for (int m = 0; m < ME; m++)
                     qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
             }
-            if constexpr (S::MAUX > 0) {
+            if constexpr (S::MAUX > 0 && !S::FW) {
                 if (A.swirl && sizeof(R) == 4) {
                     const float b0 = rowin ? swyf[Jo] : 0.0f, db = rowin ? swyf[NY + Jo] : 0.0f;
                     const float bn = J + 1 < NY ? swyf[Jo + 1] : 0.0f;
@@ -472,6 +472,33 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
+            }
+        }
+        // f-wave transport: edge data of row J (left/right/bottom/top edge
+        // velocities, x/y edge lengths, aux components 1..6), of row J+1 (the
+        // x-faces' transverse splits: bottom/top velocities, y length) and of
+        // row J-1 (the y-face below row J: top and right velocities, x length)
+        R fuL[CPL], fuR[CPL], fvB[CPL], fvT[CPL], fex[CPL], fey[CPL];
+        R fvBn[CPL], fvTn[CPL], feyn[CPL], fvTm[CPL], fuRm[CPL], fexm[CPL];
+        if constexpr (S::FW) {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                auto ld = [&](bool in, int comp, int r) {
+                    return in ? ap[((long long)comp * NY + r) * NX + col[k]] : R(0);
+                };
+                const bool i0 = cv[k] && rowin, ip = i0 && J + 1 < NY, im = i0 && J >= 1;
+                fuL[k] = ld(i0, 1, Jo);
+                fuR[k] = ld(i0, 2, Jo);
+                fvB[k] = ld(i0, 3, Jo);
+                fvT[k] = ld(i0, 4, Jo);
+                fex[k] = ld(i0, 5, Jo);
+                fey[k] = ld(i0, 6, Jo);
+                fvBn[k] = ld(ip, 3, Jo + 1);
+                fvTn[k] = ld(ip, 4, Jo + 1);
+                feyn[k] = ld(ip, 6, Jo + 1);
+                fvTm[k] = ld(im, 4, Jo - 1);
+                fuRm[k] = ld(im, 2, Jo - 1);
+                fexm[k] = ld(im, 5, Jo - 1);
             }
         }
 
@@ -570,10 +597,29 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 vl_cur[k] = from_left<CPL>(vcur, k);
                 vl_next[k] = from_left<CPL>(vnext, k);
             }
+            // f-wave: the low-side cell's right-edge velocity and its top-edge
+            // velocity and y length in rows J and J+1
+            R fw_uRl[CPL], fw_vTl[CPL], fw_eyl[CPL], fw_vTnl[CPL], fw_eynl[CPL];
+            if constexpr (S::FW) {
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    fw_uRl[k] = from_left<CPL>(fuR, k);
+                    fw_vTl[k] = from_left<CPL>(fvT, k);
+                    fw_eyl[k] = from_left<CPL>(fey, k);
+                    fw_vTnl[k] = from_left<CPL>(fvTn, k);
+                    fw_eynl[k] = from_left<CPL>(feyn, k);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
-                S::template rpn2<0>(qa[k], qc[k], cda[k], cdc[k], ucur[k], pp, fx[k], tx[k]);
-                if constexpr (S::MAUX > 0) {
+                if constexpr (S::FW) {
+                    S::rpn2f(qa[k][0], qc[k][0], fex[k], fuL[k], fw_uRl[k], fx[k]);
+                    tx[k] = Trans{S::mean(fw_eyl[k], fvB[k], fw_vTl[k]), S::mean(fw_eynl[k], fvBn[k], fw_vTnl[k]),
+                                  S::mean(fey[k], fvB[k], fw_vTl[k]), S::mean(feyn[k], fvBn[k], fw_vTnl[k])};
+                } else {
+                    S::template rpn2<0>(qa[k], qc[k], cda[k], cdc[k], ucur[k], pp, fx[k], tx[k]);
+                }
+                if constexpr (S::MAUX > 0 && !S::FW) {
                     tx[k] = Trans{vl_cur[k], vl_next[k], vcur[k], vnext[k]};
                 }
                 // Courant number on faces i = 1 .. mx+1 of every swept row
@@ -653,7 +699,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         for (int w = 0; w < MW; w++) {
                             if (SKIPZ && S::template wz<0>(w, m)) continue;
                             R as = M<R>::abs_(fx[k].s[w]);
-                            c = c + as * (R(1) - as * dtdx) * fx[k].w[w][m];
+                            if constexpr (S::FW) c = c + S::fw_weight(fx[k].s[w]) * (R(1) - as * dtdx) * fx[k].w[w][m];
+                            else c = c + as * (R(1) - as * dtdx) * fx[k].w[w][m];
                         }
                         cq[k][m] = c;
                     }
@@ -753,12 +800,33 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 tyn[k] = Trans{};
             }
         } else {
+            // f-wave: the transverse splits of face J are made here and
+            // carried with the face (low-side split: x length of row J-1,
+            // high-side: of row J; "up" is the column to the right)
+            R fw_dn[CPL], fw_up[CPL], fw_dnr[CPL], fw_upr[CPL];
+            if constexpr (S::FW) {
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    fw_dn[k] = S::mean(fexm[k], fuL[k], fuRm[k]);
+                    fw_up[k] = S::mean(fex[k], fuL[k], fuRm[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < CPL; k++) {
+                    fw_dnr[k] = from_right<CPL>(fw_dn, k);
+                    fw_upr[k] = from_right<CPL>(fw_up, k);
+                }
+            }
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
+                if constexpr (S::FW) {
+                    S::rpn2f(qm1[k][0], qc[k][0], fey[k], fvB[k], fvTm[k], fyn[k]);
+                    tyn[k] = Trans{fw_dn[k], fw_dnr[k], fw_up[k], fw_upr[k]};
+                } else {
                 R cdm[NCD];  // row J-1, recomputed rather than carried
                 if constexpr (S::NC > 0) S::cell(qm1[k], pp, cdm);
                 else cdm[0] = R(0);
                 S::template rpn2<1>(qm1[k], qc[k], cdm, cdc[k], vcur[k], pp, fyn[k], tyn[k]);
+                }
                 if (cv[k] && col[k] >= 1 && col[k] <= NX - 2 && J >= 2 && J <= NY - 2 && J > J0 && J <= Jlast) {
                     if constexpr (S::SMAX) cfl[k] = pmax(cfl[k], dtdy * S::smax(fyn[k], tyn[k]));
                     else
@@ -826,7 +894,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         for (int w = 0; w < MW; w++) {
                             if (SKIPZ && S::template wz<1>(w, m)) continue;
                             R as = M<R>::abs_(f.s[w]);
-                            c = c + as * (R(1) - as * dtdy) * f.w[w][m];
+                            if constexpr (S::FW) c = c + S::fw_weight(f.s[w]) * (R(1) - as * dtdy) * f.w[w][m];
+                            else c = c + as * (R(1) - as * dtdy) * f.w[w][m];
                         }
                         cq[m] = c;
                     }
@@ -844,7 +913,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 }
                 if (trans) {
                     Trans t = typ[k];
-                    if constexpr (S::MAUX > 0) {
+                    if constexpr (S::MAUX > 0 && !S::FW) {
                         // amdq split uses cell F-1 = J-2, apdq split cell F = J-1;
                         // "down" is this column, "up" the column to the right
                         t = Trans{um2[k], urt_m2[k], um1[k], urt_m1[k]};

@@ -58,7 +58,7 @@ static int launch_one(const StepArgs& a0, int wpg, const void* qin, void* qout,
         while (a.nband < 2 && wpg > 1 && (stg + sw) * wpg > 65536) wpg--;
         if (stg + sw > 65536) return set_error(HIPCLAW_ERR_UNSUPPORTED, "patches too large for the ghost push");
     }
-    if (a.swirl && !(SolverT<R>::MAUX > 0)) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the advection solver");
+    if (a.swirl && (!(SolverT<R>::MAUX > 0) || SolverT<R>::FW)) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the advection solver");
     if (a.swirl && !a.xylower) return set_error(HIPCLAW_ERR_ARG, "the swirl field needs the patch corners");
     const long long waves = (a.npatch + a.ppw - 1) / a.ppw;
     // banded: one workgroup of nband waves per wave's worth of patches
@@ -124,6 +124,7 @@ static int launch_step2_t(const StepArgs& a, int solver_id, int cpl, int wpg, co
         case SOLVER_SWE: return launch_one<R, Swe, 1>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER5: return launch_one<R, Euler5, 1>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER_HLLE: return launch_one<R, EulerHlle, 1>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_TRANSPORT_FW: return launch_one<R, FwCons, 1>(a, wpg, qin, qout, aux, cfl, s);
         }
     } else if (cpl == 2) {
         switch (solver) {
@@ -131,6 +132,7 @@ static int launch_step2_t(const StepArgs& a, int solver_id, int cpl, int wpg, co
         case SOLVER_SWE: return launch_one<R, Swe, 2>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER5: return launch_one<R, Euler5, 2>(a, wpg, qin, qout, aux, cfl, s);
         case SOLVER_EULER_HLLE: return launch_one<R, EulerHlle, 2>(a, wpg, qin, qout, aux, cfl, s);
+        case SOLVER_TRANSPORT_FW: return launch_one<R, FwCons, 2>(a, wpg, qin, qout, aux, cfl, s);
         }
     }
     return set_error(HIPCLAW_ERR_ARG, "unsupported solver / cols_per_lane");

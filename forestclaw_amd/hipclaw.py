@@ -14,6 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libhipclaw.so")
 
 SOLVER_ADV, SOLVER_SWE_ROE, SOLVER_EULER5_ROE, SOLVER_EULER_HLLE = 0, 1, 2, 3
+SOLVER_TRANSPORT_FW = 4
 FP64, FP32 = 0, 1
 FEATURE_MERGE_LEVELS, FEATURE_GHOST_PUSH, FEATURE_GRAPH, FEATURE_INIT_GHOSTCELL = 1, 2, 3, 4
 FEATURE_FUSE_AUX = 5

@@ -45,6 +45,15 @@ extern "C" {
                                        solver is Roe only); same interface as
                                        clawpack46_rpn2_euler5.f:1-280 /
                                        clawpack46_rpt2_euler5.f:1-122             */
+#define HIPCLAW_SOLVER_TRANSPORT_FW 4 /* conservative transport in f-wave form: the
+                                       clawpack46_flux2fw.f step (step2_wrap with
+                                       use_fwaves) with the transport solver pair
+                                       clawpack46_rpn2cons_fw_manifold.f /
+                                       clawpack46_rpt2cons_manifold.f (meqn 1,
+                                       mwaves 1, maux 7: capacity, left / right /
+                                       bottom / top edge velocities, x / y edge
+                                       lengths).  hipclaw_step2_batch only: the
+                                       AMR context does not fill these edge data */
 
 /* Flag OR-ed into the solver id: the Clawpack 5 variant of the solver family
  * (the fc2d_clawpack5 solver, src/solvers/fc2d_clawpack5/, with the

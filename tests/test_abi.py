@@ -97,3 +97,21 @@ def test_patch_list_over_no_tiles_rejected():
     ids = (ctypes.c_int * 2)(0, 1)
     with pytest.raises(H.HipclawError, match="ntiles"):
         H.step2_batch(p, 2, 1, 1, None, 1, 0.1, patch_ids=ctypes.addressof(ids), ntiles=0, stream=0)
+
+
+def test_fwave_transport_physics_and_rejections():
+    """f-wave transport: meqn 1, maux 7 by default; other aux counts, the
+    Clawpack 5 flag and the AMR context are refused before any device call."""
+    p = H.physics(H.SOLVER_TRANSPORT_FW, 16, 16)
+    assert (p.meqn, p.mwaves, p.maux) == (1, 1, 7)
+    p.maux = 2
+    with pytest.raises(H.HipclawError, match="maux=7"):
+        H.step2_batch(p, 4, 1, 1, 1, 1, 0.1, stream=0)
+    with pytest.raises(H.HipclawError, match="Clawpack 5"):
+        H.step2_batch(H.physics(H.SOLVER_TRANSPORT_FW | H.CLAW5, 16, 16), 4, 1, 1, 1, 1, 0.1,
+                      stream=0)
+    dom = H.Domain(1, 1, 0.0, 1.0, 0.0, 1.0, 2, 3, 0)
+    h = ctypes.c_void_p()
+    rc = H.lib().hipclaw_ctx_create(ctypes.byref(H.physics(H.SOLVER_TRANSPORT_FW, 16, 16)),
+                                    ctypes.byref(dom), None, ctypes.byref(h))
+    assert rc != 0 and b"hipclaw_step2_batch" in H.lib().hipclaw_last_error()

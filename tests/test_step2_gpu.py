@@ -22,7 +22,8 @@ FP32_REL_L1 = 1e-5
 SOLVERS = {"swe": (H.SOLVER_SWE_ROE, O.SOLVER_SWE_ROE),
            "euler": (H.SOLVER_EULER5_ROE, O.SOLVER_EULER5_ROE),
            "hlle": (H.SOLVER_EULER_HLLE, O.SOLVER_EULER_HLLE),
-           "adv": (H.SOLVER_ADV, O.SOLVER_ADV)}
+           "adv": (H.SOLVER_ADV, O.SOLVER_ADV),
+           "fw": (H.SOLVER_TRANSPORT_FW, O.SOLVER_TRANSPORT_FW)}
 
 
 def _t(a, dev, dtype=None):
@@ -59,7 +60,8 @@ def _golden(prefix):
     return sorted(glob.glob(os.path.join(GOLD, prefix + "*.npz")))
 
 
-@pytest.mark.parametrize("path", _golden("swe_step") + _golden("euler_step") + _golden("adv_step"))
+@pytest.mark.parametrize("path", _golden("swe_step") + _golden("euler_step") + _golden("adv_step")
+                         + _golden("fw_step"))
 @pytest.mark.parametrize("launch", [None, (1, 1), (2, 4)])
 def test_fp64_matches_reference_golden_bitwise(dev, path, launch):
     g = dict(np.load(path))
@@ -89,6 +91,9 @@ def _batch(kind, rng, n, mx, my):
             q, dx, dy = C.euler_state(rng, mx, my, 2)
             dts.append(C.euler_dt(q, dx, dy))
             aux = None
+        elif kind == "fw":
+            q, aux, dx, dy = C.fw_state(rng, mx, my, 2, per_edge=True, lengths=True)
+            dts.append(C.fw_dt(aux, dx, dy))
         else:
             q, dx, dy = C.adv_state(rng, mx, my, 2)
             aux = O.swirl_aux(mx, my, 2, rng.uniform(0, 0.5), rng.uniform(0, 0.5), dx, dy,
@@ -117,7 +122,8 @@ def _oracle_batch(kind, q, aux, dx, dy, dt, order=(2, 2), mthlim=None):
 @pytest.mark.parametrize("kind,mx,my,n", [("swe", 32, 32, 37), ("swe", 16, 24, 11),
                                           ("euler", 16, 16, 13), ("adv", 32, 32, 9),
                                           ("adv", 20, 12, 7), ("hlle", 16, 16, 13),
-                                          ("hlle", 20, 12, 9)])
+                                          ("hlle", 20, 12, 9), ("fw", 32, 32, 9),
+                                          ("fw", 20, 12, 7)])
 @pytest.mark.parametrize("launch", [(1, 4), (2, 2), (1, 1, 3), (1, 1, 4), (1, 1, 8)])
 def test_fp64_batch_bitwise_vs_oracle(dev, kind, mx, my, n, launch):
     """Whole-patch sweeps and row bands (3, 4 and 8 waves per patch; the third
@@ -156,6 +162,9 @@ def test_patch_ids_and_inplace(dev, launch):
 @pytest.mark.parametrize("order,mthlim", [((2, 1), (1, 2, 3)), ((1, 0), (4, 4, 4)),
                                           ((2, 2), (5, 6, 0)), ((1, 2), (4, 4, 4))])
 def test_fp64_methods_and_limiters(dev, order, mthlim):
+    """Orders, transverse forms and limiters other than the MC default (the
+    general kernel, not the fixed (2, 2) MC one): shallow water, and the
+    f-wave transport step with the first limiter."""
     rng = np.random.default_rng(sum(mthlim) + 10 * order[1])
     q, aux, dx, dy, dt = _batch("swe", rng, 5, 32, 32)
     ref, rcfl = _oracle_batch("swe", q, aux, dx, dy, dt, order=order, mthlim=mthlim)
@@ -163,10 +172,15 @@ def test_fp64_methods_and_limiters(dev, order, mthlim):
     inner = (slice(None), slice(None), slice(2, -2), slice(2, -2))
     assert np.array_equal(got[inner], ref[inner])
     assert np.array_equal(cfl, rcfl)
+    q, aux, dx, dy, dt = _batch("fw", rng, 5, 24, 16)
+    ref, rcfl = _oracle_batch("fw", q, aux, dx, dy, dt, order=order, mthlim=mthlim[:1])
+    got, cfl = run_gpu(dev, "fw", q, aux, dx, dy, dt, order=order, mthlim=mthlim[:1])
+    assert np.array_equal(got[inner], ref[inner])
+    assert np.array_equal(cfl, rcfl)
 
 
 @pytest.mark.parametrize("kind,mx,n", [("swe", 32, 64), ("euler", 16, 32), ("adv", 32, 16),
-                                       ("swe", 16, 37), ("hlle", 16, 32)])
+                                       ("swe", 16, 37), ("hlle", 16, 32), ("fw", 32, 16)])
 @pytest.mark.parametrize("launch", [None, (1, 4), (2, 4), (2, 1)])
 def test_fp32_error_vs_fp64_oracle(dev, kind, mx, n, launch):
     rng = np.random.default_rng(1000 + n)
