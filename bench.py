@@ -385,7 +385,7 @@ def advection(level, steps=10, warmup=3, fixed=False):
                 sim.advance(0.0, dt)
                 sim.set_feature(H.FEATURE_FUSE_AUX, True)
             sim.set_swirl(tperiod)
-            shape, _ = sim.autotune(level, 3)
+            shape, sweep = sim.autotune(level, 3)
             for s in range(warmup):
                 sim.advance(s * dt, dt)
             torch.cuda.synchronize()
@@ -404,7 +404,8 @@ def advection(level, steps=10, warmup=3, fixed=False):
                         "cell_updates_per_s": npatch * 1024 * steps / (t1 - t0),
                         "step_kernel_ms": ms / nl, "step_kernel_GBs": algo / (ms / nl * 1e-3) / 1e9,
                         "step_kernel_hbm_frac": algo / (ms / nl * 1e-3) / HBM_PEAK,
-                        "launch": {"cols_per_lane": shape[0], "waves_per_group": shape[1]}}
+                        "launch": {"cols_per_lane": shape[0], "waves_per_group": shape[1],
+                                   "sweep_ms": {f"cpl{c}_wpg{w}": round(v, 4) for (c, w), v in sweep.items()}}}
             sim.close()
     return out
 

@@ -68,14 +68,14 @@ template <> __device__ inline float pmax<float>(float a, float b)
 
 // Whole-wave neighbour exchange (lane l receives lane l-1 / l+1) with the
 // GFX9 DPP wave_shr:1 / wave_shl:1 row controls: one VALU move, no LDS round
-// trip.  Lanes at the wave edges get their own value back (bound_ctrl off,
-// old = source); callers never consume those.  Called only in wave-uniform
-// control flow, so every source lane is active.
+// trip.  Lanes at the wave edges get zero (bound_ctrl on, so the move needs
+// no copy of an old value first); callers never consume those.  Called only
+// in wave-uniform control flow, so every source lane is active.
 constexpr int DPP_WAVE_SHL1 = 0x130;
 constexpr int DPP_WAVE_SHR1 = 0x138;
 template <int CTRL> __device__ inline int dpp_i(int v)
 {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, true);
 }
 template <int CTRL> __device__ inline float dpp_f(float v)
 {

@@ -407,31 +407,6 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         // row from the per-lane tile base instead of one pointer induction
         // variable per unrolled row and field kept live across the loop
         const int Jo = __builtin_amdgcn_readfirstlane(J);
-        if constexpr (PF) {
-            // row J was loaded during row J-1 (or before the loop); row J+1
-            // goes into the slot of row J-3, which no row from J on reads, so
-            // its load latency is hidden behind this row's arithmetic
-            // (loaded unconditionally and zeroed by a select: without
-            // branches around the loads the wait for row J counts exactly the
-            // loads issued after it).  Every address is valid: a lane without
-            // a patch reads tile 0 of the launch (patch 0: slot slot0 of a
-            // merged launch, whose q pool pointer step2_launch sets for every
-            // level, or tile 0 of a batch, which it requires to exist); rows
-            // past the sweep read row Jlast (<= NY-1) of the tile; lanes past
-            // the tile's columns read column 0.
-            constexpr int SN = (P + 1) & 3;
-            const bool nin = J + 1 <= Jlast;
-            const int Jn = nin ? Jo + 1 : Jlast;
-#pragma unroll
-            for (int k = 0; k < CPL; k++) {
-                const int cc = cv[k] ? col[k] : 0;
-#pragma unroll
-                for (int m = 0; m < ME; m++) {
-                    const R v = qp[((long long)m * NY + Jn) * NX + cc];
-                    qr[SN][k][m] = (cv[k] && nin) ? v : R(0);
-                }
-            }
-        }
         R vcur[CPL], vnext[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
@@ -502,6 +477,32 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             }
         }
 
+        if constexpr (PF) {
+            // row J was loaded during row J-1 (or before the loop); row J+1
+            // goes into the slot of row J-3, which no row from J on reads, so
+            // its load latency is hidden behind this row's arithmetic
+            // (loaded unconditionally and zeroed by a select: without
+            // branches around the loads the wait for row J counts exactly the
+            // loads issued after it).  Issued after the row's velocity and
+            // edge-data loads, so the waits for those leave it in flight.  Every address is valid: a lane without
+            // a patch reads tile 0 of the launch (patch 0: slot slot0 of a
+            // merged launch, whose q pool pointer step2_launch sets for every
+            // level, or tile 0 of a batch, which it requires to exist); rows
+            // past the sweep read row Jlast (<= NY-1) of the tile; lanes past
+            // the tile's columns read column 0.
+            constexpr int SN = (P + 1) & 3;
+            const bool nin = J + 1 <= Jlast;
+            const int Jn = nin ? Jo + 1 : Jlast;
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int cc = cv[k] ? col[k] : 0;
+#pragma unroll
+                for (int m = 0; m < ME; m++) {
+                    const R v = qp[((long long)m * NY + Jn) * NX + cc];
+                    qr[SN][k][m] = (cv[k] && nin) ? v : R(0);
+                }
+            }
+        }
         // ---- per-cell solver data of row J (shared by its four faces) -------
         constexpr int NCD = S::NC > 0 ? S::NC : 1;
         R cdc[CPL][NCD];
