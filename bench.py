@@ -43,20 +43,22 @@ HBM_PEAK = 8.0e12
 # 2 x FETCH_SIZE + 1 x WRITE_SIZE from two rocprofv3 --pmc runs, the factors
 # being what the same counters read for a 1 GiB device copy
 # (profiles/k_step2_swe_humps_l7-9_hbm_bytes.md).  VALU_ACTIVE is
-# SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the level-9 launch
-# (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
+# SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the step launches of
+# a coarse step (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
 MEASURED_MESH = {7: 10481, 8: 2349, 9: 85052}
-TRAFFIC_PER_LAUNCH = 3.547e9
-VALU_ACTIVE = 0.83
-# FP32 VALU work of the level-9 step launch per cell update: wave-level VALU
-# instructions, and FP32 operations issued (64 lanes x (ADD + MUL + 2 FMA +
-# TRANS) wave instructions, SQ_INSTS_VALU_*_F32;
+TRAFFIC_PER_LAUNCH = 3.583e9
+VALU_ACTIVE = 0.87
+# FP32 VALU work of the step launches of a coarse step per cell update:
+# wave-level VALU instructions, and FP32 operations issued (64 lanes x (ADD +
+# MUL + 2 FMA + TRANS) wave instructions, SQ_INSTS_VALU_*_F32;
 # profiles/k_step2_swe_humps_l7-9_valu_mix.md).
-VALU_INSTS_PER_CELL = 14.43
-FLOP_PER_CELL = 950.4
+VALU_INSTS_PER_CELL = 12.26
+FLOP_PER_CELL = 748.2
 # MI355X FP32 vector peak without packed math: 256 CUs x 4 SIMDs x 16 lanes
 # x 2 (FMA) x 2.4 GHz.  The step kernels are built without packed FP32.
 FP32_VALU_PEAK = 78.6e12
+# VALU issue peak: one wave64 instruction per SIMD every 4 cycles.
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4
 
 
 _STREAM = []
@@ -610,6 +612,7 @@ def main():
                      "flop_per_cell_update": FLOP_PER_CELL,
                      "valu_busy": VALU_ACTIVE if measured else None,
                      "valu_insts_per_cell_update": VALU_INSTS_PER_CELL if measured else None,
+                     "valu_issue_frac": (VALU_INSTS_PER_CELL * k_rate / VALU_ISSUE_PEAK) if measured else None,
                      "traffic": TRAFFIC_PER_LAUNCH if measured else None,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
                      "algorithmic_per_launch": b_upd * r32["kcells"] / max(r32["launches"], 1),
