@@ -126,6 +126,19 @@ template <class R> __device__ inline R zsub(R x)
     else return R(0) - x;
 }
 
+// Block b of n, dealt to XCD b mod 8, renumbered so that XCD x takes the
+// contiguous range of the blocks it runs: a bijection of [0, n).
+#ifndef HC_STEP_XCD_ORDER
+#define HC_STEP_XCD_ORDER 1
+#endif
+constexpr int HC_NXCD = 8;
+__device__ inline int xcd_block(int b, int n)
+{
+    const int q = n / HC_NXCD, r = n % HC_NXCD;
+    const int x = b % HC_NXCD, i = b / HC_NXCD;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 template <int CPL, class T> __device__ inline T from_left(const T (&v)[CPL], int k)
 {
     T s = shl_up(v[CPL - 1]);
@@ -187,13 +200,19 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     constexpr bool SKIPZ = sizeof(R) == 4;
 
     const int lane = threadIdx.x & (HC_WAVE - 1);
-    const int gwave = (int)((blockIdx.x * (unsigned)blockDim.x + threadIdx.x) >> 6);
+    // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
+    // in turn, so block b runs on XCD b mod 8; renumbering the blocks so that
+    // each XCD takes one contiguous range of patches (slots are in Morton
+    // order) keeps most face neighbours in one L2, where a patch's interior
+    // stores and its neighbours' ghost pushes into the same lines combine
+    const int blk = HC_STEP_XCD_ORDER ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int gwave = (int)(((unsigned)blk * (unsigned)blockDim.x + threadIdx.x) >> 6);
     const int slot = lane / A.lpp;
     const int lp = lane - slot * A.lpp;
     // wave-uniform (scalar) band index: the row loop and its conditions stay
     // scalar when the band bounds derive from it
     const int band = BAND ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
-    const int pidx = (BAND ? (int)blockIdx.x : gwave) * A.ppw + slot;
+    const int pidx = (BAND ? blk : gwave) * A.ppw + slot;
     const bool pin = (slot < A.ppw) && (pidx < A.npatch);
     const int praw = pin ? (A.patch_ids ? A.patch_ids[pidx] : pidx) : 0;
     // a list entry outside the tile range is skipped, never dereferenced
@@ -1137,7 +1156,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             return q;
         };
         for (int sp = 0; sp < A.ppw; sp++) {
-            const int pi = (BAND ? (int)blockIdx.x : gwave) * A.ppw + sp;
+            const int pi = (BAND ? blk : gwave) * A.ppw + sp;
             if (pi >= A.npatch || pi >= A.ntiles) break;  // uniform across the wave
             const int st = A.slot0 + pi;
             // the patch's neighbour table, loaded once (uniform across the wave)
