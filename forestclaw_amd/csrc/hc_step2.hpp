@@ -803,15 +803,17 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         Face fyn[CPL];
         Trans tyn[CPL];
         R dotry[CPL][MW];
+        // e1, e2, fdm, fdp and dotry are made only on the rows whose faces
+        // exist (Jr >= 2, 3) and left unassigned on the first rows of a
+        // sweep, since nothing reads them there: the update and the time-sync
+        // slots that use them start at Jr >= 4 (fdp and dotry are read one row
+        // later, through their rings).  Zeroing them cost one move per value
+        // and row, placed before the row's branches.  hmp_new / hpp_new stay
+        // zero on those rows: the first face finalised reads them.
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
 #pragma unroll
-            for (int m = 0; m < ME; m++) {
-                e1[k][m] = e2[k][m] = fdm[k][m] = fdp[k][m] = R(0);
-                hmp_new[k][m] = hpp_new[k][m] = R(0);
-            }
-#pragma unroll
-            for (int w = 0; w < MW; w++) dotry[k][w] = R(0);
+            for (int m = 0; m < ME; m++) hmp_new[k][m] = hpp_new[k][m] = R(0);
         }
         if (Jr == 0) {
 #pragma unroll
@@ -967,6 +969,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                         e1[k][m] = zsub(hmm_y[m]) - hmp_prev[k][m];
                         e2[k][m] = zsub(hpm_y[m]) - hpp_prev[k][m];
                     }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < ME; m++) e1[k][m] = e2[k][m] = R(0);
                 }
             }
         }
