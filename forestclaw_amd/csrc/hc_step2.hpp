@@ -803,18 +803,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         Face fyn[CPL];
         Trans tyn[CPL];
         R dotry[CPL][MW];
-        // e1, e2, fdm, fdp and dotry are made only on the rows whose faces
-        // exist (Jr >= 2, 3) and left unassigned on the first rows of a
-        // sweep, since nothing reads them there: the update and the time-sync
-        // slots that use them start at Jr >= 4 (fdp and dotry are read one row
-        // later, through their rings).  Zeroing them cost one move per value
-        // and row, placed before the row's branches.  hmp_new / hpp_new stay
-        // zero on those rows: the first face finalised reads them.
-#pragma unroll
-        for (int k = 0; k < CPL; k++) {
-#pragma unroll
-            for (int m = 0; m < ME; m++) hmp_new[k][m] = hpp_new[k][m] = R(0);
-        }
+        // These are made only on the rows whose faces exist (Jr >= 2, 3) and
+        // left unassigned on the first rows of a sweep, since nothing reads
+        // them there: e1, e2, fdm and fdp are read by the update and the
+        // time-sync slots, which start at Jr >= 4; through the rings, fdp is
+        // read by the next row's update, dotry by the limiter of the next
+        // face finalised (Jr >= 3), and hmp_new / hpp_new by the next row's
+        // e1, e2, which again only the update reads.  Zeroing them cost one
+        // move per value and row, placed before the row's branches.
         if (Jr == 0) {
 #pragma unroll
             for (int k = 0; k < CPL; k++) {
