@@ -64,6 +64,37 @@ def test_single_step_update_protocol(dev, precision):
     assert np.array_equal(np.stack(work)[:, :, :2, :], np.stack(qs)[:, :, :2, :])
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_single_step_update_fwave_transport(dev, precision):
+    """Mode A with the f-wave transport solver: host patches with their edge
+    data (aux, 7 fields) staged per patch; FP64 bitwise against the oracle,
+    FP32 within 1e-5."""
+    import torch
+    rng = np.random.default_rng(17)
+    nps, cap = 11, 4
+    qs, auxs = [], []
+    for _ in range(nps):
+        q, aux, dx, dy = C.fw_state(rng, 16, 16, 2, per_edge=True, lengths=True)
+        qs.append(q)
+        auxs.append(aux)
+    dt = min(C.fw_dt(a, dx, dy) for a in auxs)
+    qb = np.stack([q.copy() for q in qs])
+    ocfl = O.step2_batch(O.make_cfg(O.SOLVER_TRANSPORT_FW, 16, 16, 2), qb, np.stack(auxs), dx, dy, dt)
+    b = H.Batch(H.physics(H.SOLVER_TRANSPORT_FW, 16, 16, precision=precision), cap,
+                torch.cuda.current_stream().cuda_stream)
+    work = [q.copy() for q in qs]
+    got = [b.single_step_update(work[it], auxs[it], dx, dy, 0.0, dt, it, nps) for it in range(nps)]
+    b.close()
+    w = np.stack(work)[:, :, 2:-2, 2:-2]
+    o = qb[:, :, 2:-2, 2:-2]
+    if precision == H.FP64:
+        assert np.array_equal(w, o)
+        assert max(got) == ocfl.max()
+    else:
+        assert np.abs(w - o).sum() / np.abs(o).sum() <= 1e-5
+        assert max(got) == pytest.approx(ocfl.max(), rel=1e-5)
+
+
 def test_single_step_update_rejects_bad_counters(dev):
     import torch
     qs, dx, dy, dt = _level(1)
