@@ -139,6 +139,15 @@ __device__ inline int xcd_block(int b, int n)
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// One-component limiter ratio (FP32): limited when w^2 would be normal, and
+// the reciprocal of the wave.  The FP64 forms are never instantiated.
+template <class R> __device__ inline bool lim_ok1(R w) { return M<R>::abs_(w) >= R(1.0842022e-19f); }
+template <class R> __device__ inline R rcp1(R w)
+{
+    if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(w);
+    else return R(1) / w;
+}
+
 template <int CPL, class T> __device__ inline T from_left(const T (&v)[CPL], int k)
 {
     T s = shl_up(v[CPL - 1]);
@@ -198,6 +207,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     // FP32 sums over a wave's components skip those that are zero by
     // construction (FP64 keeps every term of the reference's sums)
     constexpr bool SKIPZ = sizeof(R) == 4;
+    // FP32 one-equation solvers: the limiter ratio <w_up, w> / <w, w> of a
+    // one-component wave is w_up / w, taken as such (the neighbouring faces'
+    // waves are carried instead of the dot products); a wave is limited when
+    // |w| >= sqrt(FLT_MIN), i.e. when w^2 is normal, as in the general form
+    constexpr bool RAT = sizeof(R) == 4 && ME == 1 && MW == 1;
 
     const int lane = threadIdx.x & (HC_WAVE - 1);
     // XCD-contiguous block order: the hardware deals workgroups to the 8 XCDs
@@ -283,6 +297,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     // so the windows never move between registers; UNR = 1 keeps one row per
     // iteration and rotates the rings after it.
     R qr[4][CPL][ME], ur[4][CPL];
+    // advection: v of row J+1 (and of the left neighbour column), made by
+    // row J and taken by row J+1 as its own (slot (J - J0) mod 2)
+    R vnr[2][CPL], vlr[2][CPL];
     R fmr[4][CPL][ME], fpr[4][CPL][ME], Gr[4][CPL][ME];
     Face fyr[2][CPL] = {};
     Trans tyr[2][CPL] = {};
@@ -295,6 +312,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             ur[s][k] = R(0);
+            if (s < 2) vnr[s][k] = vlr[s][k] = R(0);
 #pragma unroll
             for (int m = 0; m < ME; m++) qr[s][k][m] = fmr[s][k][m] = fpr[s][k][m] = Gr[s][k][m] = R(0);
         }
@@ -381,6 +399,25 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         }
     }
 
+    // advection: v at the bottom edge of (col[k], r), 0 for lanes without a
+    // cell; as the velocity-field kernel evaluates it (or as aux holds it)
+    auto vel_v = [&](int k, int r) __attribute__((always_inline)) -> R {
+        if constexpr (S::MAUX > 0 && !S::FW) {
+            if (A.swirl && sizeof(R) == 4) {
+                return cv[k] ? R(swcv * (swyf[r] * swda[k])) : R(0);
+            } else if (A.swirl) {
+                const double a0 = swx0[k] * swx0[k], a1 = swx1[k] * swx1[k];
+                const double b0 = swy[r] * swy[r];
+                const double pij = (a0 * b0) / swpi;
+                const double v = -swvtx * ((a1 * b0) / swpi - pij);
+                return cv[k] ? R(v) : R(0);
+            } else {
+                return cv[k] ? ap[((long long)NY + r) * NX + col[k]] : R(0);
+            }
+        }
+        return R(0);
+    };
+
     auto row = [&](auto phase, const int J) __attribute__((always_inline)) {
         constexpr int P = decltype(phase)::value;
         constexpr int S0 = P & 3, S1 = (P + 3) & 3, S2 = (P + 2) & 3;  // rows J, J-1, J-2
@@ -435,39 +472,33 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     qc[k][m] = (cv[k] && rowin) ? qp[((long long)m * NY + Jo) * NX + col[k]] : R(0);
             }
             if constexpr (S::MAUX > 0 && !S::FW) {
+                // u of (c, J) and v of (c, J+1); v of (c, J) is what row J-1
+                // made as its v of the next row (or the prologue, for J0)
                 if (A.swirl && sizeof(R) == 4) {
-                    const float b0 = rowin ? swyf[Jo] : 0.0f, db = rowin ? swyf[NY + Jo] : 0.0f;
-                    const float bn = J + 1 < NY ? swyf[Jo + 1] : 0.0f;
+                    const float db = rowin ? swyf[NY + Jo] : 0.0f;
                     ucur[k] = (cv[k] && rowin) ? R(swcu * (swa0[k] * db)) : R(0);
-                    vcur[k] = (cv[k] && rowin) ? R(swcv * (b0 * swda[k])) : R(0);
-                    vnext[k] = (cv[k] && J + 1 < NY) ? R(swcv * (bn * swda[k])) : R(0);
                 } else if (A.swirl) {
-                    // velocities of (c, J) and the v of (c, J+1), as k_swirl_aux
-                    ucur[k] = vcur[k] = vnext[k] = R(0);
-                    const double a0 = swx0[k] * swx0[k], a1 = swx1[k] * swx1[k];
+                    ucur[k] = R(0);
                     if (rowin) {
+                        const double a0 = swx0[k] * swx0[k];
                         const double b0 = swy[Jo] * swy[Jo], b1 = swy[NY + Jo] * swy[NY + Jo];
                         const double pij = (a0 * b0) / swpi;
                         const double u = swvty * ((a0 * b1) / swpi - pij);
-                        const double v = -swvtx * ((a1 * b0) / swpi - pij);
                         ucur[k] = cv[k] ? R(u) : R(0);
-                        vcur[k] = cv[k] ? R(v) : R(0);
-                    }
-                    if (J + 1 < NY) {
-                        const double b0 = swy[Jo + 1] * swy[Jo + 1];
-                        const double pij = (a0 * b0) / swpi;
-                        const double v = -swvtx * ((a1 * b0) / swpi - pij);
-                        vnext[k] = cv[k] ? R(v) : R(0);
                     }
                 } else {
                     ucur[k] = (cv[k] && rowin) ? ap[(long long)Jo * NX + col[k]] : R(0);
-                    vcur[k] = (cv[k] && rowin) ? ap[((long long)NY + Jo) * NX + col[k]] : R(0);
-                    vnext[k] = (cv[k] && J + 1 < NY) ? ap[((long long)NY + Jo + 1) * NX + col[k]] : R(0);
                 }
+                vcur[k] = rowin ? vnr[T1][k] : R(0);
+                vnext[k] = J + 1 < NY ? vel_v(k, Jo + 1) : R(0);
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
             }
         }
+        R vl_next[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) vl_next[k] = (S::MAUX > 0 && !S::FW) ? from_left<CPL>(vnext, k) : R(0);
+
         // f-wave transport: edge data of row J (left/right/bottom/top edge
         // velocities, x/y edge lengths, aux components 1..6), of row J+1 (the
         // x-faces' transverse splits: bottom/top velocities, y length) and of
@@ -611,12 +642,9 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                 for (int k = 0; k < CPL; k++) cda[k][n] = S::NC > 0 ? from_left<CPL>(t, k) : R(0);
             }
-            R vl_cur[CPL], vl_next[CPL];
+            R vl_cur[CPL];
 #pragma unroll
-            for (int k = 0; k < CPL; k++) {
-                vl_cur[k] = from_left<CPL>(vcur, k);
-                vl_next[k] = from_left<CPL>(vnext, k);
-            }
+            for (int k = 0; k < CPL; k++) vl_cur[k] = rowin ? vlr[T1][k] : R(0);
             // f-wave: the low-side cell's right-edge velocity and its top-edge
             // velocity and y length in rows J and J+1
             R fw_uRl[CPL], fw_vTl[CPL], fw_eyl[CPL], fw_vTnl[CPL], fw_eynl[CPL];
@@ -684,24 +712,27 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                         for (int m = 0; m < ME; m++)
                             if (!(SKIPZ && S::template wz<0>(w, m))) d = d + fx[k].w[w][m] * wr[k][m];
-                        dotr[k][w] = d;
+                        dotr[k][w] = RAT ? wr[k][0] : d;
                     }
                 }
 #pragma unroll
                 for (int w = 0; w < MW; w++) {
                     R t[CPL];
 #pragma unroll
-                    for (int k = 0; k < CPL; k++) t[k] = dotr[k][w];
+                    for (int k = 0; k < CPL; k++) t[k] = RAT ? fx[k].w[w][0] : dotr[k][w];
 #pragma unroll
                     for (int k = 0; k < CPL; k++) {
                         R dotl = from_left<CPL>(t, k);
                         if (mth[w] != 0 && col[k] >= 2 && col[k] <= NX - 2) {
                             R wn = R(0);
+                            if constexpr (!RAT) {
 #pragma unroll
-                            for (int m = 0; m < ME; m++)
-                                if (!(SKIPZ && S::template wz<0>(w, m))) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
-                            if (M<R>::lim_ok(wn)) {
-                                R r = M<R>::divl(fx[k].s[w] > R(0) ? dotl : dotr[k][w], wn);
+                                for (int m = 0; m < ME; m++)
+                                    if (!(SKIPZ && S::template wz<0>(w, m))) wn = wn + fx[k].w[w][m] * fx[k].w[w][m];
+                            }
+                            if (RAT ? lim_ok1(fx[k].w[w][0]) : M<R>::lim_ok(wn)) {
+                                R r = RAT ? (fx[k].s[w] > R(0) ? dotl : dotr[k][w]) * rcp1(fx[k].w[w][0])
+                                          : M<R>::divl(fx[k].s[w] > R(0) ? dotl : dotr[k][w], wn);
                                 R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
                                 for (int m = 0; m < ME; m++)
@@ -871,7 +902,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
                     for (int m = 0; m < ME; m++)
                         if (!(SKIPZ && S::template wz<1>(w, m))) d = d + fyp[k].w[w][m] * fyn[k].w[w][m];
-                    dotry[k][w] = d;
+                    // one-equation FP32: the unlimited wave of face J-1 itself
+                    dotry[k][w] = RAT ? fyp[k].w[w][0] : d;
                 }
         }
         if (Jr >= 3) {
@@ -893,11 +925,15 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     for (int w = 0; w < MW; w++) {
                         if (mth[w] != 0) {
                             R wn = R(0);
+                            if constexpr (!RAT) {
 #pragma unroll
-                            for (int m = 0; m < ME; m++)
-                                if (!(SKIPZ && S::template wz<1>(w, m))) wn = wn + f.w[w][m] * f.w[w][m];
-                            if (M<R>::lim_ok(wn)) {
-                                R r = M<R>::divl(f.s[w] > R(0) ? dotly[k][w] : dotry[k][w], wn);
+                                for (int m = 0; m < ME; m++)
+                                    if (!(SKIPZ && S::template wz<1>(w, m))) wn = wn + f.w[w][m] * f.w[w][m];
+                            }
+                            // RAT: dotly is the wave of face J-2, fyn the (unlimited) face J
+                            if (RAT ? lim_ok1(f.w[w][0]) : M<R>::lim_ok(wn)) {
+                                R r = RAT ? (f.s[w] > R(0) ? dotly[k][w] : fyn[k].w[w][0]) * rcp1(f.w[w][0])
+                                          : M<R>::divl(f.s[w] > R(0) ? dotly[k][w] : dotry[k][w], wn);
                                 R phi = limiter_phi<R>(mth[w], r);
 #pragma unroll
                                 for (int m = 0; m < ME; m++)
@@ -1084,10 +1120,22 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
 #pragma unroll
             for (int w = 0; w < MW; w++) dotry_w[k][w] = dotry[k][w];
             fyn_w[k] = fyn[k];
+            if constexpr (S::MAUX > 0 && !S::FW) {
+                vnr[T0][k] = vnext[k];
+                vlr[T0][k] = vl_next[k];
+            }
             tyn_w[k] = tyn[k];
         }
     };
 
+    // the advection velocity v of the sweep's first row, as its predecessor
+    // row would have made it
+    if constexpr (S::MAUX > 0 && !S::FW) {
+#pragma unroll
+        for (int k = 0; k < CPL; k++) vnr[1][k] = vel_v(k, J0);
+#pragma unroll
+        for (int k = 0; k < CPL; k++) vlr[1][k] = from_left<CPL>(vnr[1], k);
+    }
     if constexpr (UNR == 4) {
         if constexpr (PF) {
 #pragma unroll
@@ -1112,6 +1160,8 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             for (int k = 0; k < CPL; k++) {
                 ur[2][k] = ur[3][k];
                 ur[3][k] = ur[0][k];
+                vnr[1][k] = vnr[0][k];
+                vlr[1][k] = vlr[0][k];
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
                     qr[2][k][m] = qr[3][k][m];
