@@ -46,13 +46,13 @@ HBM_PEAK = 8.0e12
 # SQ_ACTIVE_INST_VALU x 4 cycles over the SIMD-cycles of the step launches of
 # a coarse step (profiles/k_step2_swe_humps_l7-9_sq_counters.md).
 MEASURED_MESH = {7: 10481, 8: 2349, 9: 85052}
-TRAFFIC_PER_LAUNCH = 3.583e9
+TRAFFIC_PER_LAUNCH = 3.586e9
 VALU_ACTIVE = 0.87
 # FP32 VALU work of the step launches of a coarse step per cell update:
 # wave-level VALU instructions, and FP32 operations issued (64 lanes x (ADD +
 # MUL + 2 FMA + TRANS) wave instructions, SQ_INSTS_VALU_*_F32;
 # profiles/k_step2_swe_humps_l7-9_valu_mix.md).
-VALU_INSTS_PER_CELL = 12.26
+VALU_INSTS_PER_CELL = 11.86
 FLOP_PER_CELL = 748.2
 # MI355X FP32 vector peak without packed math: 256 CUs x 4 SIMDs x 16 lanes
 # x 2 (FMA) x 2.4 GHz.  The step kernels are built without packed FP32.
