@@ -102,9 +102,13 @@ template <class OP, int CAP = SEG_MAX> struct Segs {
 // register resets of a time sync: three face lists per level
 using ResetSegs = Segs<int, 3 * MAXL>;
 // time interpolations run in the average launch of a ghost fill: the weight
-// of each list (one list per interpolated level)
+// of each list (one list per interpolated level), and per level the weight of
+// an interpolation pending in the same launch (bit L of pend set), for the
+// copies that read the interpolated pool of that level in that launch
 template <class R> struct TiWeights {
     R a[SEG_MAX];
+    R al[MAXL];
+    unsigned pend;
 };
 
 struct XferOp {         // regrid transfer into the new pools

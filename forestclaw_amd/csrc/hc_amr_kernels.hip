@@ -79,6 +79,23 @@ __device__ inline void ti_ring_cell(const TiOp& op, int k, const Pools<R>& P, in
     }
 }
 
+// A copy from the time-interpolated pool of a level whose interpolation runs
+// in the same launch: the source cell interpolated here, with the operations
+// of ti_ring_cell (so the same bits), instead of read from the pool that the
+// launch is still writing.
+template <class R>
+__device__ inline void copy_ti_direct(R* __restrict__ D, const Pools<R>& P, const GhostOp& op, int i, int j,
+                                      long long T, int NX, int NY, int meqn, R alpha)
+{
+    const long long base = (long long)op.src * T * meqn;
+    const R* qc = P.q[op.slev] + base;
+    const R* ql = P.qlast[op.slev] + base;
+    for (int m = 0; m < meqn; m++) {
+        const long long s = ((long long)m * NY + j + op.oy) * NX + i + op.ox;
+        D[((long long)m * NY + j) * NX + i] = ql[s] + alpha * (qc[s] - ql[s]);
+    }
+}
+
 template <class R>
 __global__ void k_ghost(const Segs<GhostOp> SG, const Segs<TiOp> ST, const TiWeights<R> TW, Pools<R> P, int mx,
                         int my, int meqn)
@@ -118,6 +135,10 @@ __global__ void k_ghost(const Segs<GhostOp> SG, const Segs<TiOp> ST, const TiWei
         default: j = MBC + my + layer; i = along; break;
         }
         if (op.kind == G_COPY_FACE) {
+            if ((op.flags & 2) && ((TW.pend >> op.slev) & 1u)) {
+                copy_ti_direct<R>(D, P, op, i, j, T, NX, NY, meqn, TW.al[op.slev]);
+                return;
+            }
             for (int m = 0; m < meqn; m++) AT(D, i, j, m) = AT(S, i + op.ox, j + op.oy, m);
         } else {
             const int fi = 2 * i - MBC + op.ox, fj = 2 * j - MBC + op.oy;
@@ -140,6 +161,10 @@ __global__ void k_ghost(const Segs<GhostOp> SG, const Segs<TiOp> ST, const TiWei
         const int i = (c & 1) ? MBC + mx - 1 + ibc : MBC - ibc;
         const int j = (c & 2) ? MBC + my - 1 + jbc : MBC - jbc;
         if (op.kind == G_COPY_CORNER) {
+            if ((op.flags & 2) && ((TW.pend >> op.slev) & 1u)) {
+                copy_ti_direct<R>(D, P, op, i, j, T, NX, NY, meqn, TW.al[op.slev]);
+                return;
+            }
             for (int m = 0; m < meqn; m++) AT(D, i, j, m) = AT(S, i + op.ox, j + op.oy, m);
         } else {
             const int fi = 2 * i - MBC + op.ox, fj = 2 * j - MBC + op.oy;
@@ -699,12 +724,18 @@ __global__ void k_copy_tiles(const int* __restrict__ slots, int n, const R* __re
 static inline unsigned nblocks(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 template <class R>
-void launch_ghost_ti(const Segs<GhostOp>& S, const Segs<TiOp>& T, const double* alpha, const Pools<R>& P,
-                     int mx, int my, int meqn, hipStream_t s)
+void launch_ghost_ti(const Segs<GhostOp>& S, const Segs<TiOp>& T, const double* alpha, const int* ti_levels,
+                     const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
 {
     if (S.total() <= 0 && T.total() <= 0) return;
     TiWeights<R> w{};
-    for (int k = 0; k < T.n; k++) w.a[k] = R(alpha[k]);
+    for (int k = 0; k < T.n; k++) {
+        w.a[k] = R(alpha[k]);
+        if (ti_levels && ti_levels[k] >= 0 && ti_levels[k] < MAXL) {
+            w.al[ti_levels[k]] = w.a[k];
+            w.pend |= 1u << ti_levels[k];
+        }
+    }
     const long long n =
         (long long)S.total() * MBC * (mx > my ? mx : my) + (long long)T.total() * ti_ring_size(mx, my);
     hipLaunchKernelGGL(k_ghost<R>, dim3(nblocks(n, 256)), dim3(256), 0, s, S, T, w, P, mx, my, meqn);
@@ -713,7 +744,7 @@ void launch_ghost_ti(const Segs<GhostOp>& S, const Segs<TiOp>& T, const double* 
 template <class R>
 void launch_ghost(const Segs<GhostOp>& S, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
 {
-    launch_ghost_ti<R>(S, Segs<TiOp>(), nullptr, P, mx, my, meqn, s);
+    launch_ghost_ti<R>(S, Segs<TiOp>(), nullptr, nullptr, P, mx, my, meqn, s);
 }
 
 template <class R>
@@ -809,7 +840,7 @@ void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long 
     template void launch_ghost<R>(const GhostOp*, int, const Pools<R>&, int, int, int, hipStream_t); \
     template void launch_bc<R>(const BcOp*, int, const Pools<R>&, int, int, int, hipStream_t);       \
     template void launch_ghost<R>(const Segs<GhostOp>&, const Pools<R>&, int, int, int, hipStream_t); \
-    template void launch_ghost_ti<R>(const Segs<GhostOp>&, const Segs<TiOp>&, const double*,             \
+    template void launch_ghost_ti<R>(const Segs<GhostOp>&, const Segs<TiOp>&, const double*, const int*, \
                                      const Pools<R>&, int, int, int, hipStream_t);                     \
     template void launch_bc<R>(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);       \
     template void launch_timeinterp<R>(const TiOp*, int, const Pools<R>&, int, int, int, double,     \

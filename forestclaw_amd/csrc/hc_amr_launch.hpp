@@ -27,8 +27,8 @@ template <class R> void launch_ghost(const Segs<GhostOp>&, const Pools<R>&, int,
 // ghost operations and, beside them in the same launch, time interpolations
 // (list k with weight alpha[k]) of the ring cells that ghost fills read
 template <class R>
-void launch_ghost_ti(const Segs<GhostOp>&, const Segs<TiOp>&, const double* alpha, const Pools<R>&, int, int,
-                     int, hipStream_t);
+void launch_ghost_ti(const Segs<GhostOp>&, const Segs<TiOp>&, const double* alpha, const int* ti_levels,
+                     const Pools<R>&, int, int, int, hipStream_t);
 template <class R> void launch_bc(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);
 template <class R>
 void launch_timeinterp(const TiOp*, int, const Pools<R>&, int, int, int, double, hipStream_t);

@@ -372,19 +372,35 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
             for (int L = maxlevel; L >= minlevel; L--) bca.add(plan[L].bc.p, plan[L].bc.n);
             if (use_ti) bca.add(plan[tl].bc_ti.p, plan[tl].bc_ti.n);
             // the pending time interpolations write the interior ring of the
-            // interpolated pool, which no average reads or writes
+            // interpolated pool, which no average or copy writes
             Segs<TiOp> tis;
             double alpha[SEG_MAX];
+            int tilev[SEG_MAX];
             for (const auto& ti : ti_pending) {
                 if (plan[ti.first].ti.n <= 0) continue;
-                if (tis.n < SEG_MAX) alpha[tis.n] = ti.second;
+                if (tis.n < SEG_MAX) {
+                    alpha[tis.n] = ti.second;
+                    tilev[tis.n] = ti.first;
+                }
                 tis.add(plan[ti.first].ti.p, plan[ti.first].ti.n);
             }
             ti_pending.clear();
             if (!avg.ok || !copy.ok || !bcc.ok || !bca.ok || !tis.ok)
                 return set_error(HIPCLAW_ERR_UNSUPPORTED, "ghost update: too many operation lists");
-            launch_ghost_ti<R>(avg, tis, alpha, P, mx, my, meqn, stream);
-            launch_ghost<R>(copy, P, mx, my, meqn, stream);
+            // averages and copies share a launch when their lists fit one:
+            // each face and corner of a patch has one neighbour, so the two
+            // phases write disjoint ghost cells, and both read interior cells
+            // only.  A copy from the interpolated pool of a level whose time
+            // interpolation runs in that launch interpolates its source cell
+            // itself.
+            Segs<GhostOp> first = avg;
+            for (int s = 0; s < copy.n; s++) first.add(copy.ops[s], copy.start[s + 1] - copy.start[s]);
+            if (first.ok) {
+                launch_ghost_ti<R>(first, tis, alpha, tilev, P, mx, my, meqn, stream);
+            } else {
+                launch_ghost_ti<R>(avg, tis, alpha, tilev, P, mx, my, meqn, stream);
+                launch_ghost<R>(copy, P, mx, my, meqn, stream);
+            }
             launch_bc<R>(bcc, P, mx, my, meqn, stream);
             for (int L = maxcoarse; L >= mincoarse; L--)
                 launch_ghost<R>(plan[L].interp.p, plan[L].interp.n, P, mx, my, meqn, stream);
