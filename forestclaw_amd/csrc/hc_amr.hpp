@@ -56,6 +56,17 @@ struct TsOp {
     int nbr[4][2];      // same size: [0]; finer: igrid 0, 1
 };
 
+// The last phase of a ghost fill (physical BCs) and the time sync of the same
+// levels on one patch, run by one workgroup in that order: the BCs read the
+// patch's interior cells before the corrections change its edge cells, as
+// when the sync follows the fill.  bc.slot < 0: no physical boundary;
+// ts.slot < 0: no correction.
+struct BcTsOp {
+    BcOp bc;
+    TsOp ts;
+    int level;
+};
+
 // Several operation lists in one launch: list s holds the operations
 // [start[s], start[s+1]) of the launch (one list per level of a ghost-fill
 // phase or a time sync, whose levels are independent of each other).

@@ -8,6 +8,7 @@
 // 0-based tile index i corresponds to the reference's i_f = i - 1.
 #include "hc_amr.hpp"
 #include "hc_amr_launch.hpp"
+#include "hc_bc.hpp"
 #include "hc_common.hpp"
 
 namespace hc {
@@ -225,30 +226,8 @@ __global__ void k_bc_list(const Segs<BcOp> S, Pools<R> P, int mx, int my, int me
     const int o = blockIdx.x;
     if (o >= S.total()) return;
     const BcOp op = S.at(o);
-    const int NX = mx + 2 * MBC, NY = my + 2 * MBC;
-    R* qp = pool_ptr(P, op.level, op.flags & 1) + (long long)op.slot * NX * NY * meqn;
-    const int i1 = MBC, imx = mx + MBC - 1, j1 = MBC, jmy = my + MBC - 1;
-#define QB(i, j, m) qp[((long long)(m) * NY + (j)) * NX + (i)]
-    for (int t = threadIdx.x; t < NY * meqn; t += blockDim.x) {
-        const int m = t / NY, j = t - m * NY;
-        for (int ibc = 1; ibc <= MBC; ibc++) {
-            if (op.mthbc[0] == 1) QB(i1 - ibc, j, m) = QB(i1, j, m);
-            else if (op.mthbc[0] == 3) QB(i1 - ibc, j, m) = (m == 1) ? -QB(i1 + ibc - 1, j, m) : QB(i1 + ibc - 1, j, m);
-            if (op.mthbc[1] == 1) QB(imx + ibc, j, m) = QB(imx, j, m);
-            else if (op.mthbc[1] == 3) QB(imx + ibc, j, m) = (m == 1) ? -QB(imx + 1 - ibc, j, m) : QB(imx + 1 - ibc, j, m);
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NX * meqn; t += blockDim.x) {
-        const int m = t / NX, i = t - m * NX;
-        for (int jbc = 1; jbc <= MBC; jbc++) {
-            if (op.mthbc[2] == 1) QB(i, j1 - jbc, m) = QB(i, j1, m);
-            else if (op.mthbc[2] == 3) QB(i, j1 - jbc, m) = (m == 2) ? -QB(i, j1 + jbc - 1, m) : QB(i, j1 + jbc - 1, m);
-            if (op.mthbc[3] == 1) QB(i, jmy + jbc, m) = QB(i, jmy, m);
-            else if (op.mthbc[3] == 3) QB(i, jmy + jbc, m) = (m == 2) ? -QB(i, jmy + 1 - jbc, m) : QB(i, jmy + 1 - jbc, m);
-        }
-    }
-#undef QB
+    const long long T = (long long)(mx + 2 * MBC) * (my + 2 * MBC);
+    bc_patch<R>(op, pool_ptr(P, op.level, op.flags & 1) + (long long)op.slot * T * meqn, mx, my, meqn);
 }
 
 // ---------------------------------------------------------------------------

@@ -25,7 +25,9 @@ template <class R> void launch_bc(const BcOp*, int, const Pools<R>&, int, int, i
 // the same over several lists in one launch
 template <class R> void launch_ghost(const Segs<GhostOp>&, const Pools<R>&, int, int, int, hipStream_t);
 // ghost operations and, beside them in the same launch, time interpolations
-// (list k with weight alpha[k]) of the ring cells that ghost fills read
+// (list k of level ti_levels[k], weight alpha[k]) of the ring cells that ghost
+// fills read; a copy from the interpolated pool of such a level interpolates
+// its source cell itself
 template <class R>
 void launch_ghost_ti(const Segs<GhostOp>&, const Segs<TiOp>&, const double* alpha, const int* ti_levels,
                      const Pools<R>&, int, int, int, hipStream_t);
@@ -55,6 +57,10 @@ template <class R>
 void launch_ts_correct(const Segs<TsOp>&, R* const* qs, const double*, long long, const double*, int, int,
                        int, hipStream_t);
 void launch_ts_reset(const ResetSegs&, double*, long long, int, int, int, hipStream_t);
+// physical BCs, then time-sync corrections, one patch per workgroup
+template <class R>
+void launch_bc_ts(const Segs<BcTsOp>&, const Pools<R>&, const double* reg, long long regstride,
+                  const double* dxdy, int, int, int, hipStream_t);
 // Network input tiles of n patches (bf16 [n][36][36][8]); returns HIPCLAW_*.
 template <class R>
 int launch_make_tiles(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double,

@@ -313,6 +313,32 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
         HC_TRY(lp.ts_same.upload(hp.ts_same[L]));
         HC_TRY(lp.ts_f2c.upload(hp.ts_f2c[L]));
         HC_TRY(lp.ts_phys.upload(hp.ts_phys[L]));
+        // the physical BCs and the time-sync corrections of a patch as one
+        // operation (a fill's last phase fused with the sync that follows)
+        std::vector<BcTsOp> bt, btti;
+        std::vector<int> at((size_t)nn, -1);
+        auto get = [&](int slot) -> BcTsOp& {
+            if (at[slot] < 0) {
+                BcTsOp o{};
+                o.bc.slot = -1;
+                o.ts.slot = -1;
+                o.level = L;
+                at[slot] = (int)bt.size();
+                bt.push_back(o);
+            }
+            return bt[at[slot]];
+        };
+        for (const BcOp& b : bc[L]) get(b.slot).bc = b;
+        for (const TsOp& o : hp.ts[L]) get(o.slot).ts = o;
+        for (const BcOp& b : bti) {
+            BcTsOp o{};
+            o.bc = b;
+            o.ts.slot = -1;
+            o.level = L;
+            btti.push_back(o);
+        }
+        HC_TRY(lp.bcts.upload(bt));
+        HC_TRY(lp.bcts_ti.upload(btti));
     }
     // same-level push table: the copy op (dst k, face / corner c, src nb)
     // becomes "nb writes into k across its opposite face / corner"
@@ -341,11 +367,15 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
 // The physical BC pass on the fine levels that precedes the final pass in
 // the reference is subsumed by it (bc2 is idempotent on unchanged data).
 // ---------------------------------------------------------------------------
-int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
+int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp, bool sync)
 {
     if (par_dry) {
         ti_pending.clear();
         return HIPCLAW_OK;
+    }
+    if (sync && !(merge_levels && tsync)) {
+        HC_TRY(ghost_update(minlevel, maxlevel, time_interp, false));
+        return time_sync(minlevel, maxlevel);
     }
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
     const int mincoarse = minlevel, maxcoarse = maxlevel - 1;
@@ -359,7 +389,11 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
             // every operation writes ghost cells of its own patch from
             // interior cells only, so the levels of a phase are independent;
             // interpolation reads coarse ghost cells that the interpolation
-            // of the next coarser level writes and keeps its level order
+            // of the next coarser level writes and keeps its level order.
+            // With the time sync following (sync), the final BC phase and the
+            // corrections run per patch in one launch: a patch's BCs and
+            // corrections touch only that patch's cells, and its
+            // corrections read registers only.
             Segs<GhostOp> avg, copy;
             Segs<BcOp> bcc, bca;
             for (int L = maxcoarse; L >= mincoarse; L--) avg.add(plan[L].avg.p, plan[L].avg.n);
@@ -385,6 +419,14 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
                 tis.add(plan[ti.first].ti.p, plan[ti.first].ti.n);
             }
             ti_pending.clear();
+            Segs<BcTsOp> bct;
+            ResetSegs rs;
+            if (sync) {
+                for (int L = maxlevel; L >= minlevel; L--) bct.add(plan[L].bcts.p, plan[L].bcts.n);
+                if (use_ti) bct.add(plan[tl].bcts_ti.p, plan[tl].bcts_ti.n);
+                if (!bct.ok || !reset_segs(minlevel, maxlevel, rs))
+                    return set_error(HIPCLAW_ERR_UNSUPPORTED, "ghost update: too many operation lists");
+            }
             if (!avg.ok || !copy.ok || !bcc.ok || !bca.ok || !tis.ok)
                 return set_error(HIPCLAW_ERR_UNSUPPORTED, "ghost update: too many operation lists");
             // averages and copies share a launch when their lists fit one:
@@ -405,6 +447,12 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp)
             for (int L = maxcoarse; L >= mincoarse; L--)
                 launch_ghost<R>(plan[L].interp.p, plan[L].interp.n, P, mx, my, meqn, stream);
             if (use_ti) launch_ghost<R>(plan[tl].interp_ti.p, plan[tl].interp_ti.n, P, mx, my, meqn, stream);
+            if (sync) {
+                launch_bc_ts<R>(bct, P, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
+                clear_pushed();
+                launch_ts_reset(rs, d_reg, regstride, mx, my, meqn, stream);
+                return check_hip(hipGetLastError(), "ghost update and time sync");
+            }
             launch_bc<R>(bca, P, mx, my, meqn, stream);
             return check_hip(hipGetLastError(), "ghost update");
         }
@@ -611,6 +659,19 @@ int Ctx::set_time_sync(bool on)
 // (src/fclaw2d_time_sync.c:118-154): same-size faces of levels
 // [minlevel, maxlevel], coarse/fine faces between levels L and L+1 for L in
 // [minlevel, maxlevel-1], physical faces of [minlevel, maxlevel].
+// register faces a sync of [minlevel, maxlevel] resets
+// (fclaw2d_clawpatch_conservation.c:207-280): same-size and physical faces of
+// every level, coarse-fine faces between two synchronised levels
+bool Ctx::reset_segs(int minlevel, int maxlevel, ResetSegs& rs) const
+{
+    for (int L = minlevel; L <= maxlevel; L++) {
+        rs.add(plan[L].ts_same.p, plan[L].ts_same.n);
+        if (L < maxlevel) rs.add(plan[L].ts_f2c.p, plan[L].ts_f2c.n);
+        rs.add(plan[L].ts_phys.p, plan[L].ts_phys.n);
+    }
+    return rs.ok;
+}
+
 int Ctx::time_sync(int minlevel, int maxlevel)
 {
     if (!tsync || par_dry) return HIPCLAW_OK;
@@ -630,12 +691,8 @@ int Ctx::time_sync(int minlevel, int maxlevel)
             qs[cs.n] = esz == 8 ? (void*)pools<double>().q[L] : (void*)pools<float>().q[L];
             cs.add(plan[L].ts.p, plan[L].ts.n);
         }
-        for (int L = minlevel; L <= maxlevel; L++) {
-            rs.add(plan[L].ts_same.p, plan[L].ts_same.n);
-            if (L < maxlevel) rs.add(plan[L].ts_f2c.p, plan[L].ts_f2c.n);
-            rs.add(plan[L].ts_phys.p, plan[L].ts_phys.n);
-        }
-        if (!cs.ok || !rs.ok) return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync: too many face lists");
+        if (!cs.ok || !reset_segs(minlevel, maxlevel, rs))
+            return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync: too many face lists");
         if (esz == 8)
             launch_ts_correct<double>(cs, (double* const*)qs, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
         else
@@ -1000,22 +1057,14 @@ int Ctx::advance_body(double t, double dt)
                 while (c[hi].last_step % c[tl].step_inc == 0) tl--;
                 // all present levels synchronised (a time-interpolated level
                 // below lo has no patches): the update of [lo, hi]
-                if (tl < lo) {
-                    HC_TRY(ghost_update(lo, hi, 0));
-                    HC_TRY(time_sync(lo, hi));
-                } else {
-                    HC_TRY(ghost_update(tl + 1, hi, 1));
-                    HC_TRY(time_sync(tl + 1, hi));
-                }
+                if (tl < lo) HC_TRY(ghost_update(lo, hi, 0, true));
+                else HC_TRY(ghost_update(tl + 1, hi, 1, true));
             } else {
-                HC_TRY(ghost_update(lo, hi, 0));
-                HC_TRY(time_sync(lo, hi));
+                HC_TRY(ghost_update(lo, hi, 0, true));
             }
         }
     }
-    HC_TRY(ghost_update(lo, hi, 0));
-    HC_TRY(time_sync(lo, hi));
-    return HIPCLAW_OK;
+    return ghost_update(lo, hi, 0, true);
 }
 
 // Copies of every level's current tiles (interiors and ghost cells) into the

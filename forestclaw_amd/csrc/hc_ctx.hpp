@@ -36,6 +36,7 @@ struct LevelPlan {
     DevArray<TiOp> ti;                   // patches with finer neighbours
     DevArray<TsOp> ts;                   // time-sync corrections of this level's patches
     DevArray<int> ts_same, ts_f2c, ts_phys;  // register faces reset by a sync
+    DevArray<BcTsOp> bcts, bcts_ti;      // physical BCs + corrections per patch
     int s0 = 0, s1 = 0;                  // slot range of the level
 };
 
@@ -111,7 +112,8 @@ struct Ctx {
     void release_pools();
     int rebuild(bool keep_data, const std::vector<Transfer>* xfer);
     template <class R> Pools<R> pools() const;
-    int ghost_update(int minlevel, int maxlevel, int time_interp);
+    // sync: the time sync of the same levels follows (time_sync)
+    int ghost_update(int minlevel, int maxlevel, int time_interp, bool sync = false);
     int update_level(int level, double t, double dt);
     Src2Args src2_args(int s0, int np) const;
     bool fused_swirl() const;
@@ -195,6 +197,7 @@ struct Ctx {
     int set_time_sync(bool on);
     int make_tiles(void* tiles, int* tags);
     int time_sync(int minlevel, int maxlevel);
+    bool reset_segs(int minlevel, int maxlevel, ResetSegs& rs) const;
 };
 
 }  // namespace hc

@@ -16,6 +16,7 @@
 // cell: same-size faces 0..3, then finer faces 0..3 (igrid 0 then 1), every
 // addend applied in FP64 (a corner cell receives two faces' addends).
 #include "hc_amr_launch.hpp"
+#include "hc_bc.hpp"
 #include "hc_common.hpp"
 #include "hc_step2.hpp"
 
@@ -59,20 +60,14 @@ template <class R> struct QList {
     R* q[SEG_MAX];
 };
 
-// One workgroup of TS_THREADS per patch (the addends of all four faces in
-// one pass, then the edge cells).  LDS: add[f][g][m][k], f < 4, g < 2.
-constexpr int TS_THREADS = 256;
+// Corrections of one patch by one workgroup (the addends of all four faces in
+// one pass, then the edge cells).  add: LDS add[f][g][m][k], f < 4, g < 2.
 template <class R>
-__global__ void __launch_bounds__(TS_THREADS)
-k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ reg, long long regstride,
-             const double* __restrict__ dxdy, int mx, int my, int meqn)
+__device__ inline void ts_correct_patch(const TsOp& o, R* __restrict__ q, const double* __restrict__ reg,
+                                        long long regstride, const double* __restrict__ dxdy, int mx, int my,
+                                        int meqn, double* add)
 {
 #pragma clang fp contract(off)
-    extern __shared__ double add[];
-    const int op = blockIdx.x;
-    if (op >= S.total()) return;
-    const TsOp o = S.at(op);
-    R* __restrict__ q = qs.q[S.seg(op)];
     const int maxlen = mx > my ? mx : my;
     const int NX = mx + 4, NY = my + 4;
     const double area = dxdy[2 * o.slot] * dxdy[2 * o.slot + 1];
@@ -128,6 +123,37 @@ k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ r
     }
 }
 
+// One workgroup of TS_THREADS per patch.
+constexpr int TS_THREADS = 256;
+template <class R>
+__global__ void __launch_bounds__(TS_THREADS)
+k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ reg, long long regstride,
+             const double* __restrict__ dxdy, int mx, int my, int meqn)
+{
+    extern __shared__ double add[];
+    const int op = blockIdx.x;
+    if (op >= S.total()) return;
+    ts_correct_patch<R>(S.at(op), qs.q[S.seg(op)], reg, regstride, dxdy, mx, my, meqn, add);
+}
+
+// Physical BCs, then the corrections, of one patch per workgroup.
+template <class R>
+__global__ void __launch_bounds__(TS_THREADS)
+k_bc_ts(const Segs<BcTsOp> S, Pools<R> P, const double* __restrict__ reg, long long regstride,
+        const double* __restrict__ dxdy, int mx, int my, int meqn)
+{
+    extern __shared__ double add[];
+    const int op = blockIdx.x;
+    if (op >= S.total()) return;
+    const BcTsOp o = S.at(op);
+    const long long T = (long long)(mx + 4) * (my + 4) * meqn;
+    if (o.bc.slot >= 0) {
+        bc_patch<R>(o.bc, ((o.bc.flags & 1) ? P.ti : P.q[o.level]) + (long long)o.bc.slot * T, mx, my, meqn);
+        __syncthreads();
+    }
+    if (o.ts.slot >= 0) ts_correct_patch<R>(o.ts, P.q[o.level], reg, regstride, dxdy, mx, my, meqn, add);
+}
+
 // Zero the four register blocks of listed faces (entry = slot*4 + face).
 __global__ void k_ts_reset(const ResetSegs S, double* __restrict__ reg, long long regstride, int mx, int my,
                            int meqn)
@@ -157,6 +183,16 @@ void launch_ts_correct(const Segs<TsOp>& S, R* const* qs, const double* reg, lon
 }
 
 template <class R>
+void launch_bc_ts(const Segs<BcTsOp>& S, const Pools<R>& P, const double* reg, long long regstride,
+                  const double* dxdy, int mx, int my, int meqn, hipStream_t s)
+{
+    if (S.total() <= 0) return;
+    const size_t lds = sizeof(double) * 8 * meqn * (mx > my ? mx : my);
+    hipLaunchKernelGGL(k_bc_ts<R>, dim3(S.total()), dim3(TS_THREADS), lds, s, S, P, reg, regstride, dxdy, mx,
+                       my, meqn);
+}
+
+template <class R>
 void launch_ts_correct(const TsOp* ops, int nops, R* q, const double* reg, long long regstride,
                        const double* dxdy, int mx, int my, int meqn, hipStream_t s)
 {
@@ -183,6 +219,10 @@ void launch_ts_reset(const int* faces, int n, double* reg, long long regstride, 
     launch_ts_reset(S, reg, regstride, mx, my, meqn, s);
 }
 
+template void launch_bc_ts<double>(const Segs<BcTsOp>&, const Pools<double>&, const double*, long long,
+                                   const double*, int, int, int, hipStream_t);
+template void launch_bc_ts<float>(const Segs<BcTsOp>&, const Pools<float>&, const double*, long long,
+                                  const double*, int, int, int, hipStream_t);
 template void launch_ts_correct<double>(const Segs<TsOp>&, double* const*, const double*, long long,
                                         const double*, int, int, int, hipStream_t);
 template void launch_ts_correct<float>(const Segs<TsOp>&, float* const*, const double*, long long,
