@@ -57,6 +57,8 @@ template <class R>
 void launch_ts_correct(const Segs<TsOp>&, R* const* qs, const double*, long long, const double*, int, int,
                        int, hipStream_t);
 void launch_ts_reset(const ResetSegs&, double*, long long, int, int, int, hipStream_t);
+// n doubles set to zero by a kernel (safe under stream capture)
+void launch_zero(double* p, long long n, hipStream_t s);
 // physical BCs, then time-sync corrections, one patch per workgroup
 template <class R>
 void launch_bc_ts(const Segs<BcTsOp>&, const Pools<R>&, const double* reg, long long regstride,

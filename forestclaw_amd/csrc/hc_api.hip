@@ -475,6 +475,9 @@ int step2_launch(const hipclaw_physics* phys, int npatch, const void* q_in, void
         a.xylower = ml->xylower;
         a.par = ml->par;
         a.cflmax = ml->cflmax;
+        a.reg_fresh = ml->reg_fresh;
+        a.reg_lo = ml->reg_lo;
+        a.reg_cfaces = ml->reg_cfaces;
         for (int L = 0; L < STEP_MAXL; L++) a.vtl[L] = ml->vtl[L];
         // levels the launch does not cover get q_in, the pool every slot of
         // the launch lives in: the kernels' lanes without a patch read from

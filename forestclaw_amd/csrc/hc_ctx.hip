@@ -130,7 +130,7 @@ void Ctx::release_pools()
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
-    int** ibufs[] = {&d_levels, &d_iota, &d_tags, &d_push};
+    int** ibufs[] = {&d_levels, &d_iota, &d_tags, &d_push, &d_cfaces};
     for (auto b : ibufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
@@ -356,6 +356,11 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
     // coarse step, when every synchronised register has been reset)
     regstride = 8LL * phys.meqn * (phys.mx + phys.my);
     if (tsync) HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)nn * regstride));
+    reg_clean = true;
+    reg_stale = false;
+    HC_TRY(dmalloc((void**)&d_cfaces, sizeof(int) * (size_t)nn));
+    HC_TRY(check_hip(hipMemcpy(d_cfaces, hp.coarse_faces.data(), sizeof(int) * (size_t)nn, hipMemcpyHostToDevice),
+                     "coarse faces"));
     return check_hip(hipDeviceSynchronize(), "rebuild");
 }
 
@@ -450,7 +455,9 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp, bool sync)
             if (sync) {
                 launch_bc_ts<R>(bct, P, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
                 clear_pushed();
-                launch_ts_reset(rs, d_reg, regstride, mx, my, meqn, stream);
+                // a merged coarse step's updates write their first sums onto
+                // zero instead (StepArgs::reg_fresh)
+                if (!reg_fresh_run) launch_ts_reset(rs, d_reg, regstride, mx, my, meqn, stream);
                 return check_hip(hipGetLastError(), "ghost update and time sync");
             }
             launch_bc<R>(bca, P, mx, my, meqn, stream);
@@ -650,8 +657,21 @@ int Ctx::set_time_sync(bool on)
         return set_error(HIPCLAW_ERR_UNSUPPORTED, "time sync needs a conservative flux (SWE only)");
     tsync = on;
     clear_graphs();
-    if (on && !d_reg && n > 0) HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)n * regstride));
+    if (on && !d_reg && n > 0) {
+        HC_TRY(dmalloc((void**)&d_reg, sizeof(double) * (size_t)n * regstride));
+        reg_stale = false;
+    }
     return HIPCLAW_OK;
+}
+
+// Registers left by merged coarse steps: cleared before a per-level update,
+// a stand-alone time sync, a read-back or a coarse step that reads them.
+int Ctx::flush_registers()
+{
+    if (!reg_stale || !d_reg) return HIPCLAW_OK;
+    launch_zero(d_reg, (long long)n * regstride, stream);
+    reg_stale = false;
+    return check_hip(hipGetLastError(), "register clear");
 }
 
 // fclaw2d_time_sync(minlevel, maxlevel): corrections of every synchronised
@@ -789,6 +809,9 @@ int Ctx::update_levels(int lo, int hi, const double* tl, const double* dtl)
     // kernel (one per wave): no reduction launch per fine step
     const bool cfl_atomic = cflacc_count < 4095;
     ml.cflmax = cfl_atomic ? d_cflslot + cflacc_count : nullptr;
+    ml.reg_fresh = reg_fresh_run ? 1 : 0;
+    ml.reg_lo = lo;
+    ml.reg_cfaces = d_cfaces;
     HC_TRY(step2_launch(&phys, np, pool[0], pool[1], aux, d_dxdy, nullptr, np, dtl[hi], d_cfl,
                         tsync ? d_reg : nullptr, lc, stream, &ml));
     if (src2) {
@@ -880,6 +903,17 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
         return HIPCLAW_OK;
     };
     if (prepare && !graph_ok) return HIPCLAW_OK;  // nothing to capture
+    // time-sync registers: merged coarse steps from a clean state skip the
+    // register clears (StepArgs::reg_fresh); others clear what earlier
+    // merged steps left first.  Either way every register is zero for the
+    // drivers afterwards (the last sync of a coarse step covers all levels).
+    // A run that stops on an error leaves them to be cleared.
+    reg_fresh_run = tsync && merge_levels && reg_clean;
+    if (tsync && !prepare) {
+        if (!reg_fresh_run) HC_TRY(flush_registers());
+        reg_clean = true;
+        reg_stale = true;
+    }
     if (cflslot_dirty) HC_TRY(clear_cfl_slots());
     cflslot_dirty = true;
     if (!graph_ok) {
@@ -919,7 +953,8 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
         const unsigned pin = level_bits(parity), qin = level_bits(pushed);
         GraphEntry* e = nullptr;
         for (auto& g : graphs)
-            if (g.parity_in == pin && g.epoch == mesh_epoch && g.nstep == nstep) e = &g;
+            if (g.parity_in == pin && g.epoch == mesh_epoch && g.nstep == nstep && g.reg_fresh == reg_fresh_run)
+                e = &g;
         if (!e) {
             // capture: the body runs its host-side bookkeeping as usual while the
             // device work is recorded (its kernels read d_par); restore the
@@ -947,8 +982,8 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
                         (void)hipGraphExecDestroy(graphs.front().exec);
                         graphs.erase(graphs.begin());
                     }
-                    graphs.push_back({pin, qin, mesh_epoch, nstep, ex, level_bits(parity), level_bits(pushed),
-                                      cell_updates - cells0, cflacc_count});
+                    graphs.push_back({pin, qin, mesh_epoch, nstep, reg_fresh_run, ex, level_bits(parity),
+                                      level_bits(pushed), cell_updates - cells0, cflacc_count});
                     e = &graphs.back();
                     graph_captures++;
                 }
@@ -979,6 +1014,7 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
         }
         HC_TRY(check_hip(hipGraphLaunch(e->exec, stream), "graph launch"));
     }
+    if (tsync) reg_stale = reg_fresh_run;
     cflslot_dirty = false;
     if (maxcfl) {
         HC_TRY(check_hip(hipMemcpyAsync(maxcfl, cfl_dev, sizeof(double) * nstep, hipMemcpyDeviceToHost, stream),
@@ -1408,6 +1444,7 @@ int hipclaw_set_src2(hipclaw_ctx* ctx, int kind)
 int hipclaw_time_sync(hipclaw_ctx* ctx, int minlevel, int maxlevel)
 {
     CTX_READY(ctx);
+    HC_TRY(ctx->c->flush_registers());
     return ctx->c->time_sync(minlevel, maxlevel);
 }
 
@@ -1417,6 +1454,7 @@ int hipclaw_get_registers(hipclaw_ctx* ctx, double* host)
     Ctx& c = *ctx->c;
     if (!host) return set_error(HIPCLAW_ERR_ARG, "host is NULL");
     if (!c.d_reg) return set_error(HIPCLAW_ERR_ARG, "time sync is off");
+    HC_TRY(c.flush_registers());
     int rc = check_hip(hipStreamSynchronize(c.stream), "sync");
     if (rc) return rc;
     return check_hip(hipMemcpy(host, c.d_reg, sizeof(double) * (size_t)c.n * c.regstride,
@@ -1790,6 +1828,10 @@ int hipclaw_update_level(hipclaw_ctx* ctx, int level, double t, double dt, doubl
     if (c.cflslot_dirty) HC_TRY(c.clear_cfl_slots());
     c.cflacc_count = 0;
     c.cflslot_dirty = true;
+    if (c.tsync) {
+        HC_TRY(c.flush_registers());
+        c.reg_clean = false;  // its sums stay until a sync clears them
+    }
     int rc = c.update_level(level, t, dt);
     if (rc) return rc;
     if (c.cflacc_count > 0) HC_TRY(cfl_final(c.d_cflslot, c.cflacc_count, c.d_cflacc + 4095, c.stream));

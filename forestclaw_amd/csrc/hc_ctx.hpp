@@ -87,6 +87,16 @@ struct Ctx {
     int src2 = HIPCLAW_SRC2_NONE;  // source term after every patch step (src_term)
     bool fuse_aux = true;          // swirl velocities evaluated in the step kernel
     double* d_reg = nullptr;       // time-sync registers, regstride doubles per slot (FP64)
+    int* d_cfaces = nullptr;       // per slot: bit f set when face f has a coarser neighbour
+    // Register state.  reg_clean: every register is zero as far as the
+    // drivers are concerned (after a mesh change and after every coarse
+    // step, whose last sync covers all levels); reg_stale: they are not
+    // zero in memory (a merged coarse step writes its first sums onto zero
+    // instead of clearing after each sync, StepArgs::reg_fresh) and are
+    // cleared before anything else reads them (flush_registers).
+    bool reg_clean = true, reg_stale = false;
+    bool reg_fresh_run = false;    // the coarse steps being run or recorded work so
+    int flush_registers();
     long long regstride = 0;
     // tagging parameters: clawpatch refinement-criteria (HIPCLAW_TAG_*),
     // threshold variable and thresholds; the domain's refinement strategy
@@ -157,6 +167,7 @@ struct Ctx {
         unsigned parity_in, pushed_in;
         long long epoch;
         int nstep;
+        bool reg_fresh;
         hipGraphExec_t exec;
         unsigned parity_out, pushed_out;
         long long cells;

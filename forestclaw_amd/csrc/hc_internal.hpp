@@ -78,6 +78,9 @@ struct MultiLevel {
     const double* par;
     // optional atomic launch maximum of the patch Courant numbers (StepArgs::cflmax)
     unsigned long long* cflmax;
+    // time-sync registers known to be zero (StepArgs::reg_fresh)
+    int reg_fresh, reg_lo;
+    const int* reg_cfaces;
 };
 // Courant-number slots of a coarse step (hc_api.hip): the maximum of n
 // patch values stored into one slot as double bits; the maximum of n slots

@@ -24,6 +24,7 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
         hp.ts[L].clear(); hp.ts_same[L].clear(); hp.ts_f2c[L].clear(); hp.ts_phys[L].clear();
     }
     hp.has_fine.assign(nn, 0);
+    hp.coarse_faces.assign(nn, 0);
     for (int k = 0; k < nn; k++) {
         const Leaf& P = forest.leaves[k];
         const int L = P.level;
@@ -51,6 +52,7 @@ void build_host_plans(const Forest& forest, int mx, int my, const int mthbc[4], 
                 hp.ts_f2c[L].push_back(4 * k + f);
                 ts_any = true;
             } else if (kind == NB_COARSE) {
+                hp.coarse_faces[k] |= 1 << f;
                 if (L >= 1) hp.ts_f2c[L - 1].push_back(4 * k + f);
             } else if (kind == NB_BOUNDARY) {
                 hp.ts_phys[L].push_back(4 * k + f);

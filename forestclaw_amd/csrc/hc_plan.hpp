@@ -13,6 +13,7 @@ struct HostPlans {
     std::vector<BcOp> bc[MAXL];
     std::vector<TiOp> ti[MAXL];
     std::vector<int> has_fine;  // per leaf: a finer face or corner neighbour
+    std::vector<int> coarse_faces;  // per leaf: bit f set when face f has a coarser neighbour
     // time sync: corrections of level-L patches, and register faces (slot*4 +
     // face) reset after a sync that includes level L
     // (fclaw2d_clawpatch_time_sync_reset, fclaw2d_clawpatch_conservation.c:207-280):

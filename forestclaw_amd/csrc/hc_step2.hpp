@@ -77,6 +77,17 @@ struct StepArgs {
     // of every positive value (one per wave, or per patch with row bands);
     // the caller clears it beforehand
     unsigned long long* cflmax;
+    // SYNC kernels of a merged subcycled step (reg_fresh != 0): the registers
+    // are zero on entry, without having been cleared, except those of the
+    // faces of a level-reg_lo patch that have a coarser neighbour (bit f of
+    // reg_cfaces[slot]), which hold the patch's earlier substeps; only those
+    // are read, the others are written as sums onto zero.  In such a step
+    // every level's own faces are cleared by the sync that ends each of its
+    // steps, and the coarse-fine faces of level L by the sync of level L-1,
+    // which precedes L's first substep; reg_lo is the coarsest level of the
+    // launch, whose coarser neighbour level is between steps.
+    int reg_fresh, reg_lo;
+    const int* reg_cfaces;
     // Row bands (nband > 1, q_out != q_in): a workgroup of
     // nband waves takes the same ppw patches, wave b the interior rows
     // [2 + b*band_rows, 2 + (b+1)*band_rows) (clipped to my), sweeping from
@@ -255,6 +266,11 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     const R* __restrict__ qp = qbase + (long long)tix * A.qstride;
     R* __restrict__ qo = obase + (long long)tix * A.qstride;
     const R* __restrict__ ap = (S::MAUX > 0) ? aux + (long long)tix * A.astride : nullptr;
+    // SYNC: faces whose registers are read (bit f), the others being zero
+    int reg_rd = 15;
+    if constexpr (SYNC) {
+        if (A.reg_fresh) reg_rd = (pact && lev == A.reg_lo) ? A.reg_cfaces[tix] : 0;
+    }
 
     // PUSH: this patch's edge cells are staged in LDS during the sweep, per
     // patch 4*meqn*(mx+my) values: x edges [side][col][m][row], then y edges
@@ -605,11 +621,13 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 tso[k][4] = (yi && ((J == 3 && first_band) || (J == NY - 1 && last_band)))
                                 ? (int)reg_off(A.mx, A.my, ME, J == 3 ? 2 : 3, J == 3 ? 1 : 0) + (c - 2)
                                 : -1;
+                const int fsl[5] = {xf, xf, J <= 2 ? 2 : 3, Rw2 == 2 ? 2 : 3, J == 3 ? 2 : 3};
 #pragma unroll
                 for (int sl = 0; sl < 5; sl++) {
                     const int st = sl < 2 ? A.my : A.mx;  // element stride of the field m
+                    const bool rd = tso[k][sl] >= 0 && ((reg_rd >> fsl[sl]) & 1);
 #pragma unroll
-                    for (int m = 0; m < ME; m++) tsold[k][sl][m] = tso[k][sl] >= 0 ? rg[tso[k][sl] + m * st] : 0.0;
+                    for (int m = 0; m < ME; m++) tsold[k][sl][m] = rd ? rg[tso[k][sl] + m * st] : 0.0;
                 }
                 double qd[ME];
 #pragma unroll

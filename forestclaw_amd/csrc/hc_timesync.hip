@@ -15,6 +15,12 @@
 // workgroups.  Within a patch the order of the reference is kept per edge
 // cell: same-size faces 0..3, then finer faces 0..3 (igrid 0 then 1), every
 // addend applied in FP64 (a corner cell receives two faces' addends).
+//
+// In the merged-level driver the corrections run in the ghost fill's final
+// launch (k_bc_ts: each workgroup applies its patch's physical BCs first,
+// then the corrections), and the clears are left out: the step kernel writes
+// the first sums after a clear onto zero instead of reading the register
+// (StepArgs::reg_fresh).
 #include "hc_amr_launch.hpp"
 #include "hc_bc.hpp"
 #include "hc_common.hpp"
@@ -168,6 +174,19 @@ __global__ void k_ts_reset(const ResetSegs S, double* __restrict__ reg, long lon
         const int len = f < 2 ? my : mx;
         if (r < 4 * meqn * len) reg[(long long)slot * regstride + reg_off(mx, my, meqn, f, 0) + r] = 0.0;
     }
+}
+
+__global__ void k_zero(double* __restrict__ p, long long n)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = 0.0;
+}
+
+void launch_zero(double* p, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    const long long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)(nb < 65536 ? nb : 65536)), dim3(256), 0, s, p, n);
 }
 
 template <class R>

@@ -7,6 +7,9 @@
   subcycled coarse steps bitwise equal to the Python restatement of the
   drivers (oracle/amr.py) with its time-sync pass, and all registers cleared
   after a coarse step, as in the oracle.
+* Merged coarse steps, which leave out the register clears, against
+  per-level ones and through stand-alone update / sync calls: the same
+  registers and q (FP64 bitwise).
 * Conservation: with wall boundaries the leaf mass is constant to rounding
   in FP64 (1e-13 relative) with time sync and drifts without it; FP32 with
   time sync stays within FP32 rounding (1e-6).
@@ -120,3 +123,42 @@ def test_time_sync_conserves_mass_with_walls(dev, precision, tol):
     print(f"\nprecision {precision}: rel mass change with sync {dm[True]:.2e}, without {dm[False]:.2e}")
     assert dm[True] <= tol
     assert dm[False] >= 10 * tol
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_merged_steps_registers_match_per_level_steps(dev, precision):
+    """Merged coarse steps leave out the register clears after each sync (the
+    step writes its first sums after a clear onto zero; only the coarse-fine
+    faces of a level on its second substep are read).  Against per-level
+    coarse steps, which clear after every sync, and through stand-alone calls
+    that read the registers: the registers after a stand-alone update of the
+    finest level, and after a stand-alone sync, are bitwise the same, and so
+    is q after further coarse steps (which then clear, as they follow
+    stand-alone sums).  FP32: within 1e-6 relative L1 (the two drivers may
+    pick different launch shapes)."""
+    res = []
+    for merge in (True, False):
+        sim = _sim(precision=precision)
+        sim.set_feature(H.FEATURE_MERGE_LEVELS, merge)
+        info = sim.patch_info()
+        lo, hi = min(info["level"]), max(info["level"])
+        dt = 0.01
+        for s in range(2):
+            sim.advance(s * dt, dt)
+        r0 = sim.get_registers()
+        sim.update_level(hi, 2 * dt, dt / 4)
+        r1 = sim.get_registers()
+        sim.ghost_update(hi, hi)
+        sim.time_sync(hi, hi)
+        r2 = sim.get_registers()
+        for s in range(2, 4):
+            sim.advance(s * dt, dt)
+        res.append((r0, r1, r2, sim.get_registers(), sim.get_q()))
+        sim.close()
+    assert not res[0][0].any() and not res[1][0].any()
+    assert np.count_nonzero(res[1][1]) > 0
+    for a, b in zip(res[0], res[1]):
+        if precision == H.FP64:
+            assert np.array_equal(a, b)
+        else:
+            assert np.abs(a - b).sum() <= 1e-6 * np.abs(b).sum()
