@@ -316,6 +316,8 @@ def small_configs(args):
     from forestclaw_amd import configs as CF
     from forestclaw_amd import hipclaw as H
     lengths = {"C1": 200, "C2": 100, "C3": 30}
+    if getattr(args, "config", None):
+        lengths = {k: v for k, v in lengths.items() if k in args.config}
     chunk = 64
     if not _STREAM:
         _STREAM.append(torch.cuda.Stream())
@@ -494,6 +496,8 @@ def main():
     ap.add_argument("--configs", action="store_true",
                     help="only time configurations C1-C3, FP32 batched against FP64 plain "
                          "(one JSON line of its own)")
+    ap.add_argument("--config", action="append", choices=["C1", "C2", "C3"],
+                    help="with --configs: only this configuration (repeatable)")
     ap.add_argument("--advection-fixed", action="store_true",
                     help="with --advection: also time fixed velocities read from aux")
     ap.add_argument("--host-path", type=int, default=0, metavar="NPATCH",
