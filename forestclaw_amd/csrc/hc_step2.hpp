@@ -316,6 +316,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
     // advection: v of row J+1 (and of the left neighbour column), made by
     // row J and taken by row J+1 as its own (slot (J - J0) mod 2)
     R vnr[2][CPL], vlr[2][CPL];
+    // FP32 swirl: the LDS row factors a row takes (dB of its own row for u,
+    // B of the next row for v), read by the row before it (same ring slots),
+    // so that no row waits for an LDS read it has just issued
+    float ydb[2] = {0.0f, 0.0f}, yb1[2] = {0.0f, 0.0f};
     R fmr[4][CPL][ME], fpr[4][CPL][ME], Gr[4][CPL][ME];
     Face fyr[2][CPL] = {};
     Trans tyr[2][CPL] = {};
@@ -491,7 +495,7 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 // u of (c, J) and v of (c, J+1); v of (c, J) is what row J-1
                 // made as its v of the next row (or the prologue, for J0)
                 if (A.swirl && sizeof(R) == 4) {
-                    const float db = rowin ? swyf[NY + Jo] : 0.0f;
+                    const float db = rowin ? ydb[T1] : 0.0f;
                     ucur[k] = (cv[k] && rowin) ? R(swcu * (swa0[k] * db)) : R(0);
                 } else if (A.swirl) {
                     ucur[k] = R(0);
@@ -506,7 +510,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                     ucur[k] = (cv[k] && rowin) ? ap[(long long)Jo * NX + col[k]] : R(0);
                 }
                 vcur[k] = rowin ? vnr[T1][k] : R(0);
-                vnext[k] = J + 1 < NY ? vel_v(k, Jo + 1) : R(0);
+                if (A.swirl && sizeof(R) == 4)
+                    vnext[k] = (J + 1 < NY && cv[k]) ? R(swcv * (yb1[T1] * swda[k])) : R(0);
+                else
+                    vnext[k] = J + 1 < NY ? vel_v(k, Jo + 1) : R(0);
             } else {
                 ucur[k] = vcur[k] = vnext[k] = R(0);
             }
@@ -1144,6 +1151,14 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
             }
             tyn_w[k] = tyn[k];
         }
+        if constexpr (S::MAUX > 0 && !S::FW && sizeof(R) == 4) {
+            // factors of row J+1, issued after this row's stores
+            if (A.swirl) {
+                const int Jn = __builtin_amdgcn_readfirstlane(min(J + 1, NY - 1));
+                ydb[T0] = swyf[NY + Jn];
+                yb1[T0] = swyf[min(Jn + 1, NY - 1)];
+            }
+        }
     };
 
     // the advection velocity v of the sweep's first row, as its predecessor
@@ -1153,6 +1168,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
         for (int k = 0; k < CPL; k++) vnr[1][k] = vel_v(k, J0);
 #pragma unroll
         for (int k = 0; k < CPL; k++) vlr[1][k] = from_left<CPL>(vnr[1], k);
+        if (A.swirl && sizeof(R) == 4) {
+            ydb[1] = swyf[NY + J0];
+            yb1[1] = swyf[min(J0 + 1, NY - 1)];
+        }
     }
     if constexpr (UNR == 4) {
         if constexpr (PF) {
@@ -1180,6 +1199,10 @@ k_step2(const StepArgs A, const R* __restrict__ qin, R* __restrict__ qout,
                 ur[3][k] = ur[0][k];
                 vnr[1][k] = vnr[0][k];
                 vlr[1][k] = vlr[0][k];
+                if (k == 0) {
+                    ydb[1] = ydb[0];
+                    yb1[1] = yb1[0];
+                }
 #pragma unroll
                 for (int m = 0; m < ME; m++) {
                     qr[2][k][m] = qr[3][k][m];
