@@ -67,6 +67,15 @@ struct BcTsOp {
     int level;
 };
 
+// The Courant-number reduction that ends a coarse step (the maximum of the n
+// slots acc[0..n) into *out, slots cleared), run by one extra workgroup of the
+// coarse step's last ghost-fill launch; acc == nullptr: none.
+struct CflFin {
+    unsigned long long* acc;
+    int n;
+    double* out;
+};
+
 // Several operation lists in one launch: list s holds the operations
 // [start[s], start[s+1]) of the launch (one list per level of a ghost-fill
 // phase or a time sync, whose levels are independent of each other).

@@ -221,10 +221,13 @@ __global__ void k_ghost(const Segs<GhostOp> SG, const Segs<TiOp> ST, const TiWei
 // height, then bottom/top over the full width).
 // ---------------------------------------------------------------------------
 template <class R>
-__global__ void k_bc_list(const Segs<BcOp> S, Pools<R> P, int mx, int my, int meqn)
+__global__ void k_bc_list(const Segs<BcOp> S, Pools<R> P, int mx, int my, int meqn, CflFin cf)
 {
     const int o = blockIdx.x;
-    if (o >= S.total()) return;
+    if (o >= S.total()) {
+        if (o == S.total() && cf.acc) cfl_final_block(cf);
+        return;
+    }
     const BcOp op = S.at(o);
     const long long T = (long long)(mx + 2 * MBC) * (my + 2 * MBC);
     bc_patch<R>(op, pool_ptr(P, op.level, op.flags & 1) + (long long)op.slot * T * meqn, mx, my, meqn);
@@ -736,10 +739,18 @@ void launch_ghost(const GhostOp* ops, int nops, const Pools<R>& P, int mx, int m
 }
 
 template <class R>
+void launch_bc_cfl(const Segs<BcOp>& S, const Pools<R>& P, int mx, int my, int meqn, const CflFin& cf,
+                   hipStream_t s)
+{
+    const int nb = S.total() + (cf.acc ? 1 : 0);
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_bc_list<R>, dim3(nb), dim3(64), 0, s, S, P, mx, my, meqn, cf);
+}
+
+template <class R>
 void launch_bc(const Segs<BcOp>& S, const Pools<R>& P, int mx, int my, int meqn, hipStream_t s)
 {
-    if (S.total() <= 0) return;
-    hipLaunchKernelGGL(k_bc_list<R>, dim3(S.total()), dim3(64), 0, s, S, P, mx, my, meqn);
+    launch_bc_cfl<R>(S, P, mx, my, meqn, CflFin{nullptr, 0, nullptr}, s);
 }
 
 template <class R>
@@ -822,6 +833,8 @@ void launch_copy_tiles(const int* slots, int n, const R* src, R* dst, long long 
     template void launch_ghost_ti<R>(const Segs<GhostOp>&, const Segs<TiOp>&, const double*, const int*, \
                                      const Pools<R>&, int, int, int, hipStream_t);                     \
     template void launch_bc<R>(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);       \
+    template void launch_bc_cfl<R>(const Segs<BcOp>&, const Pools<R>&, int, int, int, const CflFin&,     \
+                                   hipStream_t);                                                      \
     template void launch_timeinterp<R>(const TiOp*, int, const Pools<R>&, int, int, int, double,     \
                                        hipStream_t);                                              \
     template void launch_xfer<R>(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t); \

@@ -32,6 +32,9 @@ template <class R>
 void launch_ghost_ti(const Segs<GhostOp>&, const Segs<TiOp>&, const double* alpha, const int* ti_levels,
                      const Pools<R>&, int, int, int, hipStream_t);
 template <class R> void launch_bc(const Segs<BcOp>&, const Pools<R>&, int, int, int, hipStream_t);
+// the same with the coarse step's Courant-number reduction as an extra workgroup
+template <class R>
+void launch_bc_cfl(const Segs<BcOp>&, const Pools<R>&, int, int, int, const CflFin&, hipStream_t);
 template <class R>
 void launch_timeinterp(const TiOp*, int, const Pools<R>&, int, int, int, double, hipStream_t);
 template <class R> void launch_xfer(const XferOp*, int, const Pools<R>&, R*, int, int, int, hipStream_t);
@@ -62,7 +65,7 @@ void launch_zero(double* p, long long n, hipStream_t s);
 // physical BCs, then time-sync corrections, one patch per workgroup
 template <class R>
 void launch_bc_ts(const Segs<BcTsOp>&, const Pools<R>&, const double* reg, long long regstride,
-                  const double* dxdy, int, int, int, hipStream_t);
+                  const double* dxdy, int, int, int, const CflFin&, hipStream_t);
 // Network input tiles of n patches (bf16 [n][36][36][8]); returns HIPCLAW_*.
 template <class R>
 int launch_make_tiles(const int*, const int*, int, const Pools<R>&, const double*, int, int, int, double,

@@ -42,4 +42,25 @@ __device__ inline void bc_patch(const BcOp& op, R* __restrict__ qp, int mx, int 
 #undef QB
 }
 
+// CflFin's reduction by one workgroup (blockDim a power of two, at most 256):
+// the slots hold the bits of non-negative doubles, which order like the
+// doubles, so the integer maximum is the maximum Courant number.
+__device__ inline void cfl_final_block(const CflFin& c)
+{
+    __shared__ unsigned long long s[256];
+    unsigned long long v = 0;
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x) {
+        const unsigned long long a = c.acc[i];
+        v = a > v ? a : v;
+        c.acc[i] = 0;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w && s[threadIdx.x + w] > s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *c.out = __longlong_as_double((long long)s[0]);
+}
+
 }  // namespace hc

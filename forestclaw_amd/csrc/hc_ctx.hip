@@ -372,14 +372,16 @@ int Ctx::rebuild(bool keep_data, const std::vector<Transfer>* xfer)
 // The physical BC pass on the fine levels that precedes the final pass in
 // the reference is subsumed by it (bc2 is idempotent on unchanged data).
 // ---------------------------------------------------------------------------
-int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp, bool sync)
+int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp, bool sync, double* cfl_out)
 {
     if (par_dry) {
         ti_pending.clear();
         return HIPCLAW_OK;
     }
     if (sync && !(merge_levels && tsync)) {
-        HC_TRY(ghost_update(minlevel, maxlevel, time_interp, false));
+        // a sync after the fill reads q: the reduction (slots only) may
+        // still ride on the fill's last launch
+        HC_TRY(ghost_update(minlevel, maxlevel, time_interp, false, cfl_out));
         return time_sync(minlevel, maxlevel);
     }
     const int mx = phys.mx, my = phys.my, meqn = phys.meqn;
@@ -452,15 +454,17 @@ int Ctx::ghost_update(int minlevel, int maxlevel, int time_interp, bool sync)
             for (int L = maxcoarse; L >= mincoarse; L--)
                 launch_ghost<R>(plan[L].interp.p, plan[L].interp.n, P, mx, my, meqn, stream);
             if (use_ti) launch_ghost<R>(plan[tl].interp_ti.p, plan[tl].interp_ti.n, P, mx, my, meqn, stream);
+            const CflFin cf{cfl_out ? d_cflslot : nullptr, cflacc_count < 4095 ? cflacc_count : 4095, cfl_out};
+            if (cfl_out) cfl_done = true;
             if (sync) {
-                launch_bc_ts<R>(bct, P, d_reg, regstride, d_dxdy, mx, my, meqn, stream);
+                launch_bc_ts<R>(bct, P, d_reg, regstride, d_dxdy, mx, my, meqn, cf, stream);
                 clear_pushed();
                 // a merged coarse step's updates write their first sums onto
                 // zero instead (StepArgs::reg_fresh)
                 if (!reg_fresh_run) launch_ts_reset(rs, d_reg, regstride, mx, my, meqn, stream);
                 return check_hip(hipGetLastError(), "ghost update and time sync");
             }
-            launch_bc<R>(bca, P, mx, my, meqn, stream);
+            launch_bc_cfl<R>(bca, P, mx, my, meqn, cf, stream);
             return check_hip(hipGetLastError(), "ghost update");
         }
         for (int L = maxcoarse; L >= mincoarse; L--)
@@ -896,8 +900,9 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
     auto steps = [&]() -> int {
         double tk = t;
         for (int k = 0; k < nstep; k++) {
-            HC_TRY(advance_body(tk, dt));
-            if (!par_dry) HC_TRY(cfl_final(d_cflslot, cflacc_count, cfl_dev + k, stream));
+            cfl_done = false;
+            HC_TRY(advance_body(tk, dt, cfl_dev + k));
+            if (!par_dry && !cfl_done) HC_TRY(cfl_final(d_cflslot, cflacc_count, cfl_dev + k, stream));
             tk += dt;
         }
         return HIPCLAW_OK;
@@ -1024,7 +1029,7 @@ int Ctx::advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare)
     return HIPCLAW_OK;
 }
 
-int Ctx::advance_body(double t, double dt)
+int Ctx::advance_body(double t, double dt, double* cfl_out)
 {
     const int lo = forest.global_minlevel(), hi = forest.global_maxlevel();
     struct Cnt { int last_step, step_inc, total_steps; double t, dt_step; } c[MAXL];
@@ -1100,7 +1105,7 @@ int Ctx::advance_body(double t, double dt)
             }
         }
     }
-    return ghost_update(lo, hi, 0, true);
+    return ghost_update(lo, hi, 0, true, cfl_out);
 }
 
 // Copies of every level's current tiles (interiors and ghost cells) into the

@@ -122,8 +122,12 @@ struct Ctx {
     void release_pools();
     int rebuild(bool keep_data, const std::vector<Transfer>* xfer);
     template <class R> Pools<R> pools() const;
-    // sync: the time sync of the same levels follows (time_sync)
-    int ghost_update(int minlevel, int maxlevel, int time_interp, bool sync = false);
+    // sync: the time sync of the same levels follows (time_sync).  cfl_out:
+    // the fill ends a coarse step, whose Courant-number reduction (cfl_final
+    // of the step's slots into cfl_out) the merged path runs in its last
+    // launch, setting cfl_done
+    int ghost_update(int minlevel, int maxlevel, int time_interp, bool sync = false, double* cfl_out = nullptr);
+    bool cfl_done = false;
     int update_level(int level, double t, double dt);
     Src2Args src2_args(int s0, int np) const;
     bool fused_swirl() const;
@@ -157,7 +161,7 @@ struct Ctx {
     // and leave the state as it was, running nothing
     int advance_n(double t, double dt, int nstep, double* maxcfl, bool prepare = false);
     static constexpr int ADVANCE_MAXSTEPS = 256;
-    int advance_body(double t, double dt);
+    int advance_body(double t, double dt, double* cfl_out = nullptr);
     // hipGraph of a whole coarse step, replayed when dt, the level pool
     // parities, the ghost-push state, the launch shape and the mesh match
     bool use_graph = false;

@@ -146,11 +146,14 @@ k_ts_correct(const Segs<TsOp> S, const QList<R> qs, const double* __restrict__ r
 template <class R>
 __global__ void __launch_bounds__(TS_THREADS)
 k_bc_ts(const Segs<BcTsOp> S, Pools<R> P, const double* __restrict__ reg, long long regstride,
-        const double* __restrict__ dxdy, int mx, int my, int meqn)
+        const double* __restrict__ dxdy, int mx, int my, int meqn, CflFin cf)
 {
     extern __shared__ double add[];
     const int op = blockIdx.x;
-    if (op >= S.total()) return;
+    if (op >= S.total()) {
+        if (op == S.total() && cf.acc) cfl_final_block(cf);
+        return;
+    }
     const BcTsOp o = S.at(op);
     const long long T = (long long)(mx + 4) * (my + 4) * meqn;
     if (o.bc.slot >= 0) {
@@ -203,12 +206,13 @@ void launch_ts_correct(const Segs<TsOp>& S, R* const* qs, const double* reg, lon
 
 template <class R>
 void launch_bc_ts(const Segs<BcTsOp>& S, const Pools<R>& P, const double* reg, long long regstride,
-                  const double* dxdy, int mx, int my, int meqn, hipStream_t s)
+                  const double* dxdy, int mx, int my, int meqn, const CflFin& cf, hipStream_t s)
 {
-    if (S.total() <= 0) return;
+    const int nb = S.total() + (cf.acc ? 1 : 0);
+    if (nb <= 0) return;
     const size_t lds = sizeof(double) * 8 * meqn * (mx > my ? mx : my);
-    hipLaunchKernelGGL(k_bc_ts<R>, dim3(S.total()), dim3(TS_THREADS), lds, s, S, P, reg, regstride, dxdy, mx,
-                       my, meqn);
+    hipLaunchKernelGGL(k_bc_ts<R>, dim3(nb), dim3(TS_THREADS), lds, s, S, P, reg, regstride, dxdy, mx, my,
+                       meqn, cf);
 }
 
 template <class R>
@@ -239,9 +243,9 @@ void launch_ts_reset(const int* faces, int n, double* reg, long long regstride, 
 }
 
 template void launch_bc_ts<double>(const Segs<BcTsOp>&, const Pools<double>&, const double*, long long,
-                                   const double*, int, int, int, hipStream_t);
+                                   const double*, int, int, int, const CflFin&, hipStream_t);
 template void launch_bc_ts<float>(const Segs<BcTsOp>&, const Pools<float>&, const double*, long long,
-                                  const double*, int, int, int, hipStream_t);
+                                  const double*, int, int, int, const CflFin&, hipStream_t);
 template void launch_ts_correct<double>(const Segs<TsOp>&, double* const*, const double*, long long,
                                         const double*, int, int, int, hipStream_t);
 template void launch_ts_correct<float>(const Segs<TsOp>&, float* const*, const double*, long long,
